@@ -1,0 +1,132 @@
+#!/usr/bin/env python3
+"""Throughput of the geodesic tracer on one MI355X.
+
+A step is one full frame of the hot path -- trace, sky shading,
+accumulation, bloom and tonemap through launchRenderPipeline -- on the
+Kerr a*=0.9 1920x1080 'high' workload, cycling the disk time over
+t = 0, 0.5, 1.  Inputs are device resident; timing brackets exactly
+`--steps` frames with device synchronizations, after `--warmup` frames.
+
+Prints one JSON line.  `roofline` is the trace kernel's achieved fraction of
+the FP32 vector peak: exact algorithmic flops of each frame (counted-float
+oracle, see FLOPS below) over the trace kernel's HIP-event time on the
+launch stream.  `cpu_baseline` is the CPU oracle (same algorithm, g++ -O3,
+OpenMP) on a bounded sample of the same frame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "black-hole_amd"))
+
+METRIC = "Mrays/s Kerr a*=0.9 1080p: plain FP32 RK4 vs MLP+fallback at <=1e-3 rad"
+WORKLOAD = "kerr09_1080p_high"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+# Exact algorithmic flops per frame of WORKLOAD (trace kernel: camera ray,
+# integration, disk radiative transfer), from
+#   python oracle/count_flops.py kerr09_1080p_high 0 1 2
+# (counted-float instantiation of the oracle; 1 flop per +,-,*,/,sqrt,
+# floor, transcendental).  Keyed by frame index (disk time 0, 0.5, 1).
+FLOPS = {0: 506225668732 + 80870400, 1: 506225457204 + 80870400, 2: 506228100901 + 80870400}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=9)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-row-step", type=int, default=4)
+    ap.add_argument("--math", type=int, default=None, help="0 exact, 1 fast (default: library default)")
+    return ap.parse_args()
+
+
+def cpu_baseline(P, row_step):
+    """CPU oracle (float, reference semantics) on every `row_step`-th row."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    O = pyoracle.Oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    t = time.perf_counter()
+    hdr, _, _, _ = O.trace_frame(P, 0, 0, row_step, threads)
+    dt = time.perf_counter() - t
+    rays = hdr.shape[0] * hdr.shape[1]
+    return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{WORKLOAD} t=0, every {row_step}th row ({rays} rays, {dt:.1f} s wall)"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(pixels shard across GPUs by rows, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import bhtrace as bt
+
+    lib = bt.load()
+    if lib.bhDeviceCount() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    if a.math is not None:
+        bt.set_schedule(math=a.math)
+    sched = bt.get_schedule()
+    frames = [bt.workload(WORKLOAD, f) for f in range(3)]
+    W, H = frames[0].width, frames[0].height
+    fb = bt.Framebuffer(W, H)
+    for i in range(a.warmup):
+        fb.render(frames[i % 3])
+    bt.synchronize()
+
+    import ctypes as C
+
+    lib.bhEnableKernelTimers(1)
+    tr, sh, po = C.c_float(), C.c_float(), C.c_float()
+    trace_ms, shade_ms, post_ms, per_frame = [], [], [], []
+    bt.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        f = i % 3
+        fb.render(frames[f])
+        bt.check(lib.bhLastKernelTimes(C.byref(tr), C.byref(sh), C.byref(po)), "bhLastKernelTimes")
+        trace_ms.append(tr.value)
+        shade_ms.append(sh.value)
+        post_ms.append(po.value)
+        per_frame.append(f)
+    bt.synchronize()
+    total = time.perf_counter() - t0
+    lib.bhEnableKernelTimers(0)
+
+    ms_per_step = total * 1e3 / a.steps
+    mrays = W * H * a.steps / total / 1e6
+    avg_trace = sum(trace_ms) / len(trace_ms)
+    roof = None
+    if FLOPS:
+        fl = sum(FLOPS[f] for f in per_frame) / len(per_frame)
+        achieved = fl / (avg_trace * 1e-3) / 1e12
+        roof = {"bound": "valu_fp32", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "traceKernel", "flops_per_frame": int(fl), "trace_ms": round(avg_trace, 3)}
+    res = {
+        "metric": METRIC, "value": round(mrays, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": WORKLOAD, "width": W, "height": H, "disk_times": [0.0, 0.5, 1.0],
+                   "path": "plain FP32 RK4 (no surrogate)", "math_mode": "fast" if sched.mathMode else "exact",
+                   "wg_size": sched.wgSize, "steps_per_chunk": sched.stepsPerChunk},
+        "kernel_ms": {"trace": round(avg_trace, 3), "shade": round(sum(shade_ms) / len(shade_ms), 3),
+                      "bloom_tonemap": round(sum(post_ms) / len(post_ms), 3)},
+        "roofline": roof,
+    }
+    if not a.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(frames[0], a.cpu_row_step)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

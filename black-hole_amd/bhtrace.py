@@ -1,0 +1,327 @@
+"""ctypes binding of libbhtrace.so, the gfx950 geodesic tracer's C ABI.
+
+Python is plumbing here: buffers are device allocations made through the
+library, every kernel runs through the C entry points declared in
+include/bh/bhtrace.h.  Loading fails loudly when the library is missing.
+
+    import bhtrace as bt
+    lib = bt.load()
+    P = bt.workload("kerr09_1080p_high", 0)
+    fb = bt.Framebuffer(P.width, P.height)
+    fb.render(P); rgba, hdr = fb.readback()
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libbhtrace.so")
+
+
+class F3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    """Mirror of RenderParams (include/bh/render_params.h; reference render_params.h:21-87)."""
+
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("swapRB", C.c_int),
+        ("camPos", F3), ("camForward", F3), ("camRight", F3), ("camUp", F3),
+        ("tanHalfFov", C.c_float), ("aspect", C.c_float),
+        ("model", C.c_int), ("M", C.c_float), ("aSpin", C.c_float), ("Qc", C.c_float),
+        ("rPlus", C.c_float), ("rPhoton", C.c_float), ("rErgo", C.c_float), ("allowNaked", C.c_int),
+        ("dPhi", C.c_float), ("maxSteps", C.c_int),
+        ("diskEnabled", C.c_int), ("diskInner", C.c_float), ("diskOuter", C.c_float), ("diskTime", C.c_float),
+        ("diskEmisScale", C.c_float), ("diskAbsScale", C.c_float),
+        ("hotSpotsEnabled", C.c_int), ("hotSpotStrength", C.c_float),
+        ("exposure", C.c_float), ("sampleIndex", C.c_int), ("accumMode", C.c_int),
+        ("bloomEnabled", C.c_int), ("bloomStrength", C.c_float),
+    ]
+
+    def copy(self) -> "RenderParams":
+        c = RenderParams()
+        C.memmove(C.byref(c), C.byref(self), C.sizeof(RenderParams))
+        return c
+
+
+assert C.sizeof(RenderParams) == 160
+
+
+class BHDerived(C.Structure):
+    _fields_ = [("a", C.c_float), ("Q", C.c_float), ("rPlus", C.c_float), ("rMinus", C.c_float),
+                ("rPhoton", C.c_float), ("rErgo", C.c_float), ("rIsco", C.c_float),
+                ("clamped", C.c_int), ("naked", C.c_int)]
+
+
+class BhProbe(C.Structure):
+    _fields_ = [("outcome", C.c_int), ("escDir", C.c_float * 3), ("maxHviol", C.c_float),
+                ("lum", C.c_float), ("steps", C.c_int), ("trans", C.c_float)]
+
+
+class BhSchedule(C.Structure):
+    _fields_ = [("wgSize", C.c_int), ("stepsPerChunk", C.c_int), ("raysPerLane", C.c_int),
+                ("mathMode", C.c_int)]
+
+
+PROBE_DTYPE = np.dtype([("outcome", "<i4"), ("escDir", "<f4", (3,)), ("maxHviol", "<f4"),
+                        ("lum", "<f4"), ("steps", "<i4"), ("trans", "<f4")])
+RECORD_DTYPE = np.dtype([("escDir", "<f4", (3,)), ("flags", "<i4"), ("accum", "<f4", (3,)),
+                         ("trans", "<f4")])
+
+_lib = None
+
+# (name, restype, argtypes) of every exported symbol declared in bh/bhtrace.h
+_P = C.POINTER(RenderParams)
+_vp = C.c_void_p
+SIGNATURES = [
+    ("launchRenderPipeline", C.c_int, [_vp, _vp, _vp, _vp, _P, _vp]),
+    ("launchRenderKernel", C.c_int, [_vp, _P, _vp]),
+    ("bhDefaultRenderParams", None, [_P]),
+    ("bhSanitizeRenderParams", C.c_int, [_P]),
+    ("bhValidateAndDerive", None, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), C.c_int, C.POINTER(BHDerived)]),
+    ("bhHorizonOuterFP64", C.c_double, [C.c_double, C.c_double, C.c_double]),
+    ("bhIscoFP64", C.c_double, [C.c_double, C.c_double, C.c_double]),
+    ("bhFillCamera", None, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
+    ("bhWorkloadParams", C.c_int, [C.c_char_p, C.c_int, _P]),
+    ("bhProbeGrid", C.c_int, [_vp, _P, C.c_int, C.c_int, _vp]),
+    ("bhProbeDirs", C.c_int, [_vp, _vp, C.c_int, _P, _vp]),
+    ("bhTraceRecords", C.c_int, [_vp, _P, _vp]),
+    ("bhFramebufferCreate", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("bhFramebufferDestroy", None, [_vp]),
+    ("bhFramebufferColor", _vp, [_vp]),
+    ("bhFramebufferHDR", _vp, [_vp]),
+    ("bhFramebufferRender", C.c_int, [_vp, _P, _vp]),
+    ("bhFramebufferReadback", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("bhGetSchedule", None, [C.POINTER(BhSchedule)]),
+    ("bhSetSchedule", C.c_int, [C.POINTER(BhSchedule)]),
+    ("bhEnableKernelTimers", None, [C.c_int]),
+    ("bhLastKernelTimes", C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("bhDeviceAlloc", C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    ("bhDeviceFree", C.c_int, [_vp]),
+    ("bhCopyToDevice", C.c_int, [_vp, _vp, C.c_size_t]),
+    ("bhCopyToHost", C.c_int, [_vp, _vp, C.c_size_t]),
+    ("bhDeviceFill", C.c_int, [_vp, C.c_ubyte, C.c_size_t]),
+    ("bhDeviceSynchronize", C.c_int, []),
+    ("bhStreamCreate", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("bhStreamDestroy", C.c_int, [_vp]),
+    ("bhStreamSynchronize", C.c_int, [_vp]),
+    ("bhErrorString", C.c_char_p, [C.c_int]),
+    ("bhDeviceCount", C.c_int, []),
+]
+
+
+class BhError(RuntimeError):
+    pass
+
+
+def load(path: str | None = None):
+    """Load libbhtrace.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise BhError(f"libbhtrace.so not built: {p} missing (run __graft_entry__.build())")
+    lib = C.CDLL(p)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(err: int, what: str = "") -> None:
+    if err != 0:
+        msg = load().bhErrorString(err)
+        raise BhError(f"{what}: HIP error {err} ({msg.decode() if msg else '?'})")
+
+
+# ------------------------------------------------------------------ parameters
+def workload(name: str, frame: int = 0) -> RenderParams:
+    P = RenderParams()
+    if load().bhWorkloadParams(name.encode(), frame, C.byref(P)) != 0:
+        raise ValueError(f"unknown workload {name!r} frame {frame}")
+    return P
+
+
+def defaults() -> RenderParams:
+    P = RenderParams()
+    load().bhDefaultRenderParams(C.byref(P))
+    return P
+
+
+def sanitized(P: RenderParams) -> tuple[RenderParams, bool]:
+    Q = P.copy()
+    fixed = load().bhSanitizeRenderParams(C.byref(Q))
+    return Q, bool(fixed)
+
+
+def validate_and_derive(model: int, M: float, a_star: float, q: float, allow_naked: bool = False):
+    m, a, qq = C.c_float(M), C.c_float(a_star), C.c_float(q)
+    d = BHDerived()
+    load().bhValidateAndDerive(model, C.byref(m), C.byref(a), C.byref(qq), int(allow_naked), C.byref(d))
+    return d, m.value, a.value, qq.value
+
+
+def fill_camera(P: RenderParams, az: float, el: float, dist: float, fov_deg: float) -> None:
+    load().bhFillCamera(C.byref(P), az, el, dist, fov_deg)
+
+
+# ------------------------------------------------------------------ device memory
+class DeviceBuffer:
+    """A device allocation owned by Python (freed on close/GC)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(load().bhDeviceAlloc(C.byref(p), max(self.nbytes, 1)), "bhDeviceAlloc")
+        self.ptr = p
+
+    def upload(self, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        check(load().bhCopyToDevice(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes), "upload")
+
+    def download(self, dtype, count=None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(n, dtype=dt)
+        check(load().bhCopyToHost(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes), "download")
+        return out
+
+    def fill(self, value: int) -> None:
+        check(load().bhDeviceFill(self.ptr, value, self.nbytes), "fill")
+
+    def close(self) -> None:
+        if self.ptr:
+            load().bhDeviceFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    check(load().bhDeviceSynchronize(), "bhDeviceSynchronize")
+
+
+class Framebuffer:
+    """Headless device framebuffer (uchar4 colour + float4 HDR accumulation)."""
+
+    def __init__(self, w: int, h: int):
+        self.w, self.h = w, h
+        fb = C.c_void_p()
+        check(load().bhFramebufferCreate(w, h, C.byref(fb)), "bhFramebufferCreate")
+        self.fb = fb
+
+    def render(self, P: RenderParams, stream=None) -> None:
+        check(load().bhFramebufferRender(self.fb, C.byref(P), stream), "bhFramebufferRender")
+
+    def readback(self, stream=None):
+        rgba = np.empty((self.h, self.w, 4), np.uint8)
+        hdr = np.empty((self.h, self.w, 4), np.float32)
+        check(load().bhFramebufferReadback(self.fb, rgba.ctypes.data_as(C.c_void_p),
+                                           hdr.ctypes.data_as(C.c_void_p), stream), "readback")
+        return rgba, hdr
+
+    def close(self) -> None:
+        if self.fb:
+            load().bhFramebufferDestroy(self.fb)
+            self.fb = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ convenience wrappers
+def render_pipeline(P: RenderParams, accum_init: np.ndarray | None = None, frames: int = 1):
+    """Run launchRenderPipeline `frames` times on fresh device buffers; returns
+    (rgba uint8 [H,W,4], accum float32 [H,W,4])."""
+    lib = load()
+    W, H = P.width, P.height
+    bw, bh = (W + 1) // 2, (H + 1) // 2
+    out = DeviceBuffer(W * H * 4)
+    acc = DeviceBuffer(W * H * 16)
+    bA = DeviceBuffer(bw * bh * 16)
+    bB = DeviceBuffer(bw * bh * 16)
+    if accum_init is None:
+        acc.fill(0)
+    else:
+        acc.upload(np.ascontiguousarray(accum_init, np.float32))
+    for _ in range(frames):
+        check(lib.launchRenderPipeline(out.ptr, acc.ptr, bA.ptr, bB.ptr, C.byref(P), None), "launchRenderPipeline")
+    synchronize()
+    rgba = out.download(np.uint8).reshape(H, W, 4)
+    accum = acc.download(np.float32).reshape(H, W, 4)
+    return rgba, accum
+
+
+def render_kernel(P: RenderParams) -> np.ndarray:
+    lib = load()
+    W, H = max(P.width, 1), max(P.height, 1)
+    Q, _ = sanitized(P)
+    out = DeviceBuffer(Q.width * Q.height * 4)
+    check(lib.launchRenderKernel(out.ptr, C.byref(P), None), "launchRenderKernel")
+    synchronize()
+    return out.download(np.uint8).reshape(Q.height, Q.width, 4)
+
+
+def trace_records(P: RenderParams) -> np.ndarray:
+    lib = load()
+    Q, _ = sanitized(P)
+    buf = DeviceBuffer(Q.width * Q.height * RECORD_DTYPE.itemsize)
+    check(lib.bhTraceRecords(buf.ptr, C.byref(P), None), "bhTraceRecords")
+    synchronize()
+    return buf.download(RECORD_DTYPE).reshape(Q.height, Q.width)
+
+
+def probe_grid(P: RenderParams, nx: int, ny: int) -> np.ndarray:
+    lib = load()
+    buf = DeviceBuffer(nx * ny * PROBE_DTYPE.itemsize)
+    check(lib.bhProbeGrid(buf.ptr, C.byref(P), nx, ny, None), "bhProbeGrid")
+    synchronize()
+    return buf.download(PROBE_DTYPE).reshape(ny, nx)
+
+
+def probe_dirs(P: RenderParams, dirs: np.ndarray) -> np.ndarray:
+    lib = load()
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    dbuf = DeviceBuffer(d.nbytes)
+    dbuf.upload(d)
+    buf = DeviceBuffer(len(d) * PROBE_DTYPE.itemsize)
+    check(lib.bhProbeDirs(buf.ptr, dbuf.ptr, len(d), C.byref(P), None), "bhProbeDirs")
+    synchronize()
+    return buf.download(PROBE_DTYPE)
+
+
+def get_schedule() -> BhSchedule:
+    s = BhSchedule()
+    load().bhGetSchedule(C.byref(s))
+    return s
+
+
+def set_schedule(wg: int | None = None, chunk: int | None = None, math: int | None = None,
+                 rays_per_lane: int | None = None) -> BhSchedule:
+    s = get_schedule()
+    if wg is not None:
+        s.wgSize = wg
+    if chunk is not None:
+        s.stepsPerChunk = chunk
+    if math is not None:
+        s.mathMode = math
+    if rays_per_lane is not None:
+        s.raysPerLane = rays_per_lane
+    check(load().bhSetSchedule(C.byref(s)), "bhSetSchedule")
+    return s

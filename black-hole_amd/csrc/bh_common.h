@@ -1,0 +1,491 @@
+// ---------------------------------------------------------------------------
+// bh_common.h -- device-side building blocks of the gfx950 geodesic tracer.
+//
+// Physics and shading follow the reference's trace.cuh / metric.cuh /
+// vec_math.cuh / post_process.cuh (file:line cited per function).  The
+// arithmetic goes through a math policy:
+//
+//   MathExact -- IEEE division and square root, OCML transcendentals; with
+//                -ffp-contract=off the float results follow the reference's
+//                host semantics operation for operation.
+//   MathFast  -- hardware reciprocal / rsq / sin / cos / exp2 / log2 (about
+//                1 ulp, like the reference's own --use_fast_math GPU build);
+//                compiled with FMA contraction.
+//
+// Everything whose output feeds a hash (lattice coordinates, the hash
+// arithmetic itself, the dither) is evaluated with contraction disabled in
+// both policies: a 1-ulp change there reshuffles the noise or the stars.
+// ---------------------------------------------------------------------------
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/bh/render_params.h"
+
+namespace bhk {
+
+#define BHK_INL __device__ __forceinline__
+
+// ------------------------------------------------------------------ policies
+struct MathExact
+{
+    static constexpr bool fast = false;
+    BHK_INL static float div(float a, float b) { return a / b; }
+    BHK_INL static float rcp(float b) { return 1.0f / b; }
+    BHK_INL static float sqrt_(float x) { return sqrtf(x); }
+    BHK_INL static float rsq(float x) { return 1.0f / sqrtf(x); }
+    BHK_INL static float sin_(float x) { return sinf(x); }
+    BHK_INL static float cos_(float x) { return cosf(x); }
+    BHK_INL static float exp_(float x) { return expf(x); }
+    BHK_INL static float log_(float x) { return logf(x); }
+    BHK_INL static float pow_(float x, float y) { return powf(x, y); }
+    BHK_INL static float atan2_(float y, float x) { return atan2f(y, x); }
+    BHK_INL static float acos_(float x) { return acosf(x); }
+};
+
+struct MathFast
+{
+    static constexpr bool fast = true;
+    BHK_INL static float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
+    BHK_INL static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    BHK_INL static float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+    BHK_INL static float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+    BHK_INL static float sin_(float x) { return __sinf(x); }
+    BHK_INL static float cos_(float x) { return __cosf(x); }
+    BHK_INL static float exp_(float x) { return __expf(x); }
+    BHK_INL static float log_(float x) { return __logf(x); }
+    BHK_INL static float pow_(float x, float y) { return __powf(x, y); }
+    BHK_INL static float atan2_(float y, float x) { return atan2f(y, x); }
+    BHK_INL static float acos_(float x) { return acosf(x); }
+};
+
+// ------------------------------------------------------------------ vectors (vec_math.cuh:12-53)
+struct v3 { float x, y, z; };
+struct v2 { float x, y; };
+BHK_INL v3 mk3(float x, float y, float z) { return v3{x, y, z}; }
+BHK_INL v2 mk2(float x, float y) { return v2{x, y}; }
+BHK_INL v3 operator+(v3 a, v3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+BHK_INL v3 operator-(v3 a, v3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+BHK_INL v3 operator*(v3 a, v3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+BHK_INL v3 operator*(v3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+BHK_INL v3& operator+=(v3& a, v3 b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+BHK_INL v2 operator+(v2 a, v2 b) { return mk2(a.x + b.x, a.y + b.y); }
+BHK_INL float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class M> BHK_INL float len3(v3 a) { return M::sqrt_(dot(a, a)); }
+template <class M> BHK_INL v3 normalize(v3 a) { return a * M::rsq(fmaxf(dot(a, a), 1e-20f)); }
+template <class M> BHK_INL v3 vdiv(v3 a, float s) { return a * M::rcp(s); }
+BHK_INL v3 lerp3(v3 a, v3 b, float t) { return a + (b - a) * t; }
+BHK_INL float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+BHK_INL float fract(float x) { return x - floorf(x); }
+BHK_INL float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+template <class M> BHK_INL float smoothstepf(float e0, float e1, float x)
+{
+    float t = clampf(M::div(x - e0, e1 - e0), 0.f, 1.f);
+    return t * t * (3.f - 2.f * t);
+}
+template <class M> BHK_INL v3 pow3(v3 v, float e) { return mk3(M::pow_(v.x, e), M::pow_(v.y, e), M::pow_(v.z, e)); }
+BHK_INL bool finite3(v3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+
+// ------------------------------------------------------------------ hashes (vec_math.cuh:56-98)
+// "Hash without Sine" (D. Hoskins) as the reference evaluates it.
+BHK_INL float hash12(v2 p)
+{
+#pragma clang fp contract(off)
+    float ax = fract(p.x * 0.1031f), ay = fract(p.y * 0.1031f);
+    float d = ax * (ay + 33.33f) + ay * (ax + 33.33f) + ax * (ax + 33.33f);
+    return fract(((ax + d) + (ay + d)) * (ax + d));
+}
+BHK_INL v3 hash33(v3 p)
+{
+#pragma clang fp contract(off)
+    float qx = fract(p.x * 0.1031f), qy = fract(p.y * 0.1030f), qz = fract(p.z * 0.0973f);
+    float d = qx * (qy + 33.33f) + qy * (qx + 33.33f) + qz * (qz + 33.33f);
+    qx = qx + d; qy = qy + d; qz = qz + d;
+    return mk3(fract((qx + qy) * qz), fract((qx + qz) * qy), fract((qy + qz) * qx));
+}
+// 2-D value noise on the integer lattice with a cubic fade.
+BHK_INL float vnoise(v2 p)
+{
+    float ix = floorf(p.x), iy = floorf(p.y);
+    float fx = p.x - ix, fy = p.y - iy;
+    float ux = fx * fx * (3.f - 2.f * fx), uy = fy * fy * (3.f - 2.f * fy);
+    float a = hash12(mk2(ix, iy));
+    float b = hash12(mk2(ix + 1.f, iy + 0.f));
+    float c = hash12(mk2(ix + 0.f, iy + 1.f));
+    float d = hash12(mk2(ix + 1.f, iy + 1.f));
+    return lerpf(lerpf(a, b, ux), lerpf(c, d, ux), uy);
+}
+BHK_INL float fbm(v2 p)
+{
+    float v = 0.f, amp = 0.5f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        v += amp * vnoise(p);
+        p = mk2(p.x * 2.03f + 11.7f, p.y * 2.01f - 5.2f);
+        amp *= 0.5f;
+    }
+    return v;
+}
+
+// ------------------------------------------------------------------ constants (metric.cuh:53-58, trace.cuh:66-70)
+constexpr float kPi = 3.14159265358979f;
+constexpr float kTInner = 13500.f;
+constexpr float kNtPeakInv = 1.f / 0.075f;
+constexpr float kSingCutOverM = 0.05f;
+
+// ------------------------------------------------------------------ metric (metric.cuh:63-199)
+template <class M> BHK_INL float omegaCircular(float r, float Mm, float a, float Q)
+{
+    float w2 = Mm * r - Q * Q;
+    if (w2 <= 0.f) return 0.f;
+    float sw = M::sqrt_(w2);
+    return M::div(sw, r * r + a * sw);
+}
+// g = 1/[u^t (E - Omega Lz)] of a circular equatorial emitter (metric.cuh:154-185)
+template <class M> BHK_INL float emitterG(float r, float Mm, float a, float Q, float E, float Lz, float Om)
+{
+    float r2 = r * r;
+    float m2r = 2.f * Mm * r - Q * Q;
+    float gtt = -(1.f - M::div(m2r, r2));
+    float gtp = M::div(-a * m2r, r2);
+    float gpp = r2 + a * a + M::div(m2r * a * a, r2);
+    float den = -(gtt + 2.f * Om * gtp + Om * Om * gpp);
+    if (den < 1e-4f) return 0.f;
+    float ut = M::rcp(M::sqrt_(den));
+    float Eem = ut * (E - Om * Lz);
+    if (Eem < 0.04f) Eem = 0.04f;
+    float g = M::rcp(Eem);
+    if (g < 0.04f) g = 0.04f;
+    if (g > 6.f) g = 6.f;
+    return g;
+}
+// Novikov-Thorne flux weight with an inner peak (metric.cuh:190-199)
+template <class M> BHK_INL float fluxWeight(float r, float rIn)
+{
+    if (!(r > rIn) || rIn <= 0.f) return 0.f;
+    float x = M::div(rIn, r);
+    float nt = x * x * x * fmaxf(1.f - M::sqrt_(x), 0.f);
+    float dr = r - rIn * 1.15f;
+    float sig = 0.20f * rIn;
+    float peak = M::exp_(M::div(-0.5f * dr * dr, fmaxf(sig * sig, 1e-8f)));
+    return nt * (0.50f + 1.00f * peak);
+}
+BHK_INL float captureRadius(const RenderParams& P)
+{
+    return (P.rPlus > 1e-6f) ? P.rPlus : kSingCutOverM * P.M;
+}
+
+// ------------------------------------------------------------------ blackbody (trace.cuh:101-120)
+template <class M> BHK_INL v3 blackbody(float kelvin)
+{
+    float t = M::div(clampf(kelvin, 1200.f, 40000.f), 100.f);
+    float r, g, b;
+    if (t <= 66.f) r = 255.f;
+    else r = 329.698727446f * M::pow_(t - 60.f, -0.1332047592f);
+    if (t <= 66.f) g = 99.4708025861f * M::log_(t) - 161.1195681661f;
+    else g = 288.1221695283f * M::pow_(t - 60.f, -0.0755148492f);
+    if (t >= 66.f) b = 255.f;
+    else if (t <= 19.f) b = 0.f;
+    else b = 138.5177312231f * M::log_(t - 10.f) - 305.0447927307f;
+    v3 srgb = mk3(M::div(clampf(r, 0.f, 255.f), 255.f), M::div(clampf(g, 0.f, 255.f), 255.f),
+                  M::div(clampf(b, 0.f, 255.f), 255.f));
+    return pow3<M>(srgb, 2.2f);
+}
+
+// ------------------------------------------------------------------ tetrad (trace.cuh:213-247)
+// Photon conserved E, Lz for the static observer at `o` looking along `d`.
+template <class M> BHK_INL void photonEL(v3 o, v3 d, float Mm, float a, float Q, float& E, float& Lz)
+{
+    float r0 = len3<M>(o);
+    if (r0 < 1e-4f) { E = 1.f; Lz = 0.f; return; }
+    float cth0 = clampf(M::div(o.y, r0), -1.f, 1.f);
+    float th0 = M::acos_(cth0);
+    float ph0 = M::atan2_(-o.z, o.x);
+    float s = M::sin_(th0);
+    float sp = M::sin_(ph0), cp = M::cos_(ph0);
+    float dpc = d.x * (-sp) + d.y * 0.f + d.z * (-cp);
+    float s0 = s < 1e-4f ? 1e-4f : s;
+    float s02 = s0 * s0;
+    float Sig0 = r0 * r0 + a * a * cth0 * cth0;
+    float m2r0 = 2.f * Mm * r0 - Q * Q;
+    float gtt = -(1.f - M::div(m2r0, fmaxf(Sig0, 1e-8f)));
+    if (gtt > -1e-5f) gtt = -1e-5f;
+    float gtp = M::div(-a * m2r0 * s02, Sig0);
+    float gpp = (r0 * r0 + a * a + M::div(m2r0 * a * a * s02, Sig0)) * s02;
+    float A = M::rsq(fmaxf(gpp - M::div(gtp * gtp, gtt), 1e-8f));
+    float ptUp = M::rsq(-gtt) - M::div(dpc * A * gtp, gtt);
+    float pfUp = dpc * A;
+    E = -(gtt * ptUp + gtp * pfUp);
+    Lz = gtp * ptUp + gpp * pfUp;
+    if (!(E > 1e-6f) || !isfinite(E)) { E = 1.f; Lz = 0.f; }
+}
+
+// Wrap to (-pi, pi] (trace.cuh:250-255)
+BHK_INL float wrapDeltaPhi(float d)
+{
+    d = fmodf(d + kPi, 2.f * kPi);
+    if (d < 0.f) d += 2.f * kPi;
+    return d - kPi;
+}
+
+// ------------------------------------------------------------------ disk (trace.cuh:269-403)
+struct DiskParams
+{
+    float rin, rout, invOut, maxH, M, a, Q, emisK, absK, spotS;
+    int spots;
+};
+BHK_INL DiskParams makeDisk(const RenderParams& P, float a, float Q)
+{
+    DiskParams D;
+    D.rin = fmaxf(P.diskInner, 1e-4f);
+    D.rout = P.diskOuter;
+    D.invOut = 1.f / fmaxf(P.diskOuter, D.rin);
+    D.maxH = 0.16f * P.diskOuter;
+    D.M = P.M; D.a = a; D.Q = Q;
+    D.emisK = P.diskEmisScale; D.absK = P.diskAbsScale;
+    D.spots = (P.hotSpotsEnabled && P.hotSpotStrength > 1e-4f) ? 1 : 0;
+    D.spotS = P.hotSpotStrength;
+    return D;
+}
+
+// Radiance and optical depth of one volumetric sub-sample at p.  Returns
+// false where the reference skips the sub-sample (`continue`).
+template <class M>
+BHK_INL bool diskSubSample(const DiskParams& D, float E, float Lz, v3 p, float tEmit, float ds,
+                           v3& add, float& tau, float& densOut)
+{
+    float rc = M::sqrt_(p.x * p.x + p.z * p.z);
+    if (rc < D.rin || rc > D.rout) return false;
+    float HR = 0.040f + 0.015f * (rc * D.invOut);
+    float H = HR * rc;
+    float dz = M::div(p.y, fmaxf(H, 1e-4f));
+    if (fabsf(dz) > 2.6f) return false;
+    float Fr = fluxWeight<M>(rc, D.rin) * kNtPeakInv;
+    bool wantSpots = D.spots && rc < D.rin * 1.60f;
+    if (Fr < 1e-6f && !wantSpots) return false;
+
+    float dens = M::exp_(-dz * dz);
+    dens *= smoothstepf<M>(D.rin, D.rin * 1.10f, rc);
+    dens *= 1.0f - smoothstepf<M>(D.rout * 0.60f, D.rout, rc);
+
+    float omega = omegaCircular<M>(rc, D.M, D.a, D.Q);
+    if (omega <= 0.f) return false;
+    float g = emitterG<M>(rc, D.M, D.a, D.Q, E, Lz, omega);
+    if (g <= 0.f) return false;
+
+    float azim = M::atan2_(-p.z, p.x);
+    float azr = azim - omega * tEmit;
+    float n0 = fbm(mk2(rc * 3.6f, azr * 2.3f + rc * 0.8f));
+    float st = vnoise(mk2(rc * 21.f - azr * 1.7f, azr * 3.5f + rc * 2.2f));
+    dens *= (0.20f + 1.30f * n0) * (0.40f + 1.00f * st);
+    dens *= dens;
+    float psi1 = azim - 0.55f * omega * tEmit - 1.6f * M::log_(fmaxf(rc, 0.2f));
+    dens *= (1.f + 0.38f * M::cos_(psi1));
+
+    float spotBoost = 0.f, spotHeat = 0.f;
+    if (wantSpots)
+    {
+        const float rHs0 = D.rin * 1.06f;
+        const float rHs1 = D.rin * 1.18f;
+        float om0 = omegaCircular<M>(rHs0, D.M, D.a, D.Q);
+        float om1 = omegaCircular<M>(rHs1, D.M, D.a, D.Q);
+        if (om0 <= 0.f) om0 = omega;
+        if (om1 <= 0.f) om1 = omega;
+        const float inv2sPhi = 1.f / (2.f * 0.18f * 0.18f);
+        const float inv2sR0 = M::rcp(2.f * (0.09f * rHs0) * (0.09f * rHs0));
+        const float inv2sR1 = M::rcp(2.f * (0.11f * rHs1) * (0.11f * rHs1));
+        float dph0 = wrapDeltaPhi(azim - (om0 * tEmit + 0.4f));
+        float dph1 = wrapDeltaPhi(azim - (om1 * tEmit + 0.4f + kPi));
+        float dr0 = rc - rHs0;
+        float dr1 = rc - rHs1;
+        float flick0 = 0.72f + 0.28f * M::sin_(1.7f * tEmit + 0.9f);
+        float flick1 = 0.65f + 0.35f * M::sin_(1.1f * tEmit + 2.3f);
+        float g0 = M::exp_(-(dph0 * dph0) * inv2sPhi - dr0 * dr0 * inv2sR0 - dz * dz * 0.45f);
+        float g1 = M::exp_(-(dph1 * dph1) * inv2sPhi - dr1 * dr1 * inv2sR1 - dz * dz * 0.45f);
+        float S = D.spotS;
+        spotBoost = S * (5.0f * flick0 * g0 + 3.2f * flick1 * g1);
+        spotHeat = S * (0.70f * flick0 * g0 + 0.45f * flick1 * g1);
+        dens += dens * 0.25f * (g0 + g1) * S;
+    }
+    if (dens < 1e-5f && spotBoost < 1e-4f) return false;
+
+    float Temit = kTInner * M::pow_(clampf(Fr, 1e-6f, 1.f), 0.25f);
+    Temit = clampf(Temit * (1.f + 1.6f * spotHeat), 2000.f, 26000.f);
+    float Tobs = clampf(Temit * g, 1200.f, 40000.f);
+    v3 col = blackbody<M>(Tobs);
+    float g3 = g * g * g;
+    float emis = Fr * g3 * g * (1.f + spotBoost);
+    float w = dens * ds;
+    add = col * (emis * D.emisK * w);
+    if (spotBoost > 1e-4f)
+    {
+        float wS = (0.12f + dens) * ds;
+        add += col * (spotBoost * g3 * D.emisK * 0.55f * wS);
+    }
+    tau = D.absK * w * (1.f + 0.35f * dens);
+    densOut = dens;
+    return true;
+}
+
+// Chord [a, b] (trace.cuh:269-403): culls, then up to 12 sub-samples.
+template <class M>
+BHK_INL void diskSegment(const DiskParams& D, float E, float Lz, v3 a, v3 b, float tEmit, v3& accum, float& trans)
+{
+    v3 seg = b - a;
+    float segLen = len3<M>(seg);
+    if (segLen < 1e-7f) return;
+    if (a.y > D.maxH && b.y > D.maxH) return;
+    if (a.y < -D.maxH && b.y < -D.maxH) return;
+    float rcA = M::sqrt_(a.x * a.x + a.z * a.z);
+    float rcB = M::sqrt_(b.x * b.x + b.z * b.z);
+    float rcMin = fminf(rcA, rcB), rcMax = fmaxf(rcA, rcB);
+    if (rcMin - segLen > D.rout) return;
+    if (rcMax + segLen < D.rin) return;
+    int nSub = 1 + (int)(segLen * 6.5f);
+    if (nSub > 12) nSub = 12;
+    float fn = (float)nSub;
+    float ds = M::div(segLen, fn);
+    for (int j = 0; j < nSub; ++j)
+    {
+        float t = M::div((float)j + 0.5f, fn);
+        v3 p = lerp3(a, b, t);
+        v3 add;
+        float tau, dens;
+        if (!diskSubSample<M>(D, E, Lz, p, tEmit, ds, add, tau, dens)) continue;
+        accum += add * trans;
+        trans *= M::exp_(-tau);
+        if (trans < 0.012f) return;
+    }
+}
+
+// ------------------------------------------------------------------ camera (trace.cuh:81-96, 431-438)
+template <class M> BHK_INL v3 cameraRayDir(const RenderParams& P, float fx, float fy)
+{
+    float nx = fx * 2.f - 1.f;
+    float ny = 1.f - fy * 2.f;
+    v3 F = mk3(P.camForward.x, P.camForward.y, P.camForward.z);
+    v3 R = mk3(P.camRight.x, P.camRight.y, P.camRight.z);
+    v3 U = mk3(P.camUp.x, P.camUp.y, P.camUp.z);
+    return normalize<M>(F + R * (nx * P.tanHalfFov * P.aspect) + U * (ny * P.tanHalfFov));
+}
+
+BHK_INL float cameraEnergyFactor(const RenderParams& P)
+{
+    v3 c = mk3(P.camPos.x, P.camPos.y, P.camPos.z);
+    float r = sqrtf(dot(c, c));
+    if (r < 1e-4f) return 1.f;
+    if (P.model == BH_KERR || P.model == BH_KERR_NEWMAN)
+    {
+        float cth = clampf(c.y / r, -1.f, 1.f);
+        float Sig = r * r + P.aSpin * P.aSpin * cth * cth;
+        float m2r = 2.f * P.M * r - P.Qc * P.Qc;
+        float gtt = -(1.f - m2r / fmaxf(Sig, 1e-8f));
+        float alpha = sqrtf(fmaxf(-gtt, 1e-6f));
+        return clampf(1.f / alpha, 0.5f, 4.f);
+    }
+    float f = 1.f - 2.f * P.M / r + P.Qc * P.Qc / (r * r);
+    return clampf(1.f / sqrtf(fmaxf(f, 1e-6f)), 0.5f, 4.f);
+}
+
+// Sub-pixel jitter, R2 sequence (post_process.cuh:47-53)
+BHK_INL v2 subpixelJitter(int accumMode, int sampleIndex)
+{
+#pragma clang fp contract(off)
+    if (accumMode == 0 || sampleIndex <= 0) return mk2(0.5f, 0.5f);
+    float n = (float)sampleIndex;
+    return mk2(fract(0.7548776662f * n + 0.5f), fract(0.5698402909f * n + 0.5f));
+}
+
+// ------------------------------------------------------------------ sky (trace.cuh:131-195), exact arithmetic
+BHK_INL v3 starLayer(v3 d, float cellScale, float intensity, float coreSharp)
+{
+#pragma clang fp contract(off)
+    v3 p = d * cellScale;
+    v3 id = mk3(floorf(p.x), floorf(p.y), floorf(p.z));
+    v3 f = p - id;
+    v3 h = hash33(id);
+    v3 sp = mk3(0.10f + 0.80f * h.x, 0.10f + 0.80f * h.y, 0.10f + 0.80f * h.z);
+    v3 dv = f - sp;
+    float d2 = dot(dv, dv);
+    float mag = hash12(mk2(id.x + 91.7f * id.z, id.y - 33.1f * id.x));
+    float cut = (mag > 0.9f) ? 60.f : 14.f;
+    if (d2 * coreSharp > cut) return mk3(0.f, 0.f, 0.f);
+    float bright = 0.05f * powf(mag, 9.f) + 9.0f * powf(mag, 50.f);
+    float core = expf(-d2 * coreSharp);
+    float glow = expf(-d2 * coreSharp * 0.10f);
+    float tsel = hash12(mk2(id.y + 7.3f, id.z - 3.1f));
+    float kelvin = 2600.f + 12000.f * tsel * tsel;
+    v3 col = blackbody<MathExact>(kelvin);
+    float lum = 0.2126f * col.x + 0.7152f * col.y + 0.0722f * col.z;
+    col = col * (1.f / fmaxf(lum, 1e-4f));
+    return col * ((core + 0.08f * glow) * bright * intensity);
+}
+
+BHK_INL v3 backgroundColor(v3 dir, float gCam)
+{
+#pragma clang fp contract(off)
+    v3 d = normalize<MathExact>(dir);
+    float Ts = clampf(gCam, 0.5f, 3.f);
+    float Is = Ts * Ts * Ts * Ts;
+    v3 c = starLayer(d, 80.f, 1.00f, 320.f);
+    c += starLayer(d, 190.f, 0.55f, 520.f);
+    c += starLayer(d, 420.f, 0.30f, 820.f);
+    c += starLayer(d, 900.f, 0.14f, 1500.f);
+    float s = 3.0f / (1.35f + fabsf(d.y));
+    v2 q = mk2(d.x * s, d.z * s);
+    float neb = fbm(q);
+    float dust = fbm(mk2(q.x * 2.6f + 31.4f, q.y * 2.6f - 17.2f));
+    float band = expf(-d.y * d.y * 16.f);
+    float lanes = clampf(1.25f - 1.35f * dust, 0.f, 1.f);
+    c += (mk3(0.020f, 0.024f, 0.038f) + mk3(0.030f, 0.022f, 0.014f) * neb) * (neb * band * lanes);
+    c += mk3(0.0012f, 0.0015f, 0.0026f);
+    return c * Is;
+}
+
+// ------------------------------------------------------------------ post (post_process.cuh), exact arithmetic
+BHK_INL v3 acesToneMap(v3 x)
+{
+#pragma clang fp contract(off)
+    const float A = 2.51f, B = 0.03f, C = 2.43f, D = 0.59f, E = 0.14f;
+    v3 num = x * (x * A + mk3(B, B, B));
+    v3 den = x * (x * C + mk3(D, D, D)) + mk3(E, E, E);
+    return mk3(clampf(num.x / den.x, 0.f, 1.f), clampf(num.y / den.y, 0.f, 1.f), clampf(num.z / den.z, 0.f, 1.f));
+}
+BHK_INL v3 accumBlend(v3 prev, v3 cur, int accumMode, int sampleIndex)
+{
+#pragma clang fp contract(off)
+    if (accumMode == 0 || sampleIndex <= 0) return cur;
+    if (accumMode == 1)
+    {
+        float n = (float)sampleIndex;
+        return prev + (cur - prev) * (1.0f / (n + 1.0f));
+    }
+    return lerp3(prev, cur, 0.42f);
+}
+BHK_INL v3 bloomBrightPass(v3 c)
+{
+#pragma clang fp contract(off)
+    float l = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
+    float w = smoothstepf<MathExact>(0.95f, 2.2f, l);
+    return c * w;
+}
+BHK_INL uchar4 finalizePixel(v3 hdr, v3 bloom, const RenderParams& P, int px, int py)
+{
+#pragma clang fp contract(off)
+    v3 c = hdr * P.exposure;
+    if (P.bloomEnabled) c += bloom * P.bloomStrength;
+    c = acesToneMap(c);
+    float lum = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
+    v3 lv = mk3(lum, lum, lum);
+    v3 t = lv + (c - lv) * 1.12f;
+    c = mk3(clampf(t.x, 0.f, 1.f), clampf(t.y, 0.f, 1.f), clampf(t.z, 0.f, 1.f));
+    c = pow3<MathExact>(c, 1.0f / 2.2f);
+    v2 fp = mk2((float)px, (float)py);
+    float d = (hash12(fp) + hash12(fp + mk2(41.3f, 17.7f))) - 1.0f;
+    float dd = d * (1.0f / 255.0f);
+    unsigned char r8 = (unsigned char)(clampf(c.x + dd, 0.f, 1.f) * 255.f + 0.5f);
+    unsigned char g8 = (unsigned char)(clampf(c.y + dd, 0.f, 1.f) * 255.f + 0.5f);
+    unsigned char b8 = (unsigned char)(clampf(c.z + dd, 0.f, 1.f) * 255.f + 0.5f);
+    return P.swapRB ? make_uchar4(b8, g8, r8, 255) : make_uchar4(r8, g8, b8, 255);
+}
+
+}  // namespace bhk

@@ -1,0 +1,57 @@
+// ---------------------------------------------------------------------------
+// bh_internal.h -- launch descriptors shared by the host layer and the
+// kernel objects of libbhtrace.so (not part of the public ABI).
+// ---------------------------------------------------------------------------
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/bh/bhtrace.h"
+
+namespace bhk {
+
+struct TraceArgs
+{
+    RenderParams P;         // sanitized
+    BhRayRecord* rec;       // [W*H] per-pixel trace records
+    unsigned* counter;      // ray queue head (zero before launch)
+    int tilesX;             // ceil(W/8)
+    int totalIds;           // ceil(W/8)*ceil(H/8)*64
+    int chunk;              // RK4 steps between refills
+    float2 jitter;          // sub-pixel sample position
+};
+
+struct ProbeArgs
+{
+    RenderParams P;
+    BhProbe* out;
+    const bh_float3* dirs;  // explicit directions, or null for an nx*ny grid
+    int n, nx, ny;
+};
+
+struct LaunchShape
+{
+    int wgSize;
+    int blocks;
+};
+
+hipError_t launchTraceExact(const TraceArgs& A, const LaunchShape& S, hipStream_t st);
+hipError_t launchTraceFast(const TraceArgs& A, const LaunchShape& S, hipStream_t st);
+int traceOccupancyExact(int kerr, int wgSize);
+int traceOccupancyFast(int kerr, int wgSize);
+hipError_t launchProbeExact(const ProbeArgs& A, hipStream_t st);
+hipError_t launchProbeFast(const ProbeArgs& A, hipStream_t st);
+
+// post chain (bh_post.hip)
+hipError_t launchResetCounter(unsigned* counter, hipStream_t st);
+// records -> HDR sample -> blend into accum (pipeline) ...
+hipError_t launchShadeBlend(const RenderParams& P, const BhRayRecord* rec, float4* accum, hipStream_t st);
+// ... or records -> HDR sample -> tonemapped pixel (single pass, no bloom)
+hipError_t launchShadeFinalize(const RenderParams& P, const BhRayRecord* rec, uchar4* out, hipStream_t st);
+// accum -> half-res bright pass + horizontal blur (into dstH)
+hipError_t launchBloomDownBlurH(const RenderParams& P, const float4* accum, float4* dstH, hipStream_t st);
+// vertical blur of srcH (into dstV, the final bloom image) fused with the composite
+hipError_t launchBlurVComposite(const RenderParams& P, const float4* accum, const float4* srcH, float4* dstV,
+                                uchar4* out, hipStream_t st);
+// composite without bloom
+hipError_t launchComposite(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st);
+
+}  // namespace bhk

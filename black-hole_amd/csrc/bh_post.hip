@@ -1,0 +1,225 @@
+// ---------------------------------------------------------------------------
+// bh_post.hip -- per-pixel shading and post chain (exact IEEE arithmetic).
+//
+//   shade      trace record -> sky (4 star lattices + Milky-Way band, once per
+//              escaped ray) -> HDR sample -> accumulation blend   (pixel order,
+//              full wavefronts; trace.cuh:908-926, black_hole_kernel.cu:22-39)
+//   bloom      half-resolution bright pass fused with the horizontal 9-tap blur
+//              (one LDS row segment per workgroup), then the vertical blur fused
+//              into the composite through a 2-D LDS tile
+//              (black_hole_kernel.cu:41-103, post_process.cuh:76-148).
+//
+// HBM traffic per full-resolution pixel (float4 accum, uchar4 out):
+//   shade 32 (record) + 16 (accum read) + 16 (accum write)
+//   bloom 16 (accum read) + 4 (half-res write); composite 16 + ~4 + 4 (out).
+// ---------------------------------------------------------------------------
+#include "bh_common.h"
+#include "bh_internal.h"
+
+namespace bhk {
+
+__global__ void resetCounterKernel(unsigned* c) { c[0] = 0u; }
+
+BHK_INL v3 sampleFromRecord(const BhRayRecord* rec, int idx, float gCam)
+{
+    const float4* r = reinterpret_cast<const float4*>(rec + idx);
+    float4 a = r[0], b = r[1];
+    int flags = __float_as_int(a.w);
+    v3 bg = (flags & 1) ? backgroundColor(mk3(a.x, a.y, a.z), gCam) : mk3(0.f, 0.f, 0.f);
+    v3 hdr;
+    {
+#pragma clang fp contract(off)
+        hdr = mk3(b.x, b.y, b.z) + bg * b.w;
+    }
+    if (!finite3(hdr)) hdr = mk3(0.f, 0.f, 0.f);
+    return hdr;
+}
+
+__global__ __launch_bounds__(256) void shadeBlendKernel(RenderParams P, const BhRayRecord* rec, float4* accum)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P.width * P.height) return;
+    const float gCam = cameraEnergyFactor(P);
+    v3 cur = sampleFromRecord(rec, idx, gCam);
+    float4 pv = accum[idx];
+    v3 prev = mk3(pv.x, pv.y, pv.z);
+    if (!finite3(prev)) prev = cur;
+    v3 c = accumBlend(prev, cur, P.accumMode, P.sampleIndex);
+    accum[idx] = make_float4(c.x, c.y, c.z, 1.f);
+}
+
+__global__ __launch_bounds__(256) void shadeFinalizeKernel(RenderParams P, const BhRayRecord* rec, uchar4* out)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P.width * P.height) return;
+    const float gCam = cameraEnergyFactor(P);
+    v3 hdr = sampleFromRecord(rec, idx, gCam);
+    int px = idx % P.width, py = idx / P.width;
+    out[idx] = finalizePixel(hdr, mk3(0.f, 0.f, 0.f), P, px, py);
+}
+
+BHK_INL float gaussW(int k)
+{
+    k = k < 0 ? -k : k;
+    return k == 0 ? 0.20236f : k == 1 ? 0.17925f : k == 2 ? 0.12458f : k == 3 ? 0.06794f : 0.02907f;
+}
+
+// One workgroup = one half-res row segment of kSegX outputs.
+constexpr int kSegX = 256;
+__global__ __launch_bounds__(kSegX) void bloomDownBlurHKernel(RenderParams P, const float4* accum, float4* dstH,
+                                                             int bw, int bh)
+{
+#pragma clang fp contract(off)
+    __shared__ float3 row[kSegX + 8];
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * kSegX;
+    const int W = P.width, H = P.height;
+    const int yA = y * 2, yB = (yA + 1 < H) ? yA + 1 : yA;
+    for (int t = threadIdx.x; t < kSegX + 8; t += kSegX)
+    {
+        int x = x0 - 4 + t;
+        x = x < 0 ? 0 : (x >= bw ? bw - 1 : x);
+        int xa = x * 2, xb = (xa + 1 < W) ? xa + 1 : xa;
+        float4 a = accum[yA * W + xa], b = accum[yA * W + xb];
+        float4 c = accum[yB * W + xa], d = accum[yB * W + xb];
+        v3 avg = mk3(a.x + b.x + c.x + d.x, a.y + b.y + c.y + d.y, a.z + b.z + c.z + d.z) * 0.25f;
+        v3 bp = bloomBrightPass(avg * P.exposure);
+        row[t] = make_float3(bp.x, bp.y, bp.z);
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x;
+    if (x >= bw) return;
+    v3 s = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = -4; k <= 4; ++k)
+    {
+        int xs = x + k;
+        xs = xs < 0 ? 0 : (xs >= bw ? bw - 1 : xs);
+        float3 v = row[xs - (x0 - 4)];
+        s += mk3(v.x, v.y, v.z) * gaussW(k);
+    }
+    dstH[y * bw + x] = make_float4(s.x, s.y, s.z, 1.f);
+}
+
+// Output tile of kTW x kTH full-resolution pixels.
+constexpr int kTW = 64, kTH = 16;
+constexpr int kHX = kTW / 2 + 2;      // half-res columns touched by the bilinear taps
+constexpr int kHY = kTH / 2 + 2;      // half-res rows touched
+constexpr int kBY = kHY + 8;          // rows of the horizontally blurred image needed
+
+__global__ __launch_bounds__(256) void blurVCompositeKernel(RenderParams P, const float4* accum, const float4* srcH,
+                                                            float4* dstV, uchar4* out, int bw, int bh)
+{
+#pragma clang fp contract(off)
+    __shared__ float3 tileH[kBY][kHX];
+    __shared__ float3 tileV[kHY][kHX];
+    const int X0 = blockIdx.x * kTW, Y0 = blockIdx.y * kTH;
+    const int hx0 = X0 / 2 - 1, hy0 = Y0 / 2 - 1;   // first half-res tap (unclamped)
+    const int by0 = hy0 - 4;
+    for (int t = threadIdx.x; t < kBY * kHX; t += blockDim.x)
+    {
+        int ry = t / kHX, cx = t % kHX;
+        int yy = by0 + ry, xx = hx0 + cx;
+        yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
+        xx = xx < 0 ? 0 : (xx >= bw ? bw - 1 : xx);
+        float4 v = srcH[yy * bw + xx];
+        tileH[ry][cx] = make_float3(v.x, v.y, v.z);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kHY * kHX; t += blockDim.x)
+    {
+        int ry = t / kHX, cx = t % kHX;
+        int hy = hy0 + ry;
+        int hyc = hy < 0 ? 0 : (hy >= bh ? bh - 1 : hy);
+        v3 s = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = -4; k <= 4; ++k)
+        {
+            int yy = hyc + k;
+            yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
+            float3 v = tileH[yy - by0][cx];
+            s += mk3(v.x, v.y, v.z) * gaussW(k);
+        }
+        tileV[ry][cx] = make_float3(s.x, s.y, s.z);
+        int hx = hx0 + cx;
+        if (dstV && hy == hyc && hx >= 0 && hx < bw && ry >= 1 && ry <= kHY - 2 && cx >= 1 && cx <= kHX - 2)
+            dstV[hy * bw + hx] = make_float4(s.x, s.y, s.z, 1.f);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kTW * kTH; t += blockDim.x)
+    {
+        int px = X0 + (t % kTW), py = Y0 + (t / kTW);
+        if (px >= P.width || py >= P.height) continue;
+        float x = ((float)px + 0.5f) * 0.5f - 0.5f;
+        float y = ((float)py + 0.5f) * 0.5f - 0.5f;
+        int ix = (int)floorf(x), iy = (int)floorf(y);
+        float tx = x - (float)ix, ty = y - (float)iy;
+        auto tap = [&](int xx, int yy) {
+            xx = xx < 0 ? 0 : (xx >= bw ? bw - 1 : xx);
+            yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
+            float3 v = tileV[yy - hy0][xx - hx0];
+            return mk3(v.x, v.y, v.z);
+        };
+        v3 c00 = tap(ix, iy), c10 = tap(ix + 1, iy), c01 = tap(ix, iy + 1), c11 = tap(ix + 1, iy + 1);
+        v3 bloom = lerp3(lerp3(c00, c10, tx), lerp3(c01, c11, tx), ty);
+        float4 av = accum[py * P.width + px];
+        out[py * P.width + px] = finalizePixel(mk3(av.x, av.y, av.z), bloom, P, px, py);
+    }
+}
+
+__global__ __launch_bounds__(256) void compositeKernel(RenderParams P, const float4* accum, uchar4* out)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P.width * P.height) return;
+    float4 av = accum[idx];
+    out[idx] = finalizePixel(mk3(av.x, av.y, av.z), mk3(0.f, 0.f, 0.f), P, idx % P.width, idx / P.width);
+}
+
+// ------------------------------------------------------------------ launchers
+hipError_t launchResetCounter(unsigned* counter, hipStream_t st)
+{
+    hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(1), 0, st, counter);
+    return hipGetLastError();
+}
+
+static inline unsigned blocksFor(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launchShadeBlend(const RenderParams& P, const BhRayRecord* rec, float4* accum, hipStream_t st)
+{
+    long long n = (long long)P.width * P.height;
+    hipLaunchKernelGGL(shadeBlendKernel, dim3(blocksFor(n, 256)), dim3(256), 0, st, P, rec, accum);
+    return hipGetLastError();
+}
+
+hipError_t launchShadeFinalize(const RenderParams& P, const BhRayRecord* rec, uchar4* out, hipStream_t st)
+{
+    long long n = (long long)P.width * P.height;
+    hipLaunchKernelGGL(shadeFinalizeKernel, dim3(blocksFor(n, 256)), dim3(256), 0, st, P, rec, out);
+    return hipGetLastError();
+}
+
+hipError_t launchBloomDownBlurH(const RenderParams& P, const float4* accum, float4* dstH, hipStream_t st)
+{
+    int bw = (P.width + 1) / 2, bh = (P.height + 1) / 2;
+    dim3 grid(blocksFor(bw, kSegX), bh);
+    hipLaunchKernelGGL(bloomDownBlurHKernel, grid, dim3(kSegX), 0, st, P, accum, dstH, bw, bh);
+    return hipGetLastError();
+}
+
+hipError_t launchBlurVComposite(const RenderParams& P, const float4* accum, const float4* srcH, float4* dstV,
+                                uchar4* out, hipStream_t st)
+{
+    int bw = (P.width + 1) / 2, bh = (P.height + 1) / 2;
+    dim3 grid(blocksFor(P.width, kTW), blocksFor(P.height, kTH));
+    hipLaunchKernelGGL(blurVCompositeKernel, grid, dim3(256), 0, st, P, accum, srcH, dstV, out, bw, bh);
+    return hipGetLastError();
+}
+
+hipError_t launchComposite(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st)
+{
+    long long n = (long long)P.width * P.height;
+    hipLaunchKernelGGL(compositeKernel, dim3(blocksFor(n, 256)), dim3(256), 0, st, P, accum, out);
+    return hipGetLastError();
+}
+
+}  // namespace bhk

@@ -1,0 +1,336 @@
+// ---------------------------------------------------------------------------
+// bh_rays.h -- per-ray integrator state machines.
+//
+// Each ray is advanced one RK4 step per call so a persistent wavefront can
+// interleave rays of very different lengths: lanes whose ray ends take a new
+// ray from the work queue between step chunks.  The per-ray arithmetic is
+// exactly the reference integrator's (same operations in the same order),
+// so the output does not depend on the schedule.
+//
+//   SphRay  -- Binet equation u'' = 3 M u^2 - 2 Q^2 u^3 - u in the orbital
+//              plane (trace.cuh:466-611), Schwarzschild and Reissner-Nordstrom.
+//   KerrRay -- Boyer-Lindquist Hamiltonian RK4 for (r, th, ph, p_r, p_th)
+//              with conserved E, Lz (trace.cuh:616-890), Kerr and Kerr-Newman.
+// ---------------------------------------------------------------------------
+#pragma once
+#include "bh_common.h"
+
+namespace bhk {
+
+enum : int { RAY_RUNNING = 0, RAY_CAPTURED = 1, RAY_ESCAPED = 2 };
+
+// Per-frame constants shared by every ray of a launch.
+struct FrameK
+{
+    float M, a, Q, Q2;
+    float rCap, rFloor, uHor, escR, uEscape, invNineM;
+    float dPhi, hFloor, rPhoton, diskOuter, diskTime;
+    int maxSteps, disk;
+    DiskParams D;
+};
+
+BHK_INL FrameK makeFrame(const RenderParams& P)
+{
+    FrameK K;
+    K.M = P.M; K.a = P.aSpin; K.Q = P.Qc; K.Q2 = P.Qc * P.Qc;
+    K.rCap = captureRadius(P);
+    K.rFloor = fmaxf(P.rPlus, K.rCap);
+    K.uHor = 1.0f / fmaxf(K.rCap, 1e-6f);
+    K.escR = 120.f * P.M;
+    K.uEscape = 1.0f / K.escR;
+    K.invNineM = 1.f / (9.f * P.M);
+    K.dPhi = P.dPhi;
+    K.hFloor = P.dPhi * 0.03f;
+    K.rPhoton = P.rPhoton;
+    K.diskOuter = P.diskOuter;
+    K.diskTime = P.diskTime;
+    K.maxSteps = P.maxSteps;
+    K.disk = P.diskEnabled != 0;
+    bool kerr = (P.model == BH_KERR || P.model == BH_KERR_NEWMAN);
+    K.D = makeDisk(P, kerr ? P.aSpin : 0.f, P.Qc);
+    return K;
+}
+
+// ========================================================================= spherical
+template <class M> struct SphRay
+{
+    float u, du, phi, tFlight, bImpact, E, Lz, trans;
+    v3 e1, e2, prevPos, dir3, acc, escDir;
+    int step, status;
+
+    BHK_INL void init(const FrameK& K, v3 origin, v3 dir)
+    {
+        status = RAY_RUNNING;
+        step = 0;
+        escDir = dir;
+        acc = mk3(0.f, 0.f, 0.f);
+        trans = 1.0f;
+        photonEL<M>(origin, dir, K.M, 0.f, K.Q, E, Lz);
+        float r0 = len3<M>(origin);
+        e1 = vdiv<M>(origin, r0);
+        float ddr = dot(dir, e1);
+        v3 perp = dir - e1 * ddr;
+        float pl = len3<M>(perp);
+        if (pl < 1e-6f)
+        {
+            // exactly radial ray (trace.cuh:447-459, 500-505)
+            status = (dot(dir, origin) < 0.f) ? RAY_CAPTURED : RAY_ESCAPED;
+            return;
+        }
+        e2 = vdiv<M>(perp, pl);
+        float f0 = fmaxf(1.f - M::div(2.f * K.M, r0) + M::div(K.Q2, r0 * r0), 1e-6f);
+        bImpact = fmaxf(M::div(r0 * pl, M::sqrt_(f0)), 1e-6f);
+        u = M::rcp(r0);
+        du = M::div(-M::sqrt_(f0) * ddr, r0 * pl);
+        phi = 0.f;
+        tFlight = 0.f;
+        prevPos = origin;
+        dir3 = dir;
+    }
+
+    BHK_INL float acc_(const FrameK& K, float uu) const
+    {
+        return 3.f * K.M * uu * uu - 2.f * K.Q2 * uu * uu * uu - uu;
+    }
+
+    // One RK4 step; returns true once the ray has terminated.
+    BHK_INL bool advance(const FrameK& K)
+    {
+        if (step >= K.maxSteps)
+        {
+            // budget exhausted (trace.cuh:603-609)
+            if (u > M::rcp(fmaxf(1.5f * K.rPhoton, 3.f * K.M))) status = RAY_CAPTURED;
+            else { status = RAY_ESCAPED; escDir = dir3; }
+            return true;
+        }
+        float hMax = K.dPhi * clampf(M::div(K.invNineM, u), 1.f, 6.f);
+        float h = fminf(hMax, M::div(0.10f, fmaxf(fabsf(du), 1e-6f)));
+        if (h < K.hFloor) h = K.hFloor;
+        float k1u = du;
+        float k1v = acc_(K, u);
+        float u2 = u + 0.5f * h * k1u;
+        float v2 = du + 0.5f * h * k1v;
+        float k2u = v2;
+        float k2v = acc_(K, u2);
+        float u3 = u + 0.5f * h * k2u;
+        float v3_ = du + 0.5f * h * k2v;
+        float k3u = v3_;
+        float k3v = acc_(K, u3);
+        float u4 = u + h * k3u;
+        float v4 = du + h * k3v;
+        float k4u = v4;
+        float k4v = acc_(K, u4);
+        float h6 = M::div(h, 6.f);
+        u += h6 * (k1u + 2.f * k2u + 2.f * k3u + k4u);
+        du += h6 * (k1v + 2.f * k2v + 2.f * k3v + k4v);
+        phi += h;
+        if (!isfinite(u) || !isfinite(du)) { status = RAY_CAPTURED; return true; }
+        if (u >= K.uHor) { status = RAY_CAPTURED; return true; }
+        if (u < 1e-4f) { status = RAY_ESCAPED; escDir = dir3; return true; }
+        float r = M::rcp(u);
+        float cp = M::cos_(phi), sp = M::sin_(phi);
+        v3 m = e1 * cp + e2 * sp;
+        v3 mp = e1 * (-sp) + e2 * cp;
+        float drdphi = M::div(-du, u * u);
+        v3 pos = m * r;
+        dir3 = normalize<M>(m * drdphi + mp * r);
+        float fR = fmaxf(1.f - M::div(2.f * K.M, r) + M::div(K.Q2, r * r), 1e-4f);
+        tFlight += M::div(h * (r * r), bImpact * fR);
+        if (K.disk && trans > 0.01f) diskSegment<M>(K.D, E, Lz, prevPos, pos, K.diskTime - tFlight, acc, trans);
+        prevPos = pos;
+        if (u < K.uEscape && du < 0.f) { status = RAY_ESCAPED; escDir = dir3; return true; }
+        ++step;
+        return false;
+    }
+};
+
+// ========================================================================= Kerr-Newman
+struct KDeriv { float r, th, ph, pr, pth; };
+
+// Hamiltonian right-hand side (trace.cuh:620-659); s, c = sin/cos of th.
+template <class M, bool kWant>
+BHK_INL KDeriv kerrRHS(const FrameK& K, float E, float Lz, float r, float pr, float pth, float s, float cth,
+                       float& Kout)
+{
+    if (s < 1e-4f) s = 1e-4f;
+    float s2 = s * s;
+    float r2 = r * r, a = K.a, a2 = a * a;
+    float Sig = r2 + a2 * cth * cth;
+    float Del = r2 - 2.f * K.M * r + a2 + K.Q * K.Q;
+    if (Del < 1e-6f) Del = 1e-6f;
+    float Pp = E * (r2 + a2) - a * Lz;
+    float B = M::div(Lz, s) - a * E * s;
+    float W = B * B - M::div(Pp * Pp, Del);
+    float Kc = Del * pr * pr + pth * pth + W;
+    if (kWant) Kout = Kc;
+    float dDel = 2.f * r - 2.f * K.M;
+    float dP = 2.f * r * E;
+    float Wr = M::div(-(2.f * Pp * dP * Del - Pp * Pp * dDel), Del * Del);
+    float dB = M::div(-Lz * cth, s2) - a * E * cth;
+    float Wth = 2.f * B * dB;
+    float invS = M::rcp(Sig);
+    float invS2 = invS * invS;
+    KDeriv d;
+    d.r = Del * pr * invS;
+    d.th = pth * invS;
+    d.ph = ((M::div(Lz, s2) - a * E) + M::div(a * Pp, Del)) * invS;
+    d.pr = -0.5f * invS * (dDel * pr * pr + Wr) + r * invS2 * Kc;
+    d.pth = -0.5f * invS * Wth - a2 * s * cth * invS2 * Kc;
+    return d;
+}
+
+template <class M, bool kWant> struct KerrRay
+{
+    float r, th, ph, pr, pth;     // Boyer-Lindquist state
+    float sth, cth;               // sin/cos of th at the current state
+    float E, Lz, tFlight, trans, maxHviol;
+    float dLr, dLth, dLph;        // last RK4 stage derivative (budget exit)
+    v3 prevPos, dir3, acc, escDir;
+    int step, status, haveDeriv;
+
+    BHK_INL void init(const FrameK& K, v3 origin, v3 dir)
+    {
+        status = RAY_RUNNING;
+        step = 0;
+        haveDeriv = 0;
+        escDir = dir;
+        acc = mk3(0.f, 0.f, 0.f);
+        trans = 1.0f;
+        maxHviol = 0.f;
+        const float a = K.a, Mm = K.M, Q = K.Q;
+        float r0 = len3<M>(origin);
+        float cth0 = clampf(M::div(origin.y, r0), -1.f, 1.f);
+        float th0 = M::acos_(cth0);
+        float ph0 = M::atan2_(-origin.z, origin.x);
+        float s = M::sin_(th0), c = M::cos_(th0);
+        float sp = M::sin_(ph0), cp = M::cos_(ph0);
+        // orthonormal BL basis at the camera (trace.cuh:673-682)
+        v3 er = mk3(s * cp, c, -s * sp);
+        v3 eth = mk3(c * cp, -s, -c * sp);
+        v3 eph = mk3(-sp, 0.f, -cp);
+        float drc = dot(dir, er), dtc = dot(dir, eth), dpc = dot(dir, eph);
+        // static-observer tetrad (trace.cuh:716-739)
+        float s0 = s < 1e-4f ? 1e-4f : s;
+        float s02 = s0 * s0;
+        float Sig0 = r0 * r0 + a * a * cth0 * cth0;
+        float Del0 = r0 * r0 - 2.f * Mm * r0 + a * a + Q * Q;
+        float m2r0 = 2.f * Mm * r0 - Q * Q;
+        float gtt = -(1.f - M::div(m2r0, Sig0));
+        if (gtt > -1e-5f) gtt = -1e-5f;
+        float gtp = M::div(-a * m2r0 * s02, Sig0);
+        float gpp = (r0 * r0 + a * a + M::div(m2r0 * a * a * s02, Sig0)) * s02;
+        float A = M::rsq(fmaxf(gpp - M::div(gtp * gtp, gtt), 1e-8f));
+        float ptUp = M::rsq(-gtt) - M::div(dpc * A * gtp, gtt);
+        float pfUp = dpc * A;
+        E = -(gtt * ptUp + gtp * pfUp);
+        Lz = gtp * ptUp + gpp * pfUp;
+        r = r0; th = th0; ph = ph0;
+        pr = drc * M::sqrt_(M::div(Sig0, fmaxf(Del0, 1e-6f)));
+        pth = dtc * M::sqrt_(Sig0);
+        sth = s; cth = c;
+        tFlight = 0.f;
+        prevPos = origin;
+        dir3 = dir;
+    }
+
+    BHK_INL v3 bodyDir(float dr_, float dth_, float dph_, float sp, float cp) const
+    {
+        // e_r dr + e_th r dth + e_ph r sin(th) dph (trace.cuh:776-779)
+        v3 e1b = mk3(sth * cp, cth, -sth * sp);
+        v3 e2b = mk3(cth * cp, -sth, -cth * sp);
+        v3 e3b = mk3(-sp, 0.f, -cp);
+        return normalize<M>(e1b * dr_ + e2b * (r * dth_) + e3b * (r * sth * dph_));
+    }
+
+    BHK_INL bool advance(const FrameK& K)
+    {
+        if (step >= K.maxSteps)
+        {
+            // budget exhausted (trace.cuh:871-888)
+            if (r < 6.f * K.M) status = RAY_CAPTURED;
+            else
+            {
+                status = RAY_ESCAPED;
+                if (haveDeriv) escDir = bodyDir(dLr, dLth, dLph, M::sin_(ph), M::cos_(ph));
+                else escDir = dir3;
+            }
+            return true;
+        }
+        float Kv = 0.f;
+        KDeriv d1 = kerrRHS<M, kWant>(K, E, Lz, r, pr, pth, sth, cth, Kv);
+        if (!isfinite(r) || !isfinite(pr) || !isfinite(pth) || !isfinite(d1.r) || !isfinite(d1.pr))
+        { status = RAY_CAPTURED; return true; }
+        if (r <= K.rCap * 1.002f + 1e-3f) { status = RAY_CAPTURED; return true; }
+        if (kWant && r > K.rFloor * 1.05f)
+        {
+            float viol = M::div(fabsf(Kv), E * E * (r * r + K.a * K.a) + 1e-12f);
+            if (viol > maxHviol) maxHviol = viol;
+        }
+        if (r >= K.escR && d1.r > 0.f)
+        {
+            escDir = bodyDir(d1.r, d1.th, d1.ph, M::sin_(ph), M::cos_(ph));
+            status = RAY_ESCAPED;
+            return true;
+        }
+        // adaptive affine step (trace.cuh:793-809)
+        float h = M::rcp(M::div(fabsf(d1.r), 8.0f * K.dPhi * fmaxf(r - K.rFloor, 0.02f)) +
+                         M::div(fabsf(d1.th) + fabsf(d1.ph), 2.2f * K.dPhi) + 1e-5f);
+        if (r < 1.5f * K.rPhoton) h *= 0.55f;
+        float spd = fabsf(d1.r) + r * (fabsf(d1.th) + fabsf(d1.ph)) + 1e-6f;
+        float capLen;
+        if (K.disk && r < K.diskOuter * 1.3f) capLen = (r * fabsf(cth) < 1.8f) ? 1.2f : 2.2f;
+        else capLen = fmaxf(1.2f, 0.22f * r);
+        h = fminf(h, M::div(capLen, spd));
+
+        float hh = 0.5f * h;
+        float kd;
+        float y2th = th + hh * d1.th;
+        KDeriv d2 = kerrRHS<M, false>(K, E, Lz, r + hh * d1.r, pr + hh * d1.pr, pth + hh * d1.pth,
+                                      M::sin_(y2th), M::cos_(y2th), kd);
+        float y3th = th + hh * d2.th;
+        KDeriv d3 = kerrRHS<M, false>(K, E, Lz, r + hh * d2.r, pr + hh * d2.pr, pth + hh * d2.pth,
+                                      M::sin_(y3th), M::cos_(y3th), kd);
+        float y4th = th + h * d3.th;
+        KDeriv d4 = kerrRHS<M, false>(K, E, Lz, r + h * d3.r, pr + h * d3.pr, pth + h * d3.pth,
+                                      M::sin_(y4th), M::cos_(y4th), kd);
+        float h6 = M::div(h, 6.f);
+        r += h6 * (d1.r + 2.f * d2.r + 2.f * d3.r + d4.r);
+        th += h6 * (d1.th + 2.f * d2.th + 2.f * d3.th + d4.th);
+        ph += h6 * (d1.ph + 2.f * d2.ph + 2.f * d3.ph + d4.ph);
+        pr += h6 * (d1.pr + 2.f * d2.pr + 2.f * d3.pr + d4.pr);
+        pth += h6 * (d1.pth + 2.f * d2.pth + 2.f * d3.pth + d4.pth);
+        if (th < 1e-3f) { th = 1e-3f; pth = fabsf(pth); }
+        else if (th > kPi - 1e-3f) { th = kPi - 1e-3f; pth = -fabsf(pth); }
+        dLr = d4.r; dLth = d4.th; dLph = d4.ph;
+        haveDeriv = 1;
+        sth = M::sin_(th);
+        cth = M::cos_(th);
+        {
+            // slow light (trace.cuh:840-851)
+            float s = sth < 1e-4f ? 1e-4f : sth;
+            float a = K.a;
+            float Sig = r * r + a * a * cth * cth;
+            float Del = fmaxf(r * r - 2.f * K.M * r + a * a + K.Q * K.Q, 1e-5f);
+            float Pp = E * (r * r + a * a) - a * Lz;
+            float dtdl = M::div(a * (Lz - a * E * s * s) + M::div((r * r + a * a) * Pp, Del), fmaxf(Sig, 1e-8f));
+            tFlight += h * fabsf(dtdl);
+        }
+        if (K.disk)
+        {
+            float sp = M::sin_(ph), cp = M::cos_(ph);
+            v3 pos = mk3(r * sth * cp, r * cth, -r * sth * sp);
+            v3 stepv = pos - prevPos;
+            float sl = len3<M>(stepv);
+            if (sl > 1e-6f) dir3 = stepv * M::rcp(sl);
+            if (trans > 0.01f) diskSegment<M>(K.D, E, Lz, prevPos, pos, K.diskTime - tFlight, acc, trans);
+            prevPos = pos;
+        }
+        ++step;
+        return false;
+    }
+};
+
+template <class M> BHK_INL float rayMaxHviol(const SphRay<M>&) { return 0.f; }
+template <class M, bool W> BHK_INL float rayMaxHviol(const KerrRay<M, W>& R) { return R.maxHviol; }
+
+}  // namespace bhk

@@ -1,0 +1,378 @@
+// ---------------------------------------------------------------------------
+// bh_api.cpp -- C ABI entry points of libbhtrace.so.
+//
+// launchRenderPipeline / launchRenderKernel keep the reference's contract
+// (black_hole_kernel.cu:108-163): the parameter block is copied and
+// sanitized, work is enqueued asynchronously on the caller's stream, and the
+// return value is the launch error state (no synchronization, no throw).
+//
+// Library-owned device state (documented in the public header):
+//   * per-(device, stream) scratch: the per-pixel trace records and the ray
+//     queue counter, grown on demand and kept for the life of the process;
+//     guarded by a mutex, so concurrent streams never share scratch;
+//   * the schedule (workgroup size, RK4 steps per refill chunk, math mode),
+//     process-wide, read once per launch;
+//   * optional per-stream timing events.
+// ---------------------------------------------------------------------------
+#include <hip/hip_runtime.h>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
+#include "../csrc/bh_internal.h"
+
+namespace {
+
+struct Scratch
+{
+    BhRayRecord* rec = nullptr;
+    size_t recCap = 0;     // records
+    unsigned* counter = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool evValid = false;
+};
+
+std::mutex g_mu;
+std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
+BhSchedule g_sched = {256, 16, 1, 1};   // wgSize, stepsPerChunk, raysPerLane, mathMode
+std::atomic<int> g_timers{0};
+std::pair<int, hipStream_t> g_lastTimed{-1, nullptr};
+int g_numCU[64] = {0};
+
+int numCUs(int dev)
+{
+    if (dev < 0 || dev >= 64) return 256;
+    if (g_numCU[dev] == 0)
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) g_numCU[dev] = prop.multiProcessorCount;
+        else g_numCU[dev] = 256;
+    }
+    return g_numCU[dev];
+}
+
+// Scratch for the current device and `st`, holding at least `nRec` records.
+hipError_t getScratch(hipStream_t st, size_t nRec, Scratch** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Scratch& S = g_scratch[{dev, st}];
+    if (!S.counter)
+    {
+        e = hipMalloc(&S.counter, 256);
+        if (e != hipSuccess) { S.counter = nullptr; return e; }
+        e = hipMemset(S.counter, 0, 256);
+        if (e != hipSuccess) return e;
+    }
+    if (nRec > S.recCap)
+    {
+        if (S.rec)
+        {
+            // the old buffer may still be read by work queued on this stream
+            e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return e;
+            hipFree(S.rec);
+            S.rec = nullptr;
+            S.recCap = 0;
+        }
+        e = hipMalloc(&S.rec, nRec * sizeof(BhRayRecord));
+        if (e != hipSuccess) { S.rec = nullptr; return e; }
+        S.recCap = nRec;
+    }
+    *out = &S;
+    return hipSuccess;
+}
+
+BhSchedule currentSchedule()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_sched;
+}
+
+inline bool isKerr(int model) { return model == BH_KERR || model == BH_KERR_NEWMAN; }
+
+float2 jitterOf(int accumMode, int sampleIndex)
+{
+    // R2 sequence (post_process.cuh:47-53), evaluated without contraction
+    if (accumMode == 0 || sampleIndex <= 0) return make_float2(0.5f, 0.5f);
+    const float n = (float)sampleIndex;
+    const float x = 0.7548776662f * n + 0.5f, y = 0.5698402909f * n + 0.5f;
+    return make_float2(x - std::floor(x), y - std::floor(y));
+}
+
+hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, unsigned* counter, const BhSchedule& sc,
+                        hipStream_t st)
+{
+    bhk::TraceArgs A;
+    A.P = P;
+    A.rec = rec;
+    A.counter = counter;
+    A.tilesX = (P.width + 7) / 8;
+    const int tilesY = (P.height + 7) / 8;
+    A.totalIds = A.tilesX * tilesY * 64;
+    A.chunk = sc.stepsPerChunk;
+    A.jitter = jitterOf(P.accumMode, P.sampleIndex);
+    int dev = 0;
+    hipGetDevice(&dev);
+    const int kerr = isKerr(P.model) ? 1 : 0;
+    int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize) : bhk::traceOccupancyExact(kerr, sc.wgSize);
+    long long want = (long long)occ * numCUs(dev);
+    long long maxUseful = ((long long)A.totalIds + sc.wgSize - 1) / sc.wgSize;
+    bhk::LaunchShape S;
+    S.wgSize = sc.wgSize;
+    S.blocks = (int)(want < maxUseful ? want : maxUseful);
+    if (S.blocks < 1) S.blocks = 1;
+    hipError_t e = bhk::launchResetCounter(counter, st);
+    if (e != hipSuccess) return e;
+    return sc.mathMode ? bhk::launchTraceFast(A, S, st) : bhk::launchTraceExact(A, S, st);
+}
+
+void recordEvent(Scratch* S, int i, hipStream_t st)
+{
+    if (!S->evValid)
+    {
+        for (auto& ev : S->ev)
+            if (hipEventCreate(&ev) != hipSuccess) return;
+        S->evValid = true;
+    }
+    hipEventRecord(S->ev[i], st);
+}
+
+hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bloomB, const RenderParams& p,
+                          hipStream_t st, bool singlePass)
+{
+    RenderParams P = p;
+    bhSanitizeRenderParams(&P);
+    if (singlePass) P.bloomEnabled = 0;
+    if (!out || (!singlePass && !accum)) return hipErrorInvalidValue;
+    if (!singlePass && P.bloomEnabled && (!bloomA || !bloomB)) return hipErrorInvalidValue;
+    const size_t n = (size_t)P.width * (size_t)P.height;
+    Scratch* S = nullptr;
+    hipError_t e = getScratch(st, n, &S);
+    if (e != hipSuccess) return e;
+    const BhSchedule sc = currentSchedule();
+    const bool timed = g_timers.load() != 0;
+    if (timed) recordEvent(S, 0, st);
+    e = enqueueTrace(P, S->rec, S->counter, sc, st);
+    if (e != hipSuccess) return e;
+    if (timed) recordEvent(S, 1, st);
+    if (singlePass)
+    {
+        e = bhk::launchShadeFinalize(P, S->rec, out, st);
+        if (timed) { recordEvent(S, 2, st); recordEvent(S, 3, st); }
+    }
+    else
+    {
+        e = bhk::launchShadeBlend(P, S->rec, accum, st);
+        if (e != hipSuccess) return e;
+        if (timed) recordEvent(S, 2, st);
+        if (P.bloomEnabled)
+        {
+            e = bhk::launchBloomDownBlurH(P, accum, bloomB, st);
+            if (e != hipSuccess) return e;
+            e = bhk::launchBlurVComposite(P, accum, bloomB, bloomA, out, st);
+        }
+        else e = bhk::launchComposite(P, accum, out, st);
+        if (timed) recordEvent(S, 3, st);
+    }
+    if (timed)
+    {
+        int dev = 0;
+        hipGetDevice(&dev);
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_lastTimed = {dev, st};
+    }
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+}  // namespace
+
+struct BhFramebuffer
+{
+    int w, h;
+    uchar4* color;
+    float4* hdr;
+    float4* bloomA;
+    float4* bloomB;
+};
+
+extern "C" {
+
+bh_error_t launchRenderPipeline(bh_uchar4* out, bh_float4* accum, bh_float4* bloomA, bh_float4* bloomB,
+                                const RenderParams& p, bh_stream_t stream)
+{
+    return (bh_error_t)renderPipeline(reinterpret_cast<uchar4*>(out), reinterpret_cast<float4*>(accum),
+                                      reinterpret_cast<float4*>(bloomA), reinterpret_cast<float4*>(bloomB), p,
+                                      (hipStream_t)stream, false);
+}
+
+bh_error_t launchRenderKernel(bh_uchar4* out, const RenderParams& p, bh_stream_t stream)
+{
+    return (bh_error_t)renderPipeline(reinterpret_cast<uchar4*>(out), nullptr, nullptr, nullptr, p,
+                                      (hipStream_t)stream, true);
+}
+
+bh_error_t bhTraceRecords(BhRayRecord* devOut, const RenderParams& p, bh_stream_t stream)
+{
+    if (!devOut) return hipErrorInvalidValue;
+    RenderParams P = p;
+    bhSanitizeRenderParams(&P);
+    hipStream_t st = (hipStream_t)stream;
+    Scratch* S = nullptr;
+    hipError_t e = getScratch(st, 0, &S);
+    if (e != hipSuccess) return e;
+    e = enqueueTrace(P, devOut, S->counter, currentSchedule(), st);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+static bh_error_t probeCommon(BhProbe* devOut, const bh_float3* dirs, int n, int nx, int ny, const RenderParams& p,
+                              bh_stream_t stream)
+{
+    if (!devOut || n < 0) return hipErrorInvalidValue;
+    bhk::ProbeArgs A;
+    A.P = p;
+    A.out = devOut;
+    A.dirs = dirs;
+    A.n = n;
+    A.nx = nx;
+    A.ny = ny;
+    const BhSchedule sc = currentSchedule();
+    hipError_t e = sc.mathMode ? bhk::launchProbeFast(A, (hipStream_t)stream) : bhk::launchProbeExact(A, (hipStream_t)stream);
+    return (bh_error_t)e;
+}
+
+bh_error_t bhProbeGrid(BhProbe* devOut, const RenderParams& p, int nx, int ny, bh_stream_t stream)
+{
+    if (nx <= 0 || ny <= 0) return hipErrorInvalidValue;
+    return probeCommon(devOut, nullptr, nx * ny, nx, ny, p, stream);
+}
+
+bh_error_t bhProbeDirs(BhProbe* devOut, const bh_float3* devDirs, int n, const RenderParams& p, bh_stream_t stream)
+{
+    if (!devDirs) return hipErrorInvalidValue;
+    return probeCommon(devOut, devDirs, n, 0, 0, p, stream);
+}
+
+// ---------------------------------------------------------------- framebuffer
+bh_error_t bhFramebufferCreate(int width, int height, BhFramebuffer** fb)
+{
+    if (!fb || width < 1 || height < 1) return hipErrorInvalidValue;
+    *fb = nullptr;
+    BhFramebuffer* F = new (std::nothrow) BhFramebuffer{width, height, nullptr, nullptr, nullptr, nullptr};
+    if (!F) return hipErrorOutOfMemory;
+    const size_t n = (size_t)width * height, nh = (size_t)((width + 1) / 2) * ((height + 1) / 2);
+    hipError_t e = hipMalloc(&F->color, n * sizeof(uchar4));
+    if (e == hipSuccess) e = hipMalloc(&F->hdr, n * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&F->bloomA, nh * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&F->bloomB, nh * sizeof(float4));
+    if (e == hipSuccess) e = hipMemset(F->hdr, 0, n * sizeof(float4));
+    if (e != hipSuccess)
+    {
+        bhFramebufferDestroy(F);
+        return e;
+    }
+    *fb = F;
+    return hipSuccess;
+}
+
+void bhFramebufferDestroy(BhFramebuffer* fb)
+{
+    if (!fb) return;
+    hipFree(fb->color);
+    hipFree(fb->hdr);
+    hipFree(fb->bloomA);
+    hipFree(fb->bloomB);
+    delete fb;
+}
+
+bh_uchar4* bhFramebufferColor(BhFramebuffer* fb) { return fb ? reinterpret_cast<bh_uchar4*>(fb->color) : nullptr; }
+bh_float4* bhFramebufferHDR(BhFramebuffer* fb) { return fb ? reinterpret_cast<bh_float4*>(fb->hdr) : nullptr; }
+
+bh_error_t bhFramebufferRender(BhFramebuffer* fb, const RenderParams& p, bh_stream_t stream)
+{
+    if (!fb || p.width != fb->w || p.height != fb->h) return hipErrorInvalidValue;
+    return (bh_error_t)renderPipeline(fb->color, fb->hdr, fb->bloomA, fb->bloomB, p, (hipStream_t)stream, false);
+}
+
+bh_error_t bhFramebufferReadback(BhFramebuffer* fb, bh_uchar4* hostColor, bh_float4* hostHdr, bh_stream_t stream)
+{
+    if (!fb) return hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)fb->w * fb->h;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess && hostColor) e = hipMemcpy(hostColor, fb->color, n * sizeof(uchar4), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && hostHdr) e = hipMemcpy(hostHdr, fb->hdr, n * sizeof(float4), hipMemcpyDeviceToHost);
+    return e;
+}
+
+// ---------------------------------------------------------------- schedule / timers
+void bhGetSchedule(BhSchedule* s)
+{
+    if (s) *s = currentSchedule();
+}
+
+bh_error_t bhSetSchedule(const BhSchedule* s)
+{
+    if (!s) return hipErrorInvalidValue;
+    if (s->wgSize != 64 && s->wgSize != 128 && s->wgSize != 256) return hipErrorInvalidValue;
+    if (s->stepsPerChunk < 1 || s->stepsPerChunk > 4096) return hipErrorInvalidValue;
+    if (s->mathMode != 0 && s->mathMode != 1) return hipErrorInvalidValue;
+    if (s->raysPerLane != 1) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_sched = *s;
+    return hipSuccess;
+}
+
+void bhEnableKernelTimers(int enable) { g_timers.store(enable ? 1 : 0); }
+
+bh_error_t bhLastKernelTimes(float* traceMs, float* shadeMs, float* postMs)
+{
+    Scratch* S = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_scratch.find(g_lastTimed);
+        if (it == g_scratch.end() || !it->second.evValid) return hipErrorNotReady;
+        S = &it->second;
+    }
+    hipError_t e = hipEventSynchronize(S->ev[3]);
+    if (e != hipSuccess) return e;
+    float a = 0, b = 0, c = 0;
+    e = hipEventElapsedTime(&a, S->ev[0], S->ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b, S->ev[1], S->ev[2]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&c, S->ev[2], S->ev[3]);
+    if (traceMs) *traceMs = a;
+    if (shadeMs) *shadeMs = b;
+    if (postMs) *postMs = c;
+    return e;
+}
+
+// ---------------------------------------------------------------- plumbing
+bh_error_t bhDeviceAlloc(void** ptr, size_t bytes) { return hipMalloc(ptr, bytes); }
+bh_error_t bhDeviceFree(void* ptr) { return hipFree(ptr); }
+bh_error_t bhCopyToDevice(void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); }
+bh_error_t bhCopyToHost(void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost); }
+bh_error_t bhDeviceFill(void* dst, unsigned char value, size_t bytes) { return hipMemset(dst, value, bytes); }
+bh_error_t bhDeviceSynchronize(void) { return hipDeviceSynchronize(); }
+bh_error_t bhStreamCreate(bh_stream_t* s)
+{
+    hipStream_t st = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (s) *s = (bh_stream_t)st;
+    return e;
+}
+bh_error_t bhStreamDestroy(bh_stream_t s) { return hipStreamDestroy((hipStream_t)s); }
+bh_error_t bhStreamSynchronize(bh_stream_t s) { return hipStreamSynchronize((hipStream_t)s); }
+const char* bhErrorString(bh_error_t e) { return hipGetErrorString((hipError_t)e); }
+int bhDeviceCount(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+/*
+ * bh/bhtrace.h -- C ABI of the MI355X (gfx950) null-geodesic tracer library
+ * libbhtrace.so.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and never
+ * throws.  Device pointers are HIP device allocations (hipMalloc or
+ * bhDeviceAlloc below).  Error codes are HIP error codes (0 = success).
+ *
+ * The first two entry points are the drop-in replacements for the
+ * reference's launch API; the rest are new (headless framebuffer, probes,
+ * batched multi-view launch, scheduling, timing, learned surrogate).
+ *
+ * C++ callers get the reference's `const RenderParams&` form; C callers pass
+ * `const RenderParams*` (ABI-identical).
+ */
+#ifndef BH_BHTRACE_H
+#define BH_BHTRACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "render_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#define BH_PARAMS_IN const RenderParams&
+#else
+#define BH_PARAMS_IN const RenderParams*
+#endif
+
+typedef int   bh_error_t;    /* == hipError_t (int-sized enum, 0 = hipSuccess) */
+typedef void* bh_stream_t;   /* == hipStream_t (opaque; 0 = null stream)      */
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in launch API                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Full-quality frame: one sample per pixel blended into `accum`
+ * (float4[W*H], read-modify-write), optional bloom through the half-resolution
+ * scratch images bloomA/bloomB (float4[ceil(W/2)*ceil(H/2)] each), then
+ * exposure/ACES/gamma/dither into `out` (uchar4[W*H]).  `p` is copied and
+ * sanitized; the caller's copy is untouched.  Asynchronous on `stream`.
+ * Replaces: extern "C" cudaError_t launchRenderPipeline(uchar4*, float4*,
+ * float4*, float4*, const RenderParams&, cudaStream_t)
+ * (black_hole_kernel.cu:108-134). */
+bh_error_t launchRenderPipeline(bh_uchar4* out, bh_float4* accum,
+                                bh_float4* bloomA, bh_float4* bloomB,
+                                BH_PARAMS_IN p, bh_stream_t stream);
+
+/* Single-pass frame: centered sample, no accumulation, bloom forced off,
+ * straight to `out`.
+ * Replaces: extern "C" cudaError_t launchRenderKernel(uchar4*, const
+ * RenderParams&, cudaStream_t) (black_hole_kernel.cu:151-163). */
+bh_error_t launchRenderKernel(bh_uchar4* out, BH_PARAMS_IN p, bh_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side parameter helpers                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Reference defaults (render_params.h:24-86). */
+void bhDefaultRenderParams(RenderParams* p);
+
+/* In-place sanitation; returns 1 if any field was changed.
+ * Follows sanitizeRenderParams (metric.cuh:344-419). */
+int bhSanitizeRenderParams(RenderParams* p);
+
+typedef struct BHDerived
+{
+    float a, Q;             /* code units */
+    float rPlus, rMinus;    /* horizons (0 when naked) */
+    float rPhoton, rErgo;
+    float rIsco;            /* float scan, as the reference */
+    int   clamped, naked;
+} BHDerived;
+
+/* Clamp (M, a*, q) in place and derive horizons, photon orbit, ergosphere and
+ * ISCO.  Follows bhValidateAndDerive (metric.cuh:257-329). */
+void bhValidateAndDerive(int model, float* M, float* aStar, float* q,
+                         int allowNaked, BHDerived* out);
+
+/* FP64 roots (no reference counterpart; the reference scans in float). */
+double bhHorizonOuterFP64(double M, double a, double Q);
+double bhIscoFP64(double M, double a, double Q);   /* prograde equatorial ISCO */
+
+/* Orbit camera looking at the origin: follows fillCamera (main.cpp:186-228)
+ * with no body rotation, plus an explicit vertical field of view. */
+void bhFillCamera(RenderParams* p, float azimuthRad, float elevationRad,
+                  float distance, float fovYDegrees);
+
+/* Named workload presets (camera, model, disk and post settings).  `frame`
+ * selects the disk time where a workload has several frames.  Returns 0 on
+ * success, -1 for an unknown name.  Names:
+ *   "schw_256_balanced"      Schwarzschild 256x256, disk [3,12], bloom off
+ *   "kerr09_1080p_high"      Kerr a*=0.9 1920x1080, hot spots, bloom; frame 0,1,2 -> t=0,0.5,1
+ *   "kn_m07_q04_2160p_high"  Kerr-Newman a*=-0.7 q=0.4 3840x2160, bloom */
+int bhWorkloadParams(const char* name, int frame, RenderParams* out);
+
+/* ------------------------------------------------------------------------ */
+/* Probes (verification; follow probeGridKernel/probeOneKernel,             */
+/* test_main.cu:71-98)                                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct BhProbe
+{
+    int   outcome;      /* 0 captured, 1 escaped */
+    float escDir[3];
+    float maxHviol;     /* max |K|/(E^2 (r^2+a^2)) along the ray (rotating models) */
+    float lum;          /* accum.x + accum.y + accum.z */
+    int   steps;
+    float trans;
+} BhProbe;
+
+/* nx*ny rays through pixel centres of an nx x ny grid; devOut: BhProbe[nx*ny]. */
+bh_error_t bhProbeGrid(BhProbe* devOut, BH_PARAMS_IN p, int nx, int ny, bh_stream_t stream);
+/* n rays with explicit world directions devDirs[n]; devOut: BhProbe[n]. */
+bh_error_t bhProbeDirs(BhProbe* devOut, const bh_float3* devDirs, int n,
+                       BH_PARAMS_IN p, bh_stream_t stream);
+
+/* Per-pixel trace record of one frame (no shading): for checking exit
+ * directions against an FP64 trace.  devOut: BhRayRecord[W*H]. */
+typedef struct BhRayRecord
+{
+    float escDir[3];
+    int   flags;        /* bit0 escaped, bit1 captured, bits 8.. step count */
+    float accum[3];     /* disk radiance */
+    float trans;        /* remaining transmittance */
+} BhRayRecord;
+bh_error_t bhTraceRecords(BhRayRecord* devOut, BH_PARAMS_IN p, bh_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Headless framebuffer (replaces the Vulkan interop image)                 */
+/* ------------------------------------------------------------------------ */
+typedef struct BhFramebuffer BhFramebuffer;
+bh_error_t bhFramebufferCreate(int width, int height, BhFramebuffer** fb);
+void       bhFramebufferDestroy(BhFramebuffer* fb);
+bh_uchar4* bhFramebufferColor(BhFramebuffer* fb);   /* device uchar4[W*H] */
+bh_float4* bhFramebufferHDR(BhFramebuffer* fb);     /* device float4[W*H] (accum) */
+/* launchRenderPipeline into the framebuffer's own buffers (p.width/height
+ * must match the framebuffer). */
+bh_error_t bhFramebufferRender(BhFramebuffer* fb, BH_PARAMS_IN p, bh_stream_t stream);
+/* Synchronous copy to host; either pointer may be NULL. */
+bh_error_t bhFramebufferReadback(BhFramebuffer* fb, bh_uchar4* hostColor,
+                                 bh_float4* hostHdr, bh_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Scheduling                                                               */
+/* ------------------------------------------------------------------------ */
+typedef struct BhSchedule
+{
+    int wgSize;         /* threads per workgroup of the trace kernel: 64, 128, 256 */
+    int stepsPerChunk;  /* RK4 steps a wavefront runs between queue refills: 1..4096 */
+    int raysPerLane;    /* rays in flight per lane: 1 */
+    int mathMode;       /* 0 exact IEEE arithmetic, 1 hardware fast intrinsics */
+} BhSchedule;
+void bhGetSchedule(BhSchedule* s);
+bh_error_t bhSetSchedule(const BhSchedule* s);   /* -1 (hipErrorInvalidValue-like) on bad values */
+
+/* ------------------------------------------------------------------------ */
+/* Timing (HIP events recorded on the launch stream)                        */
+/* ------------------------------------------------------------------------ */
+/* When enabled, launchRenderPipeline/launchRenderKernel record events around
+ * the trace kernel and around the post kernels of each frame. */
+void bhEnableKernelTimers(int enable);
+/* Elapsed ms of the last timed frame: trace, sky shading, bloom+composite.
+ * Synchronizes on the recorded events. */
+bh_error_t bhLastKernelTimes(float* traceMs, float* shadeMs, float* postMs);
+
+/* ------------------------------------------------------------------------ */
+/* Device memory and stream utilities (plumbing for callers without HIP)    */
+/* ------------------------------------------------------------------------ */
+bh_error_t  bhDeviceAlloc(void** ptr, size_t bytes);
+bh_error_t  bhDeviceFree(void* ptr);
+bh_error_t  bhCopyToDevice(void* dst, const void* src, size_t bytes);
+bh_error_t  bhCopyToHost(void* dst, const void* src, size_t bytes);
+bh_error_t  bhDeviceFill(void* dst, unsigned char value, size_t bytes);
+bh_error_t  bhDeviceSynchronize(void);
+bh_error_t  bhStreamCreate(bh_stream_t* s);
+bh_error_t  bhStreamDestroy(bh_stream_t s);
+bh_error_t  bhStreamSynchronize(bh_stream_t s);
+const char* bhErrorString(bh_error_t e);
+int         bhDeviceCount(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BH_BHTRACE_H */

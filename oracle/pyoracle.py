@@ -1,0 +1,165 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker of the HIP path.  The product
+(black-hole_amd/) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_P = C.c_void_p  # pointer to a RenderParams-compatible 160-byte struct
+_vp = C.c_void_p
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    def __init__(self, path: str | None = None):
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise RuntimeError(f"oracle not built: {p} (run `make -C oracle`)")
+        L = C.CDLL(p)
+        sig = {
+            "bho_abi_version": (C.c_int, []),
+            "bho_sanitize": (C.c_int, [_P]),
+            "bho_validate_derive": (None, [C.c_int, _vp, _vp, _vp, C.c_int, _vp]),
+            "bho_isco_numeric": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+            "bho_trace_frame": (None, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+            "bho_count_flops": (None, [_P, C.c_int, C.c_int, C.c_int, _vp]),
+            "bho_render_pipeline": (None, [_P, _vp, _vp, C.c_int]),
+            "bho_render_kernel": (None, [_P, _vp, C.c_int]),
+            "bho_post": (None, [_P, _vp, _vp]),
+            "bho_probe_grid": (None, [_P, C.c_int, C.c_int, C.c_int, _vp, C.c_int]),
+            "bho_probe_dirs": (None, [_P, _vp, C.c_int, C.c_int, _vp, C.c_int]),
+            "bho_kerr_invariants": (None, [_P, _vp, C.c_int, _vp]),
+            "bho_hash12": (C.c_float, [C.c_float, C.c_float]),
+            "bho_hash33": (None, [_vp, _vp]),
+            "bho_vnoise": (C.c_float, [C.c_float, C.c_float]),
+            "bho_fbm": (C.c_float, [C.c_float, C.c_float]),
+            "bho_blackbody": (None, [C.c_float, _vp]),
+            "bho_aces": (None, [_vp, _vp]),
+            "bho_background": (None, [_vp, C.c_float, _vp]),
+            "bho_finalize": (None, [_vp, _vp, _P, C.c_int, C.c_int, _vp]),
+            "bho_emitter_g": (C.c_float, [C.c_float] * 6),
+            "bho_flux_weight": (C.c_float, [C.c_float, C.c_float]),
+            "bho_omega": (C.c_float, [C.c_float] * 4),
+            "bho_camera_energy": (C.c_float, [_P]),
+            "bho_jitter": (None, [C.c_int, C.c_int, _vp]),
+            "bho_photon_EL": (None, [_vp, _vp, C.c_float, C.c_float, C.c_float, _vp]),
+            "bho_camera_dir": (None, [_P, C.c_float, C.c_float, _vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        self.L = L
+
+    # --- frames
+    def trace_frame(self, P, mode=0, row_start=0, row_step=1, threads=0):
+        W, H = P.width, P.height
+        nr = 0 if row_start >= H else (H - row_start + row_step - 1) // row_step
+        hdr = np.zeros((nr, W, 4), np.float32)
+        esc = np.zeros((nr, W, 3), np.float64)
+        steps = np.zeros((nr, W), np.int32)
+        disk = np.zeros((nr, W, 4), np.float32)
+        self.L.bho_trace_frame(C.byref(P), mode, row_start, row_step, threads, _ptr(hdr), _ptr(esc), _ptr(steps),
+                               _ptr(disk))
+        return hdr, esc, steps, disk
+
+    def count_flops(self, P, row_start=0, row_step=1, threads=0):
+        out = np.zeros(5, np.uint64)
+        self.L.bho_count_flops(C.byref(P), row_start, row_step, threads, _ptr(out))
+        return dict(trace=int(out[0]), sky=int(out[1]), setup_blend=int(out[2]), finalize=int(out[3]),
+                    steps=int(out[4]))
+
+    def render_pipeline(self, P, accum=None, threads=0):
+        W, H = max(P.width, 1), max(P.height, 1)
+        acc = np.zeros((H, W, 4), np.float32) if accum is None else np.ascontiguousarray(accum, np.float32).copy()
+        out = np.zeros((H, W, 4), np.uint8)
+        self.L.bho_render_pipeline(C.byref(P), _ptr(acc), _ptr(out), threads)
+        return out, acc
+
+    def render_kernel(self, P, threads=0):
+        Q = P.copy()
+        self.L.bho_sanitize(C.byref(Q))
+        out = np.zeros((Q.height, Q.width, 4), np.uint8)
+        self.L.bho_render_kernel(C.byref(P), _ptr(out), threads)
+        return out
+
+    def post(self, P, accum):
+        acc = np.ascontiguousarray(accum, np.float32)
+        out = np.zeros((P.height, P.width, 4), np.uint8)
+        self.L.bho_post(C.byref(P), _ptr(acc), _ptr(out))
+        return out
+
+    # --- probes: records [outcome, ex, ey, ez, maxHviol, lum, steps, trans]
+    def probe_grid(self, P, nx, ny, mode=0, threads=0):
+        rec = np.zeros((ny * nx, 8), np.float64)
+        self.L.bho_probe_grid(C.byref(P), nx, ny, mode, _ptr(rec), threads)
+        return rec.reshape(ny, nx, 8)
+
+    def probe_dirs(self, P, dirs, mode=0, threads=0):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        rec = np.zeros((len(d), 8), np.float64)
+        self.L.bho_probe_dirs(C.byref(P), _ptr(d), len(d), mode, _ptr(rec), threads)
+        return rec
+
+    def kerr_invariants(self, P, d, mode=1):
+        dd = np.ascontiguousarray(d, np.float32)
+        out = np.zeros(4, np.float64)
+        self.L.bho_kerr_invariants(C.byref(P), _ptr(dd), mode, _ptr(out))
+        return dict(carter_drift=out[0], max_hviol=out[1], carter0=out[2], steps=int(out[3]))
+
+    # --- host parameter logic
+    def sanitize(self, P):
+        Q = P.copy()
+        fixed = self.L.bho_sanitize(C.byref(Q))
+        return Q, bool(fixed)
+
+    def validate_derive(self, model, M, a_star, q, allow_naked=False):
+        m, a, qq = C.c_float(M), C.c_float(a_star), C.c_float(q)
+        out = np.zeros(9, np.float32)
+        self.L.bho_validate_derive(model, C.byref(m), C.byref(a), C.byref(qq), int(allow_naked), _ptr(out))
+        keys = ["a", "Q", "rPlus", "rMinus", "rPhoton", "rErgo", "rIsco", "clamped", "naked"]
+        return dict(zip(keys, out.tolist())), m.value, a.value, qq.value
+
+    # --- scalar functions
+    def hash12(self, x, y):
+        return self.L.bho_hash12(x, y)
+
+    def hash33(self, p):
+        a = np.asarray(p, np.float32)
+        o = np.zeros(3, np.float32)
+        self.L.bho_hash33(_ptr(a), _ptr(o))
+        return o
+
+    def blackbody(self, k):
+        o = np.zeros(3, np.float32)
+        self.L.bho_blackbody(k, _ptr(o))
+        return o
+
+    def aces(self, rgb):
+        a = np.asarray(rgb, np.float32)
+        o = np.zeros(3, np.float32)
+        self.L.bho_aces(_ptr(a), _ptr(o))
+        return o
+
+    def background(self, d, gcam=1.0):
+        a = np.asarray(d, np.float32)
+        o = np.zeros(3, np.float32)
+        self.L.bho_background(_ptr(a), gcam, _ptr(o))
+        return o
+
+    def jitter(self, mode, idx):
+        o = np.zeros(2, np.float32)
+        self.L.bho_jitter(mode, idx, _ptr(o))
+        return o
