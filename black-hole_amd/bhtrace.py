@@ -63,8 +63,11 @@ class BhProbe(C.Structure):
 
 
 class BhSchedule(C.Structure):
-    _fields_ = [("wgSize", C.c_int), ("stepsPerChunk", C.c_int), ("raysPerLane", C.c_int),
-                ("mathMode", C.c_int)]
+    _fields_ = [("wgSize", C.c_int), ("stepsPerChunk", C.c_int), ("raysPerRefill", C.c_int),
+                ("mathMode", C.c_int), ("diskCoop", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 PROBE_DTYPE = np.dtype([("outcome", "<i4"), ("escDir", "<f4", (3,)), ("maxHviol", "<f4"),
@@ -313,7 +316,7 @@ def get_schedule() -> BhSchedule:
 
 
 def set_schedule(wg: int | None = None, chunk: int | None = None, math: int | None = None,
-                 rays_per_lane: int | None = None) -> BhSchedule:
+                 refill: int | None = None, coop: int | None = None) -> BhSchedule:
     s = get_schedule()
     if wg is not None:
         s.wgSize = wg
@@ -321,7 +324,13 @@ def set_schedule(wg: int | None = None, chunk: int | None = None, math: int | No
         s.stepsPerChunk = chunk
     if math is not None:
         s.mathMode = math
-    if rays_per_lane is not None:
-        s.raysPerLane = rays_per_lane
+    if refill is not None:
+        s.raysPerRefill = refill
+    if coop is not None:
+        s.diskCoop = coop
     check(load().bhSetSchedule(C.byref(s)), "bhSetSchedule")
     return s
+
+
+def restore_schedule(s: BhSchedule) -> None:
+    check(load().bhSetSchedule(C.byref(s)), "bhSetSchedule")

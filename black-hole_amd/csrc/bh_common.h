@@ -23,12 +23,15 @@
 namespace bhk {
 
 #define BHK_INL __device__ __forceinline__
+#define BHK_HD __host__ __device__ inline
 
 // ------------------------------------------------------------------ policies
 struct MathExact
 {
     static constexpr bool fast = false;
     BHK_INL static float div(float a, float b) { return a / b; }
+    // division by a per-frame constant b whose reciprocal invb is precomputed
+    BHK_INL static float divc(float a, float b, float) { return a / b; }
     BHK_INL static float rcp(float b) { return 1.0f / b; }
     BHK_INL static float sqrt_(float x) { return sqrtf(x); }
     BHK_INL static float rsq(float x) { return 1.0f / sqrtf(x); }
@@ -46,6 +49,7 @@ struct MathFast
     static constexpr bool fast = true;
     BHK_INL static float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
     BHK_INL static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    BHK_INL static float divc(float a, float, float invb) { return a * invb; }
     BHK_INL static float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
     BHK_INL static float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
     BHK_INL static float sin_(float x) { return __sinf(x); }
@@ -133,43 +137,22 @@ constexpr float kNtPeakInv = 1.f / 0.075f;
 constexpr float kSingCutOverM = 0.05f;
 
 // ------------------------------------------------------------------ metric (metric.cuh:63-199)
-template <class M> BHK_INL float omegaCircular(float r, float Mm, float a, float Q)
+// Omega = sqrt(M r - Q^2) / (r^2 + a sqrt(M r - Q^2)), 0 without a circular orbit.
+template <class M> BHK_INL float omegaCircular(float r, float Mm, float a, float Q2)
 {
-    float w2 = Mm * r - Q * Q;
+    float w2 = Mm * r - Q2;
     if (w2 <= 0.f) return 0.f;
     float sw = M::sqrt_(w2);
     return M::div(sw, r * r + a * sw);
 }
-// g = 1/[u^t (E - Omega Lz)] of a circular equatorial emitter (metric.cuh:154-185)
-template <class M> BHK_INL float emitterG(float r, float Mm, float a, float Q, float E, float Lz, float Om)
+BHK_HD float omegaCircularHost(float r, float Mm, float a, float Q2)
 {
-    float r2 = r * r;
-    float m2r = 2.f * Mm * r - Q * Q;
-    float gtt = -(1.f - M::div(m2r, r2));
-    float gtp = M::div(-a * m2r, r2);
-    float gpp = r2 + a * a + M::div(m2r * a * a, r2);
-    float den = -(gtt + 2.f * Om * gtp + Om * Om * gpp);
-    if (den < 1e-4f) return 0.f;
-    float ut = M::rcp(M::sqrt_(den));
-    float Eem = ut * (E - Om * Lz);
-    if (Eem < 0.04f) Eem = 0.04f;
-    float g = M::rcp(Eem);
-    if (g < 0.04f) g = 0.04f;
-    if (g > 6.f) g = 6.f;
-    return g;
+    float w2 = Mm * r - Q2;
+    if (w2 <= 0.f) return 0.f;
+    float sw = sqrtf(w2);
+    return sw / (r * r + a * sw);
 }
-// Novikov-Thorne flux weight with an inner peak (metric.cuh:190-199)
-template <class M> BHK_INL float fluxWeight(float r, float rIn)
-{
-    if (!(r > rIn) || rIn <= 0.f) return 0.f;
-    float x = M::div(rIn, r);
-    float nt = x * x * x * fmaxf(1.f - M::sqrt_(x), 0.f);
-    float dr = r - rIn * 1.15f;
-    float sig = 0.20f * rIn;
-    float peak = M::exp_(M::div(-0.5f * dr * dr, fmaxf(sig * sig, 1e-8f)));
-    return nt * (0.50f + 1.00f * peak);
-}
-BHK_INL float captureRadius(const RenderParams& P)
+BHK_HD float captureRadius(const RenderParams& P)
 {
     return (P.rPlus > 1e-6f) ? P.rPlus : kSingCutOverM * P.M;
 }
@@ -228,23 +211,76 @@ BHK_INL float wrapDeltaPhi(float d)
 }
 
 // ------------------------------------------------------------------ disk (trace.cuh:269-403)
+// Per-frame disk constants, computed once on the host with the same float
+// operations the reference evaluates per sample (so they are bit-identical)
+// and delivered as kernel arguments, i.e. scalar registers.
 struct DiskParams
 {
-    float rin, rout, invOut, maxH, M, a, Q, emisK, absK, spotS;
+    float rin, rout, invOut, maxH, M, a, Q2, twoM, emisK, absK, spotS;
+    float rinWall, routFade, rinSpot, rinPeak, sig2, invSig2;   // flux/density profile
+    float rHs0, rHs1, om0, om1, inv2sR0, inv2sR1;               // hot spots
     int spots;
 };
-BHK_INL DiskParams makeDisk(const RenderParams& P, float a, float Q)
+BHK_HD DiskParams makeDisk(const RenderParams& P, float a, float Q)
 {
     DiskParams D;
     D.rin = fmaxf(P.diskInner, 1e-4f);
     D.rout = P.diskOuter;
     D.invOut = 1.f / fmaxf(P.diskOuter, D.rin);
     D.maxH = 0.16f * P.diskOuter;
-    D.M = P.M; D.a = a; D.Q = Q;
-    D.emisK = P.diskEmisScale; D.absK = P.diskAbsScale;
+    D.M = P.M;
+    D.a = a;
+    D.Q2 = Q * Q;
+    D.twoM = 2.f * P.M;
+    D.emisK = P.diskEmisScale;
+    D.absK = P.diskAbsScale;
     D.spots = (P.hotSpotsEnabled && P.hotSpotStrength > 1e-4f) ? 1 : 0;
     D.spotS = P.hotSpotStrength;
+    D.rinWall = D.rin * 1.10f;
+    D.routFade = D.rout * 0.60f;
+    D.rinSpot = D.rin * 1.60f;
+    D.rinPeak = D.rin * 1.15f;
+    const float sig = 0.20f * D.rin;
+    D.sig2 = fmaxf(sig * sig, 1e-8f);
+    D.invSig2 = 1.f / D.sig2;
+    D.rHs0 = D.rin * 1.06f;
+    D.rHs1 = D.rin * 1.18f;
+    D.om0 = omegaCircularHost(D.rHs0, D.M, a, D.Q2);
+    D.om1 = omegaCircularHost(D.rHs1, D.M, a, D.Q2);
+    D.inv2sR0 = 1.f / (2.f * (0.09f * D.rHs0) * (0.09f * D.rHs0));
+    D.inv2sR1 = 1.f / (2.f * (0.11f * D.rHs1) * (0.11f * D.rHs1));
     return D;
+}
+
+// g = 1/[u^t (E - Omega Lz)] of a circular equatorial emitter (metric.cuh:154-185)
+template <class M> BHK_INL float emitterG(const DiskParams& D, float r, float E, float Lz, float Om)
+{
+    const float a = D.a;
+    float r2 = r * r;
+    float m2r = D.twoM * r - D.Q2;
+    float gtt = -(1.f - M::div(m2r, r2));
+    float gtp = M::div(-a * m2r, r2);
+    float gpp = r2 + a * a + M::div(m2r * a * a, r2);
+    float den = -(gtt + 2.f * Om * gtp + Om * Om * gpp);
+    if (den < 1e-4f) return 0.f;
+    float ut = M::rcp(M::sqrt_(den));
+    float Eem = ut * (E - Om * Lz);
+    if (Eem < 0.04f) Eem = 0.04f;
+    float g = M::rcp(Eem);
+    if (g < 0.04f) g = 0.04f;
+    if (g > 6.f) g = 6.f;
+    return g;
+}
+// Novikov-Thorne flux weight with an inner peak (metric.cuh:190-199)
+template <class M> BHK_INL float fluxWeight(const DiskParams& D, float r)
+{
+    const float rIn = D.rin;
+    if (!(r > rIn) || rIn <= 0.f) return 0.f;
+    float x = M::div(rIn, r);
+    float nt = x * x * x * fmaxf(1.f - M::sqrt_(x), 0.f);
+    float dr = r - D.rinPeak;
+    float peak = M::exp_(M::divc(-0.5f * dr * dr, D.sig2, D.invSig2));
+    return nt * (0.50f + 1.00f * peak);
 }
 
 // Radiance and optical depth of one volumetric sub-sample at p.  Returns
@@ -259,17 +295,17 @@ BHK_INL bool diskSubSample(const DiskParams& D, float E, float Lz, v3 p, float t
     float H = HR * rc;
     float dz = M::div(p.y, fmaxf(H, 1e-4f));
     if (fabsf(dz) > 2.6f) return false;
-    float Fr = fluxWeight<M>(rc, D.rin) * kNtPeakInv;
-    bool wantSpots = D.spots && rc < D.rin * 1.60f;
+    float Fr = fluxWeight<M>(D, rc) * kNtPeakInv;
+    bool wantSpots = D.spots && rc < D.rinSpot;
     if (Fr < 1e-6f && !wantSpots) return false;
 
     float dens = M::exp_(-dz * dz);
-    dens *= smoothstepf<M>(D.rin, D.rin * 1.10f, rc);
-    dens *= 1.0f - smoothstepf<M>(D.rout * 0.60f, D.rout, rc);
+    dens *= smoothstepf<M>(D.rin, D.rinWall, rc);
+    dens *= 1.0f - smoothstepf<M>(D.routFade, D.rout, rc);
 
-    float omega = omegaCircular<M>(rc, D.M, D.a, D.Q);
+    float omega = omegaCircular<M>(rc, D.M, D.a, D.Q2);
     if (omega <= 0.f) return false;
-    float g = emitterG<M>(rc, D.M, D.a, D.Q, E, Lz, omega);
+    float g = emitterG<M>(D, rc, E, Lz, omega);
     if (g <= 0.f) return false;
 
     float azim = M::atan2_(-p.z, p.x);
@@ -284,15 +320,11 @@ BHK_INL bool diskSubSample(const DiskParams& D, float E, float Lz, v3 p, float t
     float spotBoost = 0.f, spotHeat = 0.f;
     if (wantSpots)
     {
-        const float rHs0 = D.rin * 1.06f;
-        const float rHs1 = D.rin * 1.18f;
-        float om0 = omegaCircular<M>(rHs0, D.M, D.a, D.Q);
-        float om1 = omegaCircular<M>(rHs1, D.M, D.a, D.Q);
-        if (om0 <= 0.f) om0 = omega;
-        if (om1 <= 0.f) om1 = omega;
+        const float rHs0 = D.rHs0, rHs1 = D.rHs1;
+        const float om0 = D.om0 <= 0.f ? omega : D.om0;
+        const float om1 = D.om1 <= 0.f ? omega : D.om1;
         const float inv2sPhi = 1.f / (2.f * 0.18f * 0.18f);
-        const float inv2sR0 = M::rcp(2.f * (0.09f * rHs0) * (0.09f * rHs0));
-        const float inv2sR1 = M::rcp(2.f * (0.11f * rHs1) * (0.11f * rHs1));
+        const float inv2sR0 = D.inv2sR0, inv2sR1 = D.inv2sR1;
         float dph0 = wrapDeltaPhi(azim - (om0 * tEmit + 0.4f));
         float dph1 = wrapDeltaPhi(azim - (om1 * tEmit + 0.4f + kPi));
         float dr0 = rc - rHs0;
@@ -326,34 +358,131 @@ BHK_INL bool diskSubSample(const DiskParams& D, float E, float Lz, v3 p, float t
     return true;
 }
 
-// Chord [a, b] (trace.cuh:269-403): culls, then up to 12 sub-samples.
-template <class M>
-BHK_INL void diskSegment(const DiskParams& D, float E, float Lz, v3 a, v3 b, float tEmit, v3& accum, float& trans)
+// A disk chord [a, b] that survived the slab and radial culls
+// (trace.cuh:274-291): n sub-samples of length ds at emission time tEmit.
+struct Chord
 {
+    v3 a, b;
+    float tEmit, ds;
+    int n;
+};
+
+template <class M> BHK_INL int chordPrepare(const DiskParams& D, v3 a, v3 b, float tEmit, Chord& c)
+{
+    c.n = 0;
     v3 seg = b - a;
     float segLen = len3<M>(seg);
-    if (segLen < 1e-7f) return;
-    if (a.y > D.maxH && b.y > D.maxH) return;
-    if (a.y < -D.maxH && b.y < -D.maxH) return;
+    if (segLen < 1e-7f) return 0;
+    if (a.y > D.maxH && b.y > D.maxH) return 0;
+    if (a.y < -D.maxH && b.y < -D.maxH) return 0;
     float rcA = M::sqrt_(a.x * a.x + a.z * a.z);
     float rcB = M::sqrt_(b.x * b.x + b.z * b.z);
     float rcMin = fminf(rcA, rcB), rcMax = fmaxf(rcA, rcB);
-    if (rcMin - segLen > D.rout) return;
-    if (rcMax + segLen < D.rin) return;
+    if (rcMin - segLen > D.rout) return 0;
+    if (rcMax + segLen < D.rin) return 0;
     int nSub = 1 + (int)(segLen * 6.5f);
     if (nSub > 12) nSub = 12;
-    float fn = (float)nSub;
-    float ds = M::div(segLen, fn);
-    for (int j = 0; j < nSub; ++j)
+    c.a = a;
+    c.b = b;
+    c.tEmit = tEmit;
+    c.ds = M::div(segLen, (float)nSub);
+    c.n = nSub;
+    return nSub;
+}
+
+// Sub-sample j of a chord: position at t = (j + 1/2) / n along [a, b].
+template <class M>
+BHK_INL bool chordSubSample(const DiskParams& D, float E, float Lz, v3 a, v3 b, int j, int n, float tEmit,
+                            float ds, v3& add, float& tau)
+{
+    float t = M::div((float)j + 0.5f, (float)n);
+    v3 p = lerp3(a, b, t);
+    float dens;
+    return diskSubSample<M>(D, E, Lz, p, tEmit, ds, add, tau, dens);
+}
+
+// Emission-absorption update of one sub-sample; false once the chord's
+// transmittance has dropped below the early-out level (trace.cuh:396-401).
+template <class M> BHK_INL bool accumulateSample(v3 add, float tau, v3& accum, float& trans)
+{
+    accum += add * trans;
+    trans *= M::exp_(-tau);
+    return !(trans < 0.012f);
+}
+
+// One thread, one chord: the reference's sampleDiskSegment (trace.cuh:269-403).
+template <class M>
+BHK_INL void diskChordSerial(const DiskParams& D, float E, float Lz, const Chord& c, v3& accum, float& trans)
+{
+    for (int j = 0; j < c.n; ++j)
     {
-        float t = M::div((float)j + 0.5f, fn);
-        v3 p = lerp3(a, b, t);
         v3 add;
-        float tau, dens;
-        if (!diskSubSample<M>(D, E, Lz, p, tEmit, ds, add, tau, dens)) continue;
-        accum += add * trans;
-        trans *= M::exp_(-tau);
-        if (trans < 0.012f) return;
+        float tau;
+        if (!chordSubSample<M>(D, E, Lz, c.a, c.b, j, c.n, c.tEmit, c.ds, add, tau)) continue;
+        if (!accumulateSample<M>(add, tau, accum, trans)) return;
+    }
+}
+
+// Wavefront-cooperative chord sampling.  Every lane calls this (uniform
+// control flow).  The sub-samples of all lanes' chords are numbered in lane
+// order and spread over the 64 lanes, 64 per round; the owner of each chord
+// then folds its sub-samples into its own radiance in sub-sample order.
+// Each sub-sample is evaluated exactly as in diskChordSerial and folded in
+// the same order, so the result is bit-identical -- only the wavefront's
+// idle time changes (a lane with 12 sub-samples no longer stalls 63 others).
+template <class M>
+BHK_INL void diskChordWave(const DiskParams& D, bool has, const Chord& c, float E, float Lz, v3& accum, float& trans)
+{
+    if (__ballot(has) == 0ull) return;
+    const int lane = (int)__lane_id();
+    const int n = has ? c.n : 0;
+    int incl = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const int total = __shfl(incl, 63);
+    const int start = incl - n;
+    bool live = has;
+    for (int base = 0; base < total; base += 64)
+    {
+        const int k = base + lane;
+        // owner: first lane whose inclusive count exceeds k
+        int lo = 0, hi = 63;
+#pragma unroll
+        for (int it = 0; it < 6; ++it)
+        {
+            int mid = (lo + hi) >> 1;
+            int v = __shfl(incl, mid);
+            if (v > k) hi = mid;
+            else lo = mid + 1;
+        }
+        const int owner = lo;
+        v3 oa = mk3(__shfl(c.a.x, owner), __shfl(c.a.y, owner), __shfl(c.a.z, owner));
+        v3 ob = mk3(__shfl(c.b.x, owner), __shfl(c.b.y, owner), __shfl(c.b.z, owner));
+        const float oT = __shfl(c.tEmit, owner), oDs = __shfl(c.ds, owner);
+        const float oE = __shfl(E, owner), oL = __shfl(Lz, owner);
+        const int oStart = __shfl(start, owner), oN = __shfl(n, owner);
+        v3 add = mk3(0.f, 0.f, 0.f);
+        float tau = 0.f;
+        int valid = 0;
+        if (k < total) valid = chordSubSample<M>(D, oE, oL, oa, ob, k - oStart, oN, oT, oDs, add, tau) ? 1 : 0;
+        // owners fold this round's share of their sub-samples, in order
+        const int first = start > base ? start : base;
+        const int lastX = (start + n) < (base + 64) ? (start + n) : (base + 64);
+        const int cnt = (has && lastX > first) ? lastX - first : 0;
+        for (int t = 0;; ++t)
+        {
+            const bool want = t < cnt;
+            if (__ballot(want) == 0ull) break;
+            int src = want ? first + t - base : lane;
+            float ax = __shfl(add.x, src), ay = __shfl(add.y, src), az = __shfl(add.z, src);
+            float tu = __shfl(tau, src);
+            int vv = __shfl(valid, src);
+            if (want && live && vv) live = accumulateSample<M>(mk3(ax, ay, az), tu, accum, trans);
+        }
     }
 }
 

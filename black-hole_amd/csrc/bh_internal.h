@@ -5,23 +5,28 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "../../include/bh/bhtrace.h"
+#include "bh_rays.h"
 
 namespace bhk {
 
 struct TraceArgs
 {
     RenderParams P;         // sanitized
+    FrameK K;               // per-frame constants (filled by the launcher)
     BhRayRecord* rec;       // [W*H] per-pixel trace records
     unsigned* counter;      // ray queue head (zero before launch)
     int tilesX;             // ceil(W/8)
     int totalIds;           // ceil(W/8)*ceil(H/8)*64
-    int chunk;              // RK4 steps between refills
+    int chunk;              // max RK4 steps between refill checks
+    int refillMin;          // idle lanes that trigger a refill
+    int coop;               // 1: wavefront-cooperative disk sampling
     float2 jitter;          // sub-pixel sample position
 };
 
 struct ProbeArgs
 {
     RenderParams P;
+    FrameK K;               // filled by the launcher
     BhProbe* out;
     const bh_float3* dirs;  // explicit directions, or null for an nx*ny grid
     int n, nx, ny;
@@ -35,8 +40,8 @@ struct LaunchShape
 
 hipError_t launchTraceExact(const TraceArgs& A, const LaunchShape& S, hipStream_t st);
 hipError_t launchTraceFast(const TraceArgs& A, const LaunchShape& S, hipStream_t st);
-int traceOccupancyExact(int kerr, int wgSize);
-int traceOccupancyFast(int kerr, int wgSize);
+int traceOccupancyExact(int kerr, int wgSize, int coop);
+int traceOccupancyFast(int kerr, int wgSize, int coop);
 hipError_t launchProbeExact(const ProbeArgs& A, hipStream_t st);
 hipError_t launchProbeFast(const ProbeArgs& A, hipStream_t st);
 

@@ -19,35 +19,56 @@ namespace bhk {
 
 enum : int { RAY_RUNNING = 0, RAY_CAPTURED = 1, RAY_ESCAPED = 2 };
 
-// Per-frame constants shared by every ray of a launch.
+// Per-frame constants shared by every ray of a launch.  Built on the host
+// (makeFrame) with exactly the float operations the reference repeats per
+// step, so they are bit-identical, and passed by value as kernel arguments
+// where the wavefront keeps them in scalar registers.
 struct FrameK
 {
-    float M, a, Q, Q2;
-    float rCap, rFloor, uHor, escR, uEscape, invNineM;
-    float dPhi, hFloor, rPhoton, diskOuter, diskTime;
+    float M, a, Q, a2, Q2, twoM, threeM, twoQ2;
+    float rCap, rFloor, uHor, escR, uEscape, invNineM, uBudget;
+    float capR;                      // r <= capR: captured (trace.cuh:761)
+    float dPhi, hFloor, rPhoton, rPhotonRefine, diskOuter, diskOuterCap, diskTime, sixM, rStatsFloor;
+    float kRad, kAng, invKAng;       // adaptive-step budgets 8 dPhi and 2.2 dPhi (trace.cuh:793-795)
     int maxSteps, disk;
     DiskParams D;
 };
 
-BHK_INL FrameK makeFrame(const RenderParams& P)
+BHK_HD FrameK makeFrame(const RenderParams& P)
 {
     FrameK K;
-    K.M = P.M; K.a = P.aSpin; K.Q = P.Qc; K.Q2 = P.Qc * P.Qc;
+    const bool kerr = (P.model == BH_KERR || P.model == BH_KERR_NEWMAN);
+    K.M = P.M;
+    K.a = kerr ? P.aSpin : 0.f;
+    K.Q = P.Qc;
+    K.a2 = K.a * K.a;
+    K.Q2 = P.Qc * P.Qc;
+    K.twoM = 2.f * P.M;
+    K.threeM = 3.f * P.M;
+    K.twoQ2 = 2.f * K.Q2;
     K.rCap = captureRadius(P);
     K.rFloor = fmaxf(P.rPlus, K.rCap);
     K.uHor = 1.0f / fmaxf(K.rCap, 1e-6f);
     K.escR = 120.f * P.M;
     K.uEscape = 1.0f / K.escR;
     K.invNineM = 1.f / (9.f * P.M);
+    K.uBudget = 1.0f / fmaxf(1.5f * P.rPhoton, 3.f * P.M);
+    K.capR = K.rCap * 1.002f + 1e-3f;
     K.dPhi = P.dPhi;
     K.hFloor = P.dPhi * 0.03f;
     K.rPhoton = P.rPhoton;
+    K.rPhotonRefine = 1.5f * P.rPhoton;
     K.diskOuter = P.diskOuter;
+    K.diskOuterCap = P.diskOuter * 1.3f;
     K.diskTime = P.diskTime;
+    K.sixM = 6.f * P.M;
+    K.rStatsFloor = K.rFloor * 1.05f;
+    K.kRad = 8.0f * P.dPhi;
+    K.kAng = 2.2f * P.dPhi;
+    K.invKAng = 1.f / K.kAng;
     K.maxSteps = P.maxSteps;
     K.disk = P.diskEnabled != 0;
-    bool kerr = (P.model == BH_KERR || P.model == BH_KERR_NEWMAN);
-    K.D = makeDisk(P, kerr ? P.aSpin : 0.f, P.Qc);
+    K.D = makeDisk(P, K.a, P.Qc);
     return K;
 }
 
@@ -78,7 +99,7 @@ template <class M> struct SphRay
             return;
         }
         e2 = vdiv<M>(perp, pl);
-        float f0 = fmaxf(1.f - M::div(2.f * K.M, r0) + M::div(K.Q2, r0 * r0), 1e-6f);
+        float f0 = fmaxf(1.f - M::div(K.twoM, r0) + M::div(K.Q2, r0 * r0), 1e-6f);
         bImpact = fmaxf(M::div(r0 * pl, M::sqrt_(f0)), 1e-6f);
         u = M::rcp(r0);
         du = M::div(-M::sqrt_(f0) * ddr, r0 * pl);
@@ -90,16 +111,19 @@ template <class M> struct SphRay
 
     BHK_INL float acc_(const FrameK& K, float uu) const
     {
-        return 3.f * K.M * uu * uu - 2.f * K.Q2 * uu * uu * uu - uu;
+        return K.threeM * uu * uu - K.twoQ2 * uu * uu * uu - uu;
     }
 
-    // One RK4 step; returns true once the ray has terminated.
-    BHK_INL bool advance(const FrameK& K)
+    // One RK4 step; returns true once the ray has terminated.  The step's
+    // disk chord, if it survives the culls, is returned in `ch` (ch.n > 0)
+    // and must be sampled before the next step.
+    BHK_INL bool advance(const FrameK& K, Chord& ch)
     {
+        ch.n = 0;
         if (step >= K.maxSteps)
         {
             // budget exhausted (trace.cuh:603-609)
-            if (u > M::rcp(fmaxf(1.5f * K.rPhoton, 3.f * K.M))) status = RAY_CAPTURED;
+            if (u > K.uBudget) status = RAY_CAPTURED;
             else { status = RAY_ESCAPED; escDir = dir3; }
             return true;
         }
@@ -134,9 +158,9 @@ template <class M> struct SphRay
         float drdphi = M::div(-du, u * u);
         v3 pos = m * r;
         dir3 = normalize<M>(m * drdphi + mp * r);
-        float fR = fmaxf(1.f - M::div(2.f * K.M, r) + M::div(K.Q2, r * r), 1e-4f);
+        float fR = fmaxf(1.f - M::div(K.twoM, r) + M::div(K.Q2, r * r), 1e-4f);
         tFlight += M::div(h * (r * r), bImpact * fR);
-        if (K.disk && trans > 0.01f) diskSegment<M>(K.D, E, Lz, prevPos, pos, K.diskTime - tFlight, acc, trans);
+        if (K.disk && trans > 0.01f) chordPrepare<M>(K.D, prevPos, pos, K.diskTime - tFlight, ch);
         prevPos = pos;
         if (u < K.uEscape && du < 0.f) { status = RAY_ESCAPED; escDir = dir3; return true; }
         ++step;
@@ -154,16 +178,16 @@ BHK_INL KDeriv kerrRHS(const FrameK& K, float E, float Lz, float r, float pr, fl
 {
     if (s < 1e-4f) s = 1e-4f;
     float s2 = s * s;
-    float r2 = r * r, a = K.a, a2 = a * a;
+    float r2 = r * r, a = K.a, a2 = K.a2;
     float Sig = r2 + a2 * cth * cth;
-    float Del = r2 - 2.f * K.M * r + a2 + K.Q * K.Q;
+    float Del = r2 - K.twoM * r + a2 + K.Q2;
     if (Del < 1e-6f) Del = 1e-6f;
     float Pp = E * (r2 + a2) - a * Lz;
     float B = M::div(Lz, s) - a * E * s;
     float W = B * B - M::div(Pp * Pp, Del);
     float Kc = Del * pr * pr + pth * pth + W;
     if (kWant) Kout = Kc;
-    float dDel = 2.f * r - 2.f * K.M;
+    float dDel = 2.f * r - K.twoM;
     float dP = 2.f * r * E;
     float Wr = M::div(-(2.f * Pp * dP * Del - Pp * Pp * dDel), Del * Del);
     float dB = M::div(-Lz * cth, s2) - a * E * cth;
@@ -242,12 +266,13 @@ template <class M, bool kWant> struct KerrRay
         return normalize<M>(e1b * dr_ + e2b * (r * dth_) + e3b * (r * sth * dph_));
     }
 
-    BHK_INL bool advance(const FrameK& K)
+    BHK_INL bool advance(const FrameK& K, Chord& ch)
     {
+        ch.n = 0;
         if (step >= K.maxSteps)
         {
             // budget exhausted (trace.cuh:871-888)
-            if (r < 6.f * K.M) status = RAY_CAPTURED;
+            if (r < K.sixM) status = RAY_CAPTURED;
             else
             {
                 status = RAY_ESCAPED;
@@ -260,8 +285,8 @@ template <class M, bool kWant> struct KerrRay
         KDeriv d1 = kerrRHS<M, kWant>(K, E, Lz, r, pr, pth, sth, cth, Kv);
         if (!isfinite(r) || !isfinite(pr) || !isfinite(pth) || !isfinite(d1.r) || !isfinite(d1.pr))
         { status = RAY_CAPTURED; return true; }
-        if (r <= K.rCap * 1.002f + 1e-3f) { status = RAY_CAPTURED; return true; }
-        if (kWant && r > K.rFloor * 1.05f)
+        if (r <= K.capR) { status = RAY_CAPTURED; return true; }
+        if (kWant && r > K.rStatsFloor)
         {
             float viol = M::div(fabsf(Kv), E * E * (r * r + K.a * K.a) + 1e-12f);
             if (viol > maxHviol) maxHviol = viol;
@@ -273,35 +298,41 @@ template <class M, bool kWant> struct KerrRay
             return true;
         }
         // adaptive affine step (trace.cuh:793-809)
-        float h = M::rcp(M::div(fabsf(d1.r), 8.0f * K.dPhi * fmaxf(r - K.rFloor, 0.02f)) +
-                         M::div(fabsf(d1.th) + fabsf(d1.ph), 2.2f * K.dPhi) + 1e-5f);
-        if (r < 1.5f * K.rPhoton) h *= 0.55f;
+        float h = M::rcp(M::div(fabsf(d1.r), K.kRad * fmaxf(r - K.rFloor, 0.02f)) +
+                         M::divc(fabsf(d1.th) + fabsf(d1.ph), K.kAng, K.invKAng) + 1e-5f);
+        if (r < K.rPhotonRefine) h *= 0.55f;
         float spd = fabsf(d1.r) + r * (fabsf(d1.th) + fabsf(d1.ph)) + 1e-6f;
         float capLen;
-        if (K.disk && r < K.diskOuter * 1.3f) capLen = (r * fabsf(cth) < 1.8f) ? 1.2f : 2.2f;
+        if (K.disk && r < K.diskOuterCap) capLen = (r * fabsf(cth) < 1.8f) ? 1.2f : 2.2f;
         else capLen = fmaxf(1.2f, 0.22f * r);
         h = fminf(h, M::div(capLen, spd));
 
-        float hh = 0.5f * h;
+        // classic RK4; the weighted stage sum (d1 + 2 d2) + 2 d3 + d4 is
+        // accumulated as each stage completes (same operations, same order)
+        const float hh = 0.5f * h;
         float kd;
-        float y2th = th + hh * d1.th;
-        KDeriv d2 = kerrRHS<M, false>(K, E, Lz, r + hh * d1.r, pr + hh * d1.pr, pth + hh * d1.pth,
-                                      M::sin_(y2th), M::cos_(y2th), kd);
-        float y3th = th + hh * d2.th;
-        KDeriv d3 = kerrRHS<M, false>(K, E, Lz, r + hh * d2.r, pr + hh * d2.pr, pth + hh * d2.pth,
-                                      M::sin_(y3th), M::cos_(y3th), kd);
-        float y4th = th + h * d3.th;
-        KDeriv d4 = kerrRHS<M, false>(K, E, Lz, r + h * d3.r, pr + h * d3.pr, pth + h * d3.pth,
-                                      M::sin_(y4th), M::cos_(y4th), kd);
-        float h6 = M::div(h, 6.f);
-        r += h6 * (d1.r + 2.f * d2.r + 2.f * d3.r + d4.r);
-        th += h6 * (d1.th + 2.f * d2.th + 2.f * d3.th + d4.th);
-        ph += h6 * (d1.ph + 2.f * d2.ph + 2.f * d3.ph + d4.ph);
-        pr += h6 * (d1.pr + 2.f * d2.pr + 2.f * d3.pr + d4.pr);
-        pth += h6 * (d1.pth + 2.f * d2.pth + 2.f * d3.pth + d4.pth);
+        KDeriv sum = d1;
+        float yth = th + hh * d1.th;
+        KDeriv dk = kerrRHS<M, false>(K, E, Lz, r + hh * d1.r, pr + hh * d1.pr, pth + hh * d1.pth,
+                                      M::sin_(yth), M::cos_(yth), kd);
+        sum.r += 2.f * dk.r; sum.th += 2.f * dk.th; sum.ph += 2.f * dk.ph; sum.pr += 2.f * dk.pr; sum.pth += 2.f * dk.pth;
+        yth = th + hh * dk.th;
+        dk = kerrRHS<M, false>(K, E, Lz, r + hh * dk.r, pr + hh * dk.pr, pth + hh * dk.pth, M::sin_(yth),
+                               M::cos_(yth), kd);
+        sum.r += 2.f * dk.r; sum.th += 2.f * dk.th; sum.ph += 2.f * dk.ph; sum.pr += 2.f * dk.pr; sum.pth += 2.f * dk.pth;
+        yth = th + h * dk.th;
+        dk = kerrRHS<M, false>(K, E, Lz, r + h * dk.r, pr + h * dk.pr, pth + h * dk.pth, M::sin_(yth), M::cos_(yth),
+                               kd);
+        sum.r += dk.r; sum.th += dk.th; sum.ph += dk.ph; sum.pr += dk.pr; sum.pth += dk.pth;
+        const float h6 = M::div(h, 6.f);
+        r += h6 * sum.r;
+        th += h6 * sum.th;
+        ph += h6 * sum.ph;
+        pr += h6 * sum.pr;
+        pth += h6 * sum.pth;
         if (th < 1e-3f) { th = 1e-3f; pth = fabsf(pth); }
         else if (th > kPi - 1e-3f) { th = kPi - 1e-3f; pth = -fabsf(pth); }
-        dLr = d4.r; dLth = d4.th; dLph = d4.ph;
+        dLr = dk.r; dLth = dk.th; dLph = dk.ph;
         haveDeriv = 1;
         sth = M::sin_(th);
         cth = M::cos_(th);
@@ -309,10 +340,10 @@ template <class M, bool kWant> struct KerrRay
             // slow light (trace.cuh:840-851)
             float s = sth < 1e-4f ? 1e-4f : sth;
             float a = K.a;
-            float Sig = r * r + a * a * cth * cth;
-            float Del = fmaxf(r * r - 2.f * K.M * r + a * a + K.Q * K.Q, 1e-5f);
-            float Pp = E * (r * r + a * a) - a * Lz;
-            float dtdl = M::div(a * (Lz - a * E * s * s) + M::div((r * r + a * a) * Pp, Del), fmaxf(Sig, 1e-8f));
+            float Sig = r * r + K.a2 * cth * cth;
+            float Del = fmaxf(r * r - K.twoM * r + K.a2 + K.Q2, 1e-5f);
+            float Pp = E * (r * r + K.a2) - a * Lz;
+            float dtdl = M::div(a * (Lz - a * E * s * s) + M::div((r * r + K.a2) * Pp, Del), fmaxf(Sig, 1e-8f));
             tFlight += h * fabsf(dtdl);
         }
         if (K.disk)
@@ -322,7 +353,7 @@ template <class M, bool kWant> struct KerrRay
             v3 stepv = pos - prevPos;
             float sl = len3<M>(stepv);
             if (sl > 1e-6f) dir3 = stepv * M::rcp(sl);
-            if (trans > 0.01f) diskSegment<M>(K.D, E, Lz, prevPos, pos, K.diskTime - tFlight, acc, trans);
+            if (trans > 0.01f) chordPrepare<M>(K.D, prevPos, pos, K.diskTime - tFlight, ch);
             prevPos = pos;
         }
         ++step;

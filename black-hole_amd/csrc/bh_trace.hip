@@ -5,13 +5,16 @@
 // (MathFast, -ffp-contract=on).  Each object exports its own launchers.
 //
 // traceKernel: persistent wavefront-refill tracer.  Every lane owns one ray;
-// between chunks of `chunk` RK4 steps the lanes whose ray has terminated
-// write the ray's record (exit direction, outcome, disk radiance,
-// transmittance, step count) and take the next ray id from a device-wide
-// queue counter -- one atomic per wavefront per refill.  Ray ids walk the
+// a lane whose ray has terminated writes the ray's record (exit direction,
+// outcome, disk radiance, transmittance, step count) and later takes the
+// next ray id from a device-wide queue counter -- one atomic per wavefront
+// per refill.  Ray ids walk the
 // image in 8x8 pixel tiles so a wavefront's rays start spatially coherent.
 // Sky shading is deferred to a separate pixel-order kernel (bh_post.hip):
 // it runs once per ray and would otherwise execute with most lanes idle.
+// Disk chords of a step are sampled cooperatively by the whole wavefront
+// (diskChordWave) when kCoop is set.  Lanes whose ray ends write its record
+// at once; the wavefront refills only when `refillMin` lanes are idle.
 //
 // probeKernel: one ray per thread with constraint statistics (tests).
 // ---------------------------------------------------------------------------
@@ -46,35 +49,37 @@ BHK_INL void writeRecord(BhRayRecord* rec, int idx, const Ray& R)
     o[1] = make_float4(R.acc.x, R.acc.y, R.acc.z, R.trans);
 }
 
-template <class Ray>
+template <class Ray, bool kCoop>
 __global__ __launch_bounds__(256) void traceKernel(TraceArgs A)
 {
     const RenderParams& P = A.P;
-    const FrameK K = makeFrame(P);
+    const FrameK& K = A.K;
     const v3 cam = mk3(P.camPos.x, P.camPos.y, P.camPos.z);
     const unsigned lane = __lane_id();
     const unsigned total = (unsigned)A.totalIds;
     const int chunk = A.chunk;
+    const int refillMin = A.refillMin;
     const float2 jit = A.jitter;
 
     Ray R;
-    bool active = false, drained = false, done = false;
+    Chord ch;
+    ch.n = 0;
+    bool active = false, drained = false;
     int pix = 0;
     for (;;)
     {
-        bool need = !active && !drained;
-        unsigned long long needMask = __ballot(need);
+        // ---- refill: one queue atomic per wavefront for all idle lanes
+        const bool need = !active && !drained;
+        const unsigned long long needMask = __ballot(need);
         if (needMask)
         {
-            int leader = __ffsll((long long)needMask) - 1;
-            unsigned n = (unsigned)__popcll(needMask);
+            const int leader = __ffsll((long long)needMask) - 1;
             unsigned base = 0;
-            if ((int)lane == leader) base = atomicAdd(A.counter, n);
+            if ((int)lane == leader) base = atomicAdd(A.counter, (unsigned)__popcll(needMask));
             base = __shfl(base, leader);
             if (need)
             {
-                unsigned rank = (unsigned)__popcll(needMask & ((1ull << lane) - 1ull));
-                unsigned id = base + rank;
+                const unsigned id = base + (unsigned)__popcll(needMask & ((1ull << lane) - 1ull));
                 if (id >= total) drained = true;
                 else
                 {
@@ -82,32 +87,36 @@ __global__ __launch_bounds__(256) void traceKernel(TraceArgs A)
                     pixelOfId(id, A.tilesX, px, py);
                     if (px < P.width && py < P.height)
                     {
-                        float fx = Pol::div((float)px + jit.x, (float)P.width);
-                        float fy = Pol::div((float)py + jit.y, (float)P.height);
-                        v3 dir = cameraRayDir<Pol>(P, fx, fy);
-                        R.init(K, cam, dir);
+                        const float fx = Pol::div((float)px + jit.x, (float)P.width);
+                        const float fy = Pol::div((float)py + jit.y, (float)P.height);
+                        R.init(K, cam, cameraRayDir<Pol>(P, fx, fy));
                         pix = py * P.width + px;
-                        active = true;
-                        done = (R.status != RAY_RUNNING);
+                        if (R.status == RAY_RUNNING) active = true;
+                        else writeRecord(A.rec, pix, R);    // exactly radial ray
                     }
                 }
             }
         }
-        if (__ballot(active) == 0)
+        if (__ballot(active) == 0ull)
         {
-            if (__ballot(!drained) == 0) break;
+            if (__ballot(!drained) == 0ull) break;
             continue;
         }
+        // ---- up to `chunk` RK4 steps; leave early once enough lanes idle
         for (int k = 0; k < chunk; ++k)
         {
-            if (active && !done) done = R.advance(K);
-            if (__ballot(active && !done) == 0) break;
-        }
-        if (active && done)
-        {
-            writeRecord(A.rec, pix, R);
-            active = false;
-            done = false;
+            bool done = false;
+            if (active) done = R.advance(K, ch);
+            else ch.n = 0;
+            if (kCoop) diskChordWave<Pol>(K.D, active && ch.n > 0, ch, R.E, R.Lz, R.acc, R.trans);
+            else if (active && ch.n > 0) diskChordSerial<Pol>(K.D, R.E, R.Lz, ch, R.acc, R.trans);
+            if (done)
+            {
+                writeRecord(A.rec, pix, R);
+                active = false;
+            }
+            const unsigned long long idle = __ballot(!active && !drained);
+            if (__popcll(idle) >= refillMin || __ballot(active) == 0ull) break;
         }
     }
 }
@@ -118,7 +127,7 @@ __global__ __launch_bounds__(256) void probeKernel(ProbeArgs A)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
     const RenderParams& P = A.P;
-    const FrameK K = makeFrame(P);
+    const FrameK& K = A.K;
     v3 dir;
     if (A.dirs)
     {
@@ -134,8 +143,13 @@ __global__ __launch_bounds__(256) void probeKernel(ProbeArgs A)
     }
     Ray R;
     R.init(K, mk3(P.camPos.x, P.camPos.y, P.camPos.z), dir);
-    if (R.status == RAY_RUNNING)
-        while (!R.advance(K)) {}
+    Chord ch;
+    bool done = (R.status != RAY_RUNNING);
+    while (!done)
+    {
+        done = R.advance(K, ch);
+        if (ch.n > 0) diskChordSerial<Pol>(K.D, R.E, R.Lz, ch, R.acc, R.trans);
+    }
     BhProbe o;
     o.outcome = (R.status == RAY_ESCAPED) ? 1 : 0;
     o.escDir[0] = R.escDir.x; o.escDir[1] = R.escDir.y; o.escDir[2] = R.escDir.z;
@@ -146,17 +160,22 @@ __global__ __launch_bounds__(256) void probeKernel(ProbeArgs A)
     A.out[i] = o;
 }
 
-template <class Ray> static hipError_t launchTraceT(const TraceArgs& A, const LaunchShape& S, hipStream_t st)
+template <class Ray> static hipError_t launchTraceT(const TraceArgs& Ain, const LaunchShape& S, hipStream_t st)
 {
+    TraceArgs A = Ain;
+    A.K = makeFrame(A.P);
     dim3 block(S.wgSize), grid(S.blocks);
-    hipLaunchKernelGGL(traceKernel<Ray>, grid, block, 0, st, A);
+    if (A.coop) hipLaunchKernelGGL((traceKernel<Ray, true>), grid, block, 0, st, A);
+    else hipLaunchKernelGGL((traceKernel<Ray, false>), grid, block, 0, st, A);
     return hipGetLastError();
 }
 
-template <class Ray> static int occupancyT(int wgSize)
+template <class Ray> static int occupancyT(int wgSize, int coop)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceKernel<Ray>, wgSize, 0) != hipSuccess) return 1;
+    hipError_t e = coop ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceKernel<Ray, true>, wgSize, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceKernel<Ray, false>, wgSize, 0);
+    if (e != hipSuccess) return 1;
     return nb > 0 ? nb : 1;
 }
 
@@ -166,13 +185,15 @@ hipError_t BHK_SUFFIX(launchTrace)(const TraceArgs& A, const LaunchShape& S, hip
     return kerr ? launchTraceT<KerrRay<Pol, false>>(A, S, st) : launchTraceT<SphRay<Pol>>(A, S, st);
 }
 
-int BHK_SUFFIX(traceOccupancy)(int kerr, int wgSize)
+int BHK_SUFFIX(traceOccupancy)(int kerr, int wgSize, int coop)
 {
-    return kerr ? occupancyT<KerrRay<Pol, false>>(wgSize) : occupancyT<SphRay<Pol>>(wgSize);
+    return kerr ? occupancyT<KerrRay<Pol, false>>(wgSize, coop) : occupancyT<SphRay<Pol>>(wgSize, coop);
 }
 
-hipError_t BHK_SUFFIX(launchProbe)(const ProbeArgs& A, hipStream_t st)
+hipError_t BHK_SUFFIX(launchProbe)(const ProbeArgs& Ain, hipStream_t st)
 {
+    ProbeArgs A = Ain;
+    A.K = makeFrame(A.P);
     const bool kerr = (A.P.model == BH_KERR || A.P.model == BH_KERR_NEWMAN);
     dim3 block(128), grid((A.n + 127) / 128);
     if (A.n <= 0) return hipSuccess;

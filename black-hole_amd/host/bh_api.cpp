@@ -35,7 +35,7 @@ struct Scratch
 
 std::mutex g_mu;
 std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
-BhSchedule g_sched = {256, 16, 1, 1};   // wgSize, stepsPerChunk, raysPerLane, mathMode
+BhSchedule g_sched = {256, 4096, 64, 1, 0};   // wgSize, stepsPerChunk, raysPerRefill, mathMode, diskCoop
 std::atomic<int> g_timers{0};
 std::pair<int, hipStream_t> g_lastTimed{-1, nullptr};
 int g_numCU[64] = {0};
@@ -74,7 +74,7 @@ hipError_t getScratch(hipStream_t st, size_t nRec, Scratch** out)
             // the old buffer may still be read by work queued on this stream
             e = hipStreamSynchronize(st);
             if (e != hipSuccess) return e;
-            hipFree(S.rec);
+            (void)hipFree(S.rec);
             S.rec = nullptr;
             S.recCap = 0;
         }
@@ -114,11 +114,14 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, unsigned* count
     const int tilesY = (P.height + 7) / 8;
     A.totalIds = A.tilesX * tilesY * 64;
     A.chunk = sc.stepsPerChunk;
+    A.refillMin = sc.raysPerRefill;
+    A.coop = sc.diskCoop;
     A.jitter = jitterOf(P.accumMode, P.sampleIndex);
     int dev = 0;
-    hipGetDevice(&dev);
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     const int kerr = isKerr(P.model) ? 1 : 0;
-    int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize) : bhk::traceOccupancyExact(kerr, sc.wgSize);
+    int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, sc.diskCoop)
+                          : bhk::traceOccupancyExact(kerr, sc.wgSize, sc.diskCoop);
     long long want = (long long)occ * numCUs(dev);
     long long maxUseful = ((long long)A.totalIds + sc.wgSize - 1) / sc.wgSize;
     bhk::LaunchShape S;
@@ -138,7 +141,7 @@ void recordEvent(Scratch* S, int i, hipStream_t st)
             if (hipEventCreate(&ev) != hipSuccess) return;
         S->evValid = true;
     }
-    hipEventRecord(S->ev[i], st);
+    (void)hipEventRecord(S->ev[i], st);
 }
 
 hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bloomB, const RenderParams& p,
@@ -181,7 +184,7 @@ hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bl
     if (timed)
     {
         int dev = 0;
-        hipGetDevice(&dev);
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
         std::lock_guard<std::mutex> lk(g_mu);
         g_lastTimed = {dev, st};
     }
@@ -283,10 +286,10 @@ bh_error_t bhFramebufferCreate(int width, int height, BhFramebuffer** fb)
 void bhFramebufferDestroy(BhFramebuffer* fb)
 {
     if (!fb) return;
-    hipFree(fb->color);
-    hipFree(fb->hdr);
-    hipFree(fb->bloomA);
-    hipFree(fb->bloomB);
+    (void)hipFree(fb->color);
+    (void)hipFree(fb->hdr);
+    (void)hipFree(fb->bloomA);
+    (void)hipFree(fb->bloomB);
     delete fb;
 }
 
@@ -322,7 +325,8 @@ bh_error_t bhSetSchedule(const BhSchedule* s)
     if (s->wgSize != 64 && s->wgSize != 128 && s->wgSize != 256) return hipErrorInvalidValue;
     if (s->stepsPerChunk < 1 || s->stepsPerChunk > 4096) return hipErrorInvalidValue;
     if (s->mathMode != 0 && s->mathMode != 1) return hipErrorInvalidValue;
-    if (s->raysPerLane != 1) return hipErrorInvalidValue;
+    if (s->raysPerRefill < 1 || s->raysPerRefill > 64) return hipErrorInvalidValue;
+    if (s->diskCoop != 0 && s->diskCoop != 1) return hipErrorInvalidValue;
     std::lock_guard<std::mutex> lk(g_mu);
     g_sched = *s;
     return hipSuccess;

@@ -146,9 +146,11 @@ bh_error_t bhFramebufferReadback(BhFramebuffer* fb, bh_uchar4* hostColor,
 typedef struct BhSchedule
 {
     int wgSize;         /* threads per workgroup of the trace kernel: 64, 128, 256 */
-    int stepsPerChunk;  /* RK4 steps a wavefront runs between queue refills: 1..4096 */
-    int raysPerLane;    /* rays in flight per lane: 1 */
+    int stepsPerChunk;  /* max RK4 steps a wavefront runs between refill checks: 1..4096 */
+    int raysPerRefill;  /* rays a wavefront takes from the queue at once: it refills
+                           when this many of its 64 lanes are idle (1..64) */
     int mathMode;       /* 0 exact IEEE arithmetic, 1 hardware fast intrinsics */
+    int diskCoop;       /* 1: disk chords sampled cooperatively by the wavefront */
 } BhSchedule;
 void bhGetSchedule(BhSchedule* s);
 bh_error_t bhSetSchedule(const BhSchedule* s);   /* -1 (hipErrorInvalidValue-like) on bad values */

@@ -530,6 +530,24 @@ void bho_kerr_invariants(const RenderParams* Pp, const float* dir, int mode, dou
     }
 }
 
+// Final integrator state of one ray (double): out[0..7] = state (see
+// FinalState), out[8] outcome, out[9..11] exit direction, out[12] steps.
+void bho_final_state(const RenderParams* Pp, const double* dir, double* out13)
+{
+    const RenderParams P = *Pp;
+    V3<double> d = mk3(dir[0], dir[1], dir[2]);
+    V3<double> cam = mk3((double)P.camPos.x, (double)P.camPos.y, (double)P.camPos.z);
+    FinalState<double> f{};
+    TraceResult<double> t;
+    if (P.model == BH_KERR || P.model == BH_KERR_NEWMAN)
+        t = traceKerr<double, NoObserver<double>>(P, cam, d, false, NoObserver<double>(), &f);
+    else t = traceSpherical<double>(P, cam, d, &f);
+    for (int k = 0; k < 8; ++k) out13[k] = f.v[k];
+    out13[8] = t.escaped ? 1 : 0;
+    out13[9] = t.escDir.x; out13[10] = t.escDir.y; out13[11] = t.escDir.z;
+    out13[12] = t.steps;
+}
+
 // ------------------------------------------------------------ scalar exports
 float bho_hash12(float x, float y) { return hash12(mk2(x, y)); }
 void bho_hash33(const float* p, float* out)

@@ -523,7 +523,12 @@ inline void diskChord(const PT<Shade<T>>& PS, const DiskCtx<Shade<T>>& ctx, V3<T
 }
 
 // Spherical (Binet) integrator (trace.cuh:466-611)
-template <class T> inline TraceResult<T> traceSpherical(const RenderParams& Pr, V3<T> origin, V3<T> dir)
+// Final integrator state (tests): spherical u, u', phi; Kerr r, th, ph, p_r,
+// p_th, E, Lz.
+template <class T> struct FinalState { T v[8]; };
+
+template <class T>
+inline TraceResult<T> traceSpherical(const RenderParams& Pr, V3<T> origin, V3<T> dir, FinalState<T>* fin = nullptr)
 {
     using S = Shade<T>;
     PT<T> P(Pr);
@@ -614,6 +619,7 @@ template <class T> inline TraceResult<T> traceSpherical(const RenderParams& Pr, 
         if (u < uEscape && du < T(0.f)) { R.escaped = true; R.escDir = dir3; break; }
     }
     R.steps = step;
+    if (fin) { fin->v[0] = u; fin->v[1] = du; fin->v[2] = phi; fin->v[3] = bImpact; }
     if (!R.horizon && !R.escaped)
     {
         if (u > T(1.0f) / mfmax(T(1.5f) * P.rPhoton, T(3.f) * M)) R.horizon = true;
@@ -678,7 +684,7 @@ template <class T> struct NoObserver { void operator()(const KState<T>&, T, T) c
 
 template <class T, class Obs = NoObserver<T>>
 inline TraceResult<T> traceKerr(const RenderParams& Pr, V3<T> origin, V3<T> dir, bool wantStats,
-                                const Obs& obs = Obs())
+                                const Obs& obs = Obs(), FinalState<T>* fin = nullptr)
 {
     using S = Shade<T>;
     PT<T> P(Pr);
@@ -815,6 +821,11 @@ inline TraceResult<T> traceKerr(const RenderParams& Pr, V3<T> origin, V3<T> dir,
         }
     }
     R.steps = step;
+    if (fin)
+    {
+        fin->v[0] = y.r; fin->v[1] = y.th; fin->v[2] = y.ph; fin->v[3] = y.pr;
+        fin->v[4] = y.pth; fin->v[5] = E; fin->v[6] = Lz; fin->v[7] = T(0.f);
+    }
     if (!R.horizon && !R.escaped)
     {
         if (y.r < T(6.f) * M) R.horizon = true;

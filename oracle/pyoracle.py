@@ -56,6 +56,7 @@ class Oracle:
             "bho_jitter": (None, [C.c_int, C.c_int, _vp]),
             "bho_photon_EL": (None, [_vp, _vp, C.c_float, C.c_float, C.c_float, _vp]),
             "bho_camera_dir": (None, [_P, C.c_float, C.c_float, _vp]),
+            "bho_final_state": (None, [_P, _vp, _vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -118,6 +119,35 @@ class Oracle:
         out = np.zeros(4, np.float64)
         self.L.bho_kerr_invariants(C.byref(P), _ptr(dd), mode, _ptr(out))
         return dict(carter_drift=out[0], max_hviol=out[1], carter0=out[2], steps=int(out[3]))
+
+    def final_state(self, P, d):
+        dd = np.ascontiguousarray(d, np.float64)
+        out = np.zeros(13, np.float64)
+        self.L.bho_final_state(C.byref(P), _ptr(dd), _ptr(out))
+        return dict(state=out[:8].copy(), escaped=bool(out[8]), esc=out[9:12].copy(), steps=int(out[12]))
+
+    def emitter_g(self, r, M, a, Q, E, Lz):
+        return self.L.bho_emitter_g(r, M, a, Q, E, Lz)
+
+    def omega(self, r, M, a, Q):
+        return self.L.bho_omega(r, M, a, Q)
+
+    def flux_weight(self, r, r_in):
+        return self.L.bho_flux_weight(r, r_in)
+
+    def camera_energy(self, P):
+        return self.L.bho_camera_energy(C.byref(P))
+
+    def photon_EL(self, origin, d, M, a, Q):
+        o = np.zeros(2, np.float32)
+        self.L.bho_photon_EL(_ptr(np.asarray(origin, np.float32)), _ptr(np.asarray(d, np.float32)), M, a, Q, _ptr(o))
+        return o
+
+    def vnoise(self, x, y):
+        return self.L.bho_vnoise(x, y)
+
+    def fbm(self, x, y):
+        return self.L.bho_fbm(x, y)
 
     # --- host parameter logic
     def sanitize(self, P):

@@ -25,7 +25,7 @@ def say(*a):
 def default_schedule(gpu):
     s0 = gpu.get_schedule()
     yield
-    gpu.set_schedule(wg=s0.wgSize, chunk=s0.stepsPerChunk, math=s0.mathMode, rays_per_lane=s0.raysPerLane)
+    gpu.restore_schedule(s0)
 
 
 @pytest.mark.parametrize("math", MODES)

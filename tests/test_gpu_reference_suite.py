@@ -25,7 +25,7 @@ def exact_schedule(gpu):
     s0 = gpu.get_schedule()
     gpu.set_schedule(math=0)
     yield
-    gpu.set_schedule(wg=s0.wgSize, chunk=s0.stepsPerChunk, math=s0.mathMode, rays_per_lane=s0.raysPerLane)
+    gpu.restore_schedule(s0)
 
 
 def grid(gpu, P, nx=NX, ny=NY):
