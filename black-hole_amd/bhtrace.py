@@ -91,6 +91,9 @@ SIGNATURES = [
     ("bhIscoFP64", C.c_double, [C.c_double, C.c_double, C.c_double]),
     ("bhFillCamera", None, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
     ("bhWorkloadParams", C.c_int, [C.c_char_p, C.c_int, _P]),
+    ("bhWorkloadViews", C.c_int, [C.c_char_p, C.c_uint64, C.c_int, _P]),
+    ("bhRenderBatch", C.c_int, [C.c_int, _P, _vp, _vp, _vp, _vp, _vp]),
+    ("bhBatchViewLatency", C.c_int, [C.POINTER(C.c_float), C.c_int, _vp]),
     ("bhProbeGrid", C.c_int, [_vp, _P, C.c_int, C.c_int, _vp]),
     ("bhProbeDirs", C.c_int, [_vp, _vp, C.c_int, _P, _vp]),
     ("bhTraceRecords", C.c_int, [_vp, _P, _vp]),
@@ -151,6 +154,13 @@ def workload(name: str, frame: int = 0) -> RenderParams:
     if load().bhWorkloadParams(name.encode(), frame, C.byref(P)) != 0:
         raise ValueError(f"unknown workload {name!r} frame {frame}")
     return P
+
+
+def workload_views(name: str = "uneven_views", seed: int = 1, n: int = 64):
+    arr = (RenderParams * n)()
+    if load().bhWorkloadViews(name.encode(), seed, n, arr) != n:
+        raise ValueError(f"unknown view batch {name!r}")
+    return arr
 
 
 def defaults() -> RenderParams:
@@ -334,3 +344,35 @@ def set_schedule(wg: int | None = None, chunk: int | None = None, math: int | No
 
 def restore_schedule(s: BhSchedule) -> None:
     check(load().bhSetSchedule(C.byref(s)), "bhSetSchedule")
+
+
+class BatchBuffers:
+    """Device buffers for a batch of views (out, accum, bloomA, bloomB per view)."""
+
+    def __init__(self, views):
+        self.views = views
+        self.n = len(views)
+        self.bufs = []
+        for P in views:
+            W, H = P.width, P.height
+            nh = ((W + 1) // 2) * ((H + 1) // 2)
+            b = [DeviceBuffer(W * H * 4), DeviceBuffer(W * H * 16), DeviceBuffer(nh * 16), DeviceBuffer(nh * 16)]
+            b[1].fill(0)
+            self.bufs.append(b)
+        ptrs = lambda k: (C.c_void_p * self.n)(*[b[k].ptr.value for b in self.bufs])
+        self.outs, self.accs, self.bAs, self.bBs = ptrs(0), ptrs(1), ptrs(2), ptrs(3)
+
+    def render(self, stream=None):
+        check(load().bhRenderBatch(self.n, self.views, self.outs, self.accs, self.bAs, self.bBs, stream),
+              "bhRenderBatch")
+
+    def latency_ms(self, stream=None):
+        out = (C.c_float * self.n)()
+        check(load().bhBatchViewLatency(out, self.n, stream), "bhBatchViewLatency")
+        return list(out)
+
+    def readback(self, v):
+        P = self.views[v]
+        rgba = self.bufs[v][0].download(np.uint8).reshape(P.height, P.width, 4)
+        acc = self.bufs[v][1].download(np.float32).reshape(P.height, P.width, 4)
+        return rgba, acc

@@ -32,6 +32,33 @@ struct ProbeArgs
     int n, nx, ny;
 };
 
+// One view of a batched launch (device-resident descriptor table).
+struct ViewDesc
+{
+    RenderParams P;         // sanitized
+    FrameK K;
+    BhRayRecord* rec;       // per-pixel trace records of this view
+    uchar4* out;
+    float4* accum;
+    float4* bloomA;
+    float4* bloomB;
+    int tilesX;             // ceil(W/8)
+    int tileBase;           // first queue index of this view
+    int tileCount;          // ceil(W/8) * ceil(H/8)
+    int pad;
+    float2 jitter;
+};
+
+struct BatchArgs
+{
+    const ViewDesc* views;  // device table, ordered by tileBase
+    int nViews;
+    int totalTiles;
+    unsigned* counter;      // tile queue head
+    unsigned* viewDone;     // [nViews] finished tiles per view
+    unsigned long long* stamps;   // [nViews + 1]: completion per view, [nViews] = start
+};
+
 struct LaunchShape
 {
     int wgSize;
@@ -40,10 +67,14 @@ struct LaunchShape
 
 hipError_t launchTraceExact(const TraceArgs& A, const LaunchShape& S, hipStream_t st);
 hipError_t launchTraceFast(const TraceArgs& A, const LaunchShape& S, hipStream_t st);
-int traceOccupancyExact(int kerr, int wgSize, int coop);
-int traceOccupancyFast(int kerr, int wgSize, int coop);
+int traceOccupancyExact(int kerr, int wgSize, int coop, int tiles);
+int traceOccupancyFast(int kerr, int wgSize, int coop, int tiles);
 hipError_t launchProbeExact(const ProbeArgs& A, hipStream_t st);
 hipError_t launchProbeFast(const ProbeArgs& A, hipStream_t st);
+hipError_t launchTraceBatchExact(const BatchArgs& A, const LaunchShape& S, hipStream_t st);
+hipError_t launchTraceBatchFast(const BatchArgs& A, const LaunchShape& S, hipStream_t st);
+int traceBatchOccupancyExact(int wgSize);
+int traceBatchOccupancyFast(int wgSize);
 
 // post chain (bh_post.hip)
 hipError_t launchResetCounter(unsigned* counter, hipStream_t st);
@@ -58,5 +89,9 @@ hipError_t launchBlurVComposite(const RenderParams& P, const float4* accum, cons
                                 uchar4* out, hipStream_t st);
 // composite without bloom
 hipError_t launchComposite(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st);
+// batched post chain over a device ViewDesc table (max sizes bound the grids)
+hipError_t launchResetBatch(const BatchArgs& A, hipStream_t st);
+hipError_t launchShadeBlendBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);
+hipError_t launchBloomBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);
 
 }  // namespace bhk

@@ -1,5 +1,6 @@
 // ---------------------------------------------------------------------------
-// bh_post.hip -- per-pixel shading and post chain (exact IEEE arithmetic).
+// bh_post.hip -- per-pixel shading and post chain (exact IEEE arithmetic),
+// single-view kernels and their batched forms (one grid z-slice per view).
 //
 //   shade      trace record -> sky (4 star lattices + Milky-Way band, once per
 //              escaped ray) -> HDR sample -> accumulation blend   (pixel order,
@@ -35,17 +36,29 @@ BHK_INL v3 sampleFromRecord(const BhRayRecord* rec, int idx, float gCam)
     return hdr;
 }
 
-__global__ __launch_bounds__(256) void shadeBlendKernel(RenderParams P, const BhRayRecord* rec, float4* accum)
+BHK_INL void shadeBlendAt(const RenderParams& P, const BhRayRecord* rec, float4* accum, int idx, float gCam)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= P.width * P.height) return;
-    const float gCam = cameraEnergyFactor(P);
     v3 cur = sampleFromRecord(rec, idx, gCam);
     float4 pv = accum[idx];
     v3 prev = mk3(pv.x, pv.y, pv.z);
     if (!finite3(prev)) prev = cur;
     v3 c = accumBlend(prev, cur, P.accumMode, P.sampleIndex);
     accum[idx] = make_float4(c.x, c.y, c.z, 1.f);
+}
+
+__global__ __launch_bounds__(256) void shadeBlendKernel(RenderParams P, const BhRayRecord* rec, float4* accum)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P.width * P.height) return;
+    shadeBlendAt(P, rec, accum, idx, cameraEnergyFactor(P));
+}
+
+__global__ __launch_bounds__(256) void shadeBlendBatchKernel(const ViewDesc* views)
+{
+    const ViewDesc& V = views[blockIdx.z];
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= V.P.width * V.P.height) return;
+    shadeBlendAt(V.P, V.rec, V.accum, idx, cameraEnergyFactor(V.P));
 }
 
 __global__ __launch_bounds__(256) void shadeFinalizeKernel(RenderParams P, const BhRayRecord* rec, uchar4* out)
@@ -64,15 +77,15 @@ BHK_INL float gaussW(int k)
     return k == 0 ? 0.20236f : k == 1 ? 0.17925f : k == 2 ? 0.12458f : k == 3 ? 0.06794f : 0.02907f;
 }
 
-// One workgroup = one half-res row segment of kSegX outputs.
+// One workgroup = one half-res row segment of kSegX outputs: 2x2 box
+// average of the exposed accumulation, bright pass, then the horizontal
+// 9-tap blur from an LDS row with a 4-texel clamped halo on each side.
 constexpr int kSegX = 256;
-__global__ __launch_bounds__(kSegX) void bloomDownBlurHKernel(RenderParams P, const float4* accum, float4* dstH,
-                                                             int bw, int bh)
+BHK_INL void bloomRowSegment(const RenderParams& P, const float4* accum, float4* dstH, int bw, int bh, int seg, int y,
+                             float3* row)
 {
 #pragma clang fp contract(off)
-    __shared__ float3 row[kSegX + 8];
-    const int y = blockIdx.y;
-    const int x0 = blockIdx.x * kSegX;
+    const int x0 = seg * kSegX;
     const int W = P.width, H = P.height;
     const int yA = y * 2, yB = (yA + 1 < H) ? yA + 1 : yA;
     for (int t = threadIdx.x; t < kSegX + 8; t += kSegX)
@@ -101,78 +114,133 @@ __global__ __launch_bounds__(kSegX) void bloomDownBlurHKernel(RenderParams P, co
     dstH[y * bw + x] = make_float4(s.x, s.y, s.z, 1.f);
 }
 
-// Output tile of kTW x kTH full-resolution pixels.
+__global__ __launch_bounds__(kSegX) void bloomDownBlurHKernel(RenderParams P, const float4* accum, float4* dstH,
+                                                             int bw, int bh)
+{
+    __shared__ float3 row[kSegX + 8];
+    bloomRowSegment(P, accum, dstH, bw, bh, blockIdx.x, blockIdx.y, row);
+}
+
+__global__ __launch_bounds__(kSegX) void bloomBatchKernel(const ViewDesc* views)
+{
+    __shared__ float3 row[kSegX + 8];
+    const ViewDesc& V = views[blockIdx.z];
+    if (!V.P.bloomEnabled) return;
+    const int bw = (V.P.width + 1) / 2, bh = (V.P.height + 1) / 2;
+    if ((int)blockIdx.x * kSegX >= bw || (int)blockIdx.y >= bh) return;
+    bloomRowSegment(V.P, V.accum, V.bloomB, bw, bh, blockIdx.x, blockIdx.y, row);
+}
+
+// Output tile of kTW x kTH full-resolution pixels: vertical 9-tap blur of
+// the horizontally blurred half-res image into LDS, then bilinear
+// upsampling and the tonemap of each pixel.  Without bloom the tile is a
+// plain per-pixel tonemap.
 constexpr int kTW = 64, kTH = 16;
 constexpr int kHX = kTW / 2 + 2;      // half-res columns touched by the bilinear taps
 constexpr int kHY = kTH / 2 + 2;      // half-res rows touched
 constexpr int kBY = kHY + 8;          // rows of the horizontally blurred image needed
 
-__global__ __launch_bounds__(256) void blurVCompositeKernel(RenderParams P, const float4* accum, const float4* srcH,
-                                                            float4* dstV, uchar4* out, int bw, int bh)
+struct CompositeLds
+{
+    float3 tileH[kBY][kHX];
+    float3 tileV[kHY][kHX];
+};
+
+BHK_INL void compositeTile(const RenderParams& P, const float4* accum, const float4* srcH, float4* dstV, uchar4* out,
+                           int tx, int ty, CompositeLds& L)
 {
 #pragma clang fp contract(off)
-    __shared__ float3 tileH[kBY][kHX];
-    __shared__ float3 tileV[kHY][kHX];
-    const int X0 = blockIdx.x * kTW, Y0 = blockIdx.y * kTH;
+    const int X0 = tx * kTW, Y0 = ty * kTH;
+    const int bw = (P.width + 1) / 2, bh = (P.height + 1) / 2;
     const int hx0 = X0 / 2 - 1, hy0 = Y0 / 2 - 1;   // first half-res tap (unclamped)
     const int by0 = hy0 - 4;
-    for (int t = threadIdx.x; t < kBY * kHX; t += blockDim.x)
+    const bool bloom = P.bloomEnabled != 0;
+    if (bloom)
     {
-        int ry = t / kHX, cx = t % kHX;
-        int yy = by0 + ry, xx = hx0 + cx;
-        yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
-        xx = xx < 0 ? 0 : (xx >= bw ? bw - 1 : xx);
-        float4 v = srcH[yy * bw + xx];
-        tileH[ry][cx] = make_float3(v.x, v.y, v.z);
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < kHY * kHX; t += blockDim.x)
-    {
-        int ry = t / kHX, cx = t % kHX;
-        int hy = hy0 + ry;
-        int hyc = hy < 0 ? 0 : (hy >= bh ? bh - 1 : hy);
-        v3 s = mk3(0.f, 0.f, 0.f);
-#pragma unroll
-        for (int k = -4; k <= 4; ++k)
+        for (int t = threadIdx.x; t < kBY * kHX; t += blockDim.x)
         {
-            int yy = hyc + k;
+            int ry = t / kHX, cx = t % kHX;
+            int yy = by0 + ry, xx = hx0 + cx;
             yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
-            float3 v = tileH[yy - by0][cx];
-            s += mk3(v.x, v.y, v.z) * gaussW(k);
+            xx = xx < 0 ? 0 : (xx >= bw ? bw - 1 : xx);
+            float4 v = srcH[yy * bw + xx];
+            L.tileH[ry][cx] = make_float3(v.x, v.y, v.z);
         }
-        tileV[ry][cx] = make_float3(s.x, s.y, s.z);
-        int hx = hx0 + cx;
-        if (dstV && hy == hyc && hx >= 0 && hx < bw && ry >= 1 && ry <= kHY - 2 && cx >= 1 && cx <= kHX - 2)
-            dstV[hy * bw + hx] = make_float4(s.x, s.y, s.z, 1.f);
+        __syncthreads();
+        for (int t = threadIdx.x; t < kHY * kHX; t += blockDim.x)
+        {
+            int ry = t / kHX, cx = t % kHX;
+            int hy = hy0 + ry;
+            int hyc = hy < 0 ? 0 : (hy >= bh ? bh - 1 : hy);
+            v3 s = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = -4; k <= 4; ++k)
+            {
+                int yy = hyc + k;
+                yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
+                float3 v = L.tileH[yy - by0][cx];
+                s += mk3(v.x, v.y, v.z) * gaussW(k);
+            }
+            L.tileV[ry][cx] = make_float3(s.x, s.y, s.z);
+            int hx = hx0 + cx;
+            if (dstV && hy == hyc && hx >= 0 && hx < bw && ry >= 1 && ry <= kHY - 2 && cx >= 1 && cx <= kHX - 2)
+                dstV[hy * bw + hx] = make_float4(s.x, s.y, s.z, 1.f);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     for (int t = threadIdx.x; t < kTW * kTH; t += blockDim.x)
     {
         int px = X0 + (t % kTW), py = Y0 + (t / kTW);
         if (px >= P.width || py >= P.height) continue;
-        float x = ((float)px + 0.5f) * 0.5f - 0.5f;
-        float y = ((float)py + 0.5f) * 0.5f - 0.5f;
-        int ix = (int)floorf(x), iy = (int)floorf(y);
-        float tx = x - (float)ix, ty = y - (float)iy;
-        auto tap = [&](int xx, int yy) {
-            xx = xx < 0 ? 0 : (xx >= bw ? bw - 1 : xx);
-            yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
-            float3 v = tileV[yy - hy0][xx - hx0];
-            return mk3(v.x, v.y, v.z);
-        };
-        v3 c00 = tap(ix, iy), c10 = tap(ix + 1, iy), c01 = tap(ix, iy + 1), c11 = tap(ix + 1, iy + 1);
-        v3 bloom = lerp3(lerp3(c00, c10, tx), lerp3(c01, c11, tx), ty);
+        v3 bl = mk3(0.f, 0.f, 0.f);
+        if (bloom)
+        {
+            float x = ((float)px + 0.5f) * 0.5f - 0.5f;
+            float y = ((float)py + 0.5f) * 0.5f - 0.5f;
+            int ix = (int)floorf(x), iy = (int)floorf(y);
+            float fx = x - (float)ix, fy = y - (float)iy;
+            auto tap = [&](int xx, int yy) {
+                xx = xx < 0 ? 0 : (xx >= bw ? bw - 1 : xx);
+                yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
+                float3 v = L.tileV[yy - hy0][xx - hx0];
+                return mk3(v.x, v.y, v.z);
+            };
+            v3 c00 = tap(ix, iy), c10 = tap(ix + 1, iy), c01 = tap(ix, iy + 1), c11 = tap(ix + 1, iy + 1);
+            bl = lerp3(lerp3(c00, c10, fx), lerp3(c01, c11, fx), fy);
+        }
         float4 av = accum[py * P.width + px];
-        out[py * P.width + px] = finalizePixel(mk3(av.x, av.y, av.z), bloom, P, px, py);
+        out[py * P.width + px] = finalizePixel(mk3(av.x, av.y, av.z), bl, P, px, py);
     }
 }
 
-__global__ __launch_bounds__(256) void compositeKernel(RenderParams P, const float4* accum, uchar4* out)
+__global__ __launch_bounds__(256) void blurVCompositeKernel(RenderParams P, const float4* accum, const float4* srcH,
+                                                            float4* dstV, uchar4* out)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= P.width * P.height) return;
-    float4 av = accum[idx];
-    out[idx] = finalizePixel(mk3(av.x, av.y, av.z), mk3(0.f, 0.f, 0.f), P, idx % P.width, idx / P.width);
+    __shared__ CompositeLds L;
+    compositeTile(P, accum, srcH, dstV, out, blockIdx.x, blockIdx.y, L);
+}
+
+__global__ __launch_bounds__(256) void compositeBatchKernel(const ViewDesc* views)
+{
+    __shared__ CompositeLds L;
+    const ViewDesc& V = views[blockIdx.z];
+    if ((int)blockIdx.x * kTW >= V.P.width || (int)blockIdx.y * kTH >= V.P.height) return;
+    compositeTile(V.P, V.accum, V.bloomB, nullptr, V.out, blockIdx.x, blockIdx.y, L);
+}
+
+__global__ void resetBatchKernel(BatchArgs A)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
+    {
+        A.counter[0] = 0u;
+        A.stamps[A.nViews] = (unsigned long long)wall_clock64();
+    }
+    if (i < A.nViews)
+    {
+        A.viewDone[i] = 0u;
+        A.stamps[i] = 0ull;
+    }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -209,16 +277,39 @@ hipError_t launchBloomDownBlurH(const RenderParams& P, const float4* accum, floa
 hipError_t launchBlurVComposite(const RenderParams& P, const float4* accum, const float4* srcH, float4* dstV,
                                 uchar4* out, hipStream_t st)
 {
-    int bw = (P.width + 1) / 2, bh = (P.height + 1) / 2;
     dim3 grid(blocksFor(P.width, kTW), blocksFor(P.height, kTH));
-    hipLaunchKernelGGL(blurVCompositeKernel, grid, dim3(256), 0, st, P, accum, srcH, dstV, out, bw, bh);
+    hipLaunchKernelGGL(blurVCompositeKernel, grid, dim3(256), 0, st, P, accum, srcH, dstV, out);
     return hipGetLastError();
 }
 
 hipError_t launchComposite(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st)
 {
-    long long n = (long long)P.width * P.height;
-    hipLaunchKernelGGL(compositeKernel, dim3(blocksFor(n, 256)), dim3(256), 0, st, P, accum, out);
+    RenderParams Q = P;
+    Q.bloomEnabled = 0;
+    return launchBlurVComposite(Q, accum, nullptr, nullptr, out, st);
+}
+
+hipError_t launchResetBatch(const BatchArgs& A, hipStream_t st)
+{
+    hipLaunchKernelGGL(resetBatchKernel, dim3(blocksFor(A.nViews + 1, 64)), dim3(64), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t launchShadeBlendBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st)
+{
+    dim3 grid(blocksFor((long long)maxW * maxH, 256), 1, nViews);
+    hipLaunchKernelGGL(shadeBlendBatchKernel, grid, dim3(256), 0, st, views);
+    return hipGetLastError();
+}
+
+hipError_t launchBloomBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st)
+{
+    const int bw = (maxW + 1) / 2, bh = (maxH + 1) / 2;
+    hipLaunchKernelGGL(bloomBatchKernel, dim3(blocksFor(bw, kSegX), bh, nViews), dim3(kSegX), 0, st, views);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(compositeBatchKernel, dim3(blocksFor(maxW, kTW), blocksFor(maxH, kTH), nViews), dim3(256), 0,
+                       st, views);
     return hipGetLastError();
 }
 

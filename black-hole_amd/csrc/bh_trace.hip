@@ -121,6 +121,102 @@ __global__ __launch_bounds__(256) void traceKernel(TraceArgs A)
     }
 }
 
+// ---------------------------------------------------------------- whole tiles
+// One wavefront traces one 8x8 tile to completion, then takes the next tile
+// from the queue (no mid-flight refill: the rays of a tile cross the disk
+// during the same steps, which keeps the wavefront coherent).  Used by the
+// single-view launch when raysPerRefill == 64 and by the batched launch.
+template <class Ray, bool kCoop>
+BHK_INL void traceTile(const RenderParams& P, const FrameK& K, BhRayRecord* rec, int tilesX, float2 jit,
+                       unsigned tile, unsigned lane)
+{
+    const int px = (int)((tile % (unsigned)tilesX) * 8u + (lane & 7u));
+    const int py = (int)((tile / (unsigned)tilesX) * 8u + (lane >> 3));
+    Ray R;
+    Chord ch;
+    ch.n = 0;
+    bool active = false;
+    int pix = 0;
+    if (px < P.width && py < P.height)
+    {
+        const float fx = Pol::div((float)px + jit.x, (float)P.width);
+        const float fy = Pol::div((float)py + jit.y, (float)P.height);
+        R.init(K, mk3(P.camPos.x, P.camPos.y, P.camPos.z), cameraRayDir<Pol>(P, fx, fy));
+        pix = py * P.width + px;
+        if (R.status == RAY_RUNNING) active = true;
+        else writeRecord(rec, pix, R);
+    }
+    while (__ballot(active) != 0ull)
+    {
+        bool done = false;
+        if (active) done = R.advance(K, ch);
+        else ch.n = 0;
+        if (kCoop) diskChordWave<Pol>(K.D, active && ch.n > 0, ch, R.E, R.Lz, R.acc, R.trans);
+        else if (active && ch.n > 0) diskChordSerial<Pol>(K.D, R.E, R.Lz, ch, R.acc, R.trans);
+        if (done)
+        {
+            writeRecord(rec, pix, R);
+            active = false;
+        }
+    }
+}
+
+BHK_INL unsigned nextTile(unsigned* counter, unsigned lane)
+{
+    unsigned t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1u);
+    return (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl(t, 0));
+}
+
+template <class Ray, bool kCoop>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Pol::fast ? 4 : 3))) void traceTileKernel(
+    TraceArgs A)
+{
+    const unsigned lane = __lane_id();
+    const unsigned nTiles = (unsigned)A.totalIds / 64u;
+    for (;;)
+    {
+        const unsigned t = nextTile(A.counter, lane);
+        if (t >= nTiles) break;
+        traceTile<Ray, kCoop>(A.P, A.K, A.rec, A.tilesX, A.jitter, t, lane);
+    }
+}
+
+// ---------------------------------------------------------------- batched views
+// Wavefronts take (view, tile) items from one queue over all views.  The
+// view index is wavefront-uniform, so its descriptor is read through scalar
+// loads; per-view completion times are stamped for latency reporting.
+template <class SRay, class KRay>
+__global__ __launch_bounds__(256) void traceBatchKernel(const ViewDesc* __restrict__ views, BatchArgs A)
+{
+    const unsigned lane = __lane_id();
+    for (;;)
+    {
+        const unsigned t = nextTile(A.counter, lane);
+        if (t >= (unsigned)A.totalTiles) break;
+        int lo = 0, hi = A.nViews - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((unsigned)views[mid].tileBase <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int v = __builtin_amdgcn_readfirstlane(lo);
+        const ViewDesc& V = views[v];
+        const unsigned tile = t - (unsigned)V.tileBase;
+        if (V.P.model == BH_KERR || V.P.model == BH_KERR_NEWMAN)
+            traceTile<KRay, false>(V.P, V.K, V.rec, V.tilesX, V.jitter, tile, lane);
+        else traceTile<SRay, false>(V.P, V.K, V.rec, V.tilesX, V.jitter, tile, lane);
+        if (lane == 0)
+        {
+            // all of this wavefront's record stores are issued before the count
+            __threadfence();
+            const unsigned d = atomicAdd(&A.viewDone[v], 1u) + 1u;
+            if (d == (unsigned)V.tileCount) A.stamps[v] = (unsigned long long)wall_clock64();
+        }
+    }
+}
+
 template <class Ray>
 __global__ __launch_bounds__(256) void probeKernel(ProbeArgs A)
 {
@@ -165,16 +261,22 @@ template <class Ray> static hipError_t launchTraceT(const TraceArgs& Ain, const 
     TraceArgs A = Ain;
     A.K = makeFrame(A.P);
     dim3 block(S.wgSize), grid(S.blocks);
-    if (A.coop) hipLaunchKernelGGL((traceKernel<Ray, true>), grid, block, 0, st, A);
+    const bool tiles = A.refillMin >= 64;
+    if (tiles && A.coop) hipLaunchKernelGGL((traceTileKernel<Ray, true>), grid, block, 0, st, A);
+    else if (tiles) hipLaunchKernelGGL((traceTileKernel<Ray, false>), grid, block, 0, st, A);
+    else if (A.coop) hipLaunchKernelGGL((traceKernel<Ray, true>), grid, block, 0, st, A);
     else hipLaunchKernelGGL((traceKernel<Ray, false>), grid, block, 0, st, A);
     return hipGetLastError();
 }
 
-template <class Ray> static int occupancyT(int wgSize, int coop)
+template <class Ray> static int occupancyT(int wgSize, int coop, int tiles)
 {
     int nb = 0;
-    hipError_t e = coop ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceKernel<Ray, true>, wgSize, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceKernel<Ray, false>, wgSize, 0);
+    hipError_t e;
+    if (tiles && coop) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceTileKernel<Ray, true>, wgSize, 0);
+    else if (tiles) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceTileKernel<Ray, false>, wgSize, 0);
+    else if (coop) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceKernel<Ray, true>, wgSize, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceKernel<Ray, false>, wgSize, 0);
     if (e != hipSuccess) return 1;
     return nb > 0 ? nb : 1;
 }
@@ -185,9 +287,26 @@ hipError_t BHK_SUFFIX(launchTrace)(const TraceArgs& A, const LaunchShape& S, hip
     return kerr ? launchTraceT<KerrRay<Pol, false>>(A, S, st) : launchTraceT<SphRay<Pol>>(A, S, st);
 }
 
-int BHK_SUFFIX(traceOccupancy)(int kerr, int wgSize, int coop)
+int BHK_SUFFIX(traceOccupancy)(int kerr, int wgSize, int coop, int tiles)
 {
-    return kerr ? occupancyT<KerrRay<Pol, false>>(wgSize, coop) : occupancyT<SphRay<Pol>>(wgSize, coop);
+    return kerr ? occupancyT<KerrRay<Pol, false>>(wgSize, coop, tiles) : occupancyT<SphRay<Pol>>(wgSize, coop, tiles);
+}
+
+hipError_t BHK_SUFFIX(launchTraceBatch)(const BatchArgs& A, const LaunchShape& S, hipStream_t st)
+{
+    using SR = SphRay<Pol>;
+    using KR = KerrRay<Pol, false>;
+    hipLaunchKernelGGL((traceBatchKernel<SR, KR>), dim3(S.blocks), dim3(S.wgSize), 0, st, A.views, A);
+    return hipGetLastError();
+}
+
+int BHK_SUFFIX(traceBatchOccupancy)(int wgSize)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, traceBatchKernel<SphRay<Pol>, KerrRay<Pol, false>>, wgSize,
+                                                     0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
 }
 
 hipError_t BHK_SUFFIX(launchProbe)(const ProbeArgs& Ain, hipStream_t st)

@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <utility>
+#include <vector>
 #include "../csrc/bh_internal.h"
 
 namespace {
@@ -31,6 +32,17 @@ struct Scratch
     unsigned* counter = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool evValid = false;
+    // batched launches: view table (device + pinned host staging), per-view
+    // counters and stamps; `staged` marks the completion of the last upload
+    // out of `hostViews`, which must not be rewritten before it.
+    bhk::ViewDesc* views = nullptr;
+    bhk::ViewDesc* hostViews = nullptr;
+    hipEvent_t staged = nullptr;
+    unsigned* viewDone = nullptr;
+    unsigned long long* stamps = nullptr;
+    int viewCap = 0;
+    int lastViews = 0;
+    std::vector<hipEvent_t> viewEvents;   // [0] batch start, [v+1] view v done
 };
 
 std::mutex g_mu;
@@ -86,6 +98,32 @@ hipError_t getScratch(hipStream_t st, size_t nRec, Scratch** out)
     return hipSuccess;
 }
 
+hipError_t getBatchScratch(Scratch* S, int nViews, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    hipError_t e = hipSuccess;
+    if (!S->staged)
+    {
+        e = hipEventCreateWithFlags(&S->staged, hipEventDisableTiming);
+        if (e != hipSuccess) { S->staged = nullptr; return e; }
+    }
+    if (nViews <= S->viewCap) return hipSuccess;
+    // the old tables may still be in use by queued work
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    (void)hipFree(S->views);
+    (void)hipHostFree(S->hostViews);
+    (void)hipFree(S->viewDone);
+    (void)hipFree(S->stamps);
+    S->views = nullptr; S->hostViews = nullptr; S->viewDone = nullptr; S->stamps = nullptr; S->viewCap = 0;
+    e = hipMalloc(&S->views, sizeof(bhk::ViewDesc) * nViews);
+    if (e == hipSuccess) e = hipHostMalloc(&S->hostViews, sizeof(bhk::ViewDesc) * nViews, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&S->viewDone, sizeof(unsigned) * nViews);
+    if (e == hipSuccess) e = hipMalloc(&S->stamps, sizeof(unsigned long long) * (nViews + 1));
+    if (e == hipSuccess) S->viewCap = nViews;
+    return e;
+}
+
 BhSchedule currentSchedule()
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -120,8 +158,9 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, unsigned* count
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     const int kerr = isKerr(P.model) ? 1 : 0;
-    int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, sc.diskCoop)
-                          : bhk::traceOccupancyExact(kerr, sc.wgSize, sc.diskCoop);
+    const int tiles = sc.raysPerRefill >= 64 ? 1 : 0;
+    int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, sc.diskCoop, tiles)
+                          : bhk::traceOccupancyExact(kerr, sc.wgSize, sc.diskCoop, tiles);
     long long want = (long long)occ * numCUs(dev);
     long long maxUseful = ((long long)A.totalIds + sc.wgSize - 1) / sc.wgSize;
     bhk::LaunchShape S;
@@ -192,6 +231,141 @@ hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bl
     return hipGetLastError();
 }
 
+// The persistent multi-view trace kernel (traceBatchKernel: one queue of
+// (view, tile) items over all views) hung on its first GPU runs and stays
+// switched off until the cause is shown; see renderBatchPersistent.  Batches
+// therefore run as consecutive single-view pipelines on the caller's stream,
+// which use only the tile kernel of launchRenderPipeline.
+constexpr bool kPersistentBatchEnabled = false;
+
+hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const* outs, float4* const* accums,
+                                 float4* const* bloomAs, float4* const* bloomBs, hipStream_t st)
+{
+    if (!kPersistentBatchEnabled) return hipErrorNotSupported;
+
+    if (n < 1 || !views || !outs || !accums) return hipErrorInvalidValue;
+    std::vector<bhk::ViewDesc> D((size_t)n);
+    size_t totalPix = 0;
+    int tiles = 0, maxW = 1, maxH = 1;
+    for (int v = 0; v < n; ++v)
+    {
+        RenderParams P = views[v];
+        bhSanitizeRenderParams(&P);
+        if (!outs[v] || !accums[v]) return hipErrorInvalidValue;
+        if (P.bloomEnabled && (!bloomAs || !bloomBs || !bloomAs[v] || !bloomBs[v])) return hipErrorInvalidValue;
+        bhk::ViewDesc& d = D[(size_t)v];
+        std::memset(&d, 0, sizeof(d));
+        d.P = P;
+        d.K = bhk::makeFrame(P);
+        d.out = outs[v];
+        d.accum = accums[v];
+        d.bloomA = bloomAs ? bloomAs[v] : nullptr;
+        d.bloomB = bloomBs ? bloomBs[v] : nullptr;
+        d.tilesX = (P.width + 7) / 8;
+        d.tileBase = tiles;
+        d.tileCount = d.tilesX * ((P.height + 7) / 8);
+        d.jitter = jitterOf(P.accumMode, P.sampleIndex);
+        tiles += d.tileCount;
+        totalPix += (size_t)P.width * P.height;
+        maxW = P.width > maxW ? P.width : maxW;
+        maxH = P.height > maxH ? P.height : maxH;
+    }
+    Scratch* S = nullptr;
+    hipError_t e = getScratch(st, totalPix, &S);
+    if (e != hipSuccess) return e;
+    e = getBatchScratch(S, n, st);
+    if (e != hipSuccess) return e;
+    size_t off = 0;
+    for (auto& d : D)
+    {
+        d.rec = S->rec + off;
+        off += (size_t)d.P.width * d.P.height;
+    }
+    // stage the table in pinned memory (after the previous upload from it
+    // has completed) and copy it asynchronously on the stream
+    e = hipEventSynchronize(S->staged);
+    if (e != hipSuccess) return e;
+    std::memcpy(S->hostViews, D.data(), sizeof(bhk::ViewDesc) * n);
+    e = hipMemcpyAsync(S->views, S->hostViews, sizeof(bhk::ViewDesc) * n, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(S->staged, st);
+    if (e != hipSuccess) return e;
+    bhk::BatchArgs A;
+    A.views = S->views;
+    A.nViews = n;
+    A.totalTiles = tiles;
+    A.counter = S->counter;
+    A.viewDone = S->viewDone;
+    A.stamps = S->stamps;
+    S->lastViews = n;
+    const BhSchedule sc = currentSchedule();
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    const int occ = sc.mathMode ? bhk::traceBatchOccupancyFast(sc.wgSize) : bhk::traceBatchOccupancyExact(sc.wgSize);
+    bhk::LaunchShape L;
+    L.wgSize = sc.wgSize;
+    const long long want = (long long)occ * numCUs(dev), useful = ((long long)tiles * 64 + sc.wgSize - 1) / sc.wgSize;
+    L.blocks = (int)(want < useful ? want : useful);
+    if (L.blocks < 1) L.blocks = 1;
+    const bool timed = g_timers.load() != 0;
+    if (timed) recordEvent(S, 0, st);
+    e = bhk::launchResetBatch(A, st);
+    if (e != hipSuccess) return e;
+    e = sc.mathMode ? bhk::launchTraceBatchFast(A, L, st) : bhk::launchTraceBatchExact(A, L, st);
+    if (e != hipSuccess) return e;
+    if (timed) recordEvent(S, 1, st);
+    e = bhk::launchShadeBlendBatch(S->views, n, maxW, maxH, st);
+    if (e != hipSuccess) return e;
+    if (timed) recordEvent(S, 2, st);
+    e = bhk::launchBloomBatch(S->views, n, maxW, maxH, st);
+    if (e != hipSuccess) return e;
+    if (timed)
+    {
+        recordEvent(S, 3, st);
+        int d2 = 0;
+        if (hipGetDevice(&d2) != hipSuccess) d2 = 0;
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_lastTimed = {d2, st};
+    }
+    return hipGetLastError();
+}
+
+
+hipError_t renderBatch(int n, const RenderParams* views, uchar4* const* outs, float4* const* accums,
+                       float4* const* bloomAs, float4* const* bloomBs, hipStream_t st)
+{
+    if (n < 1 || !views || !outs || !accums) return hipErrorInvalidValue;
+    if (kPersistentBatchEnabled) return renderBatchPersistent(n, views, outs, accums, bloomAs, bloomBs, st);
+    Scratch* S = nullptr;
+    hipError_t e = getScratch(st, 0, &S);
+    if (e != hipSuccess) return e;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if ((int)S->viewEvents.size() < n + 1)
+        {
+            for (size_t k = S->viewEvents.size(); k < (size_t)n + 1; ++k)
+            {
+                hipEvent_t ev = nullptr;
+                e = hipEventCreate(&ev);
+                if (e != hipSuccess) return e;
+                S->viewEvents.push_back(ev);
+            }
+        }
+        S->lastViews = n;
+    }
+    e = hipEventRecord(S->viewEvents[0], st);
+    if (e != hipSuccess) return e;
+    for (int v = 0; v < n; ++v)
+    {
+        e = renderPipeline(outs[v], accums[v], bloomAs ? bloomAs[v] : nullptr, bloomBs ? bloomBs[v] : nullptr,
+                           views[v], st, false);
+        if (e != hipSuccess) return e;
+        e = hipEventRecord(S->viewEvents[(size_t)v + 1], st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct BhFramebuffer
@@ -259,6 +433,33 @@ bh_error_t bhProbeDirs(BhProbe* devOut, const bh_float3* devDirs, int n, const R
 {
     if (!devDirs) return hipErrorInvalidValue;
     return probeCommon(devOut, devDirs, n, 0, 0, p, stream);
+}
+
+// ---------------------------------------------------------------- batched views
+bh_error_t bhRenderBatch(int nViews, const RenderParams* views, bh_uchar4* const* outs, bh_float4* const* accums,
+                         bh_float4* const* bloomAs, bh_float4* const* bloomBs, bh_stream_t stream)
+{
+    return (bh_error_t)renderBatch(nViews, views, reinterpret_cast<uchar4* const*>(outs),
+                                   reinterpret_cast<float4* const*>(accums), reinterpret_cast<float4* const*>(bloomAs),
+                                   reinterpret_cast<float4* const*>(bloomBs), (hipStream_t)stream);
+}
+
+bh_error_t bhBatchViewLatency(float* msOut, int nViews, bh_stream_t stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    Scratch* S = nullptr;
+    hipError_t e = getScratch(st, 0, &S);
+    if (e != hipSuccess) return e;
+    if (!msOut || nViews < 1 || nViews > S->lastViews || (int)S->viewEvents.size() < nViews + 1)
+        return hipErrorInvalidValue;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    for (int v = 0; v < nViews; ++v)
+    {
+        e = hipEventElapsedTime(&msOut[v], S->viewEvents[0], S->viewEvents[(size_t)v + 1]);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------- framebuffer

@@ -7,6 +7,7 @@
 //   * the orbit camera (main.cpp:186-228) and the named workload presets.
 // ---------------------------------------------------------------------------
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include "../../include/bh/bhtrace.h"
 
@@ -113,6 +114,21 @@ bool orbitEnergy(double r, double M, double a, double Q, double& E, double& dE)
     dE = e.d;
     return true;
 }
+
+// SplitMix64 (Steele, Lea & Flood 2014): the batch generator's PRNG.
+struct SplitMix64
+{
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double uniform(double lo, double hi) { return lo + (hi - lo) * uniform(); }
+};
 
 }  // namespace
 
@@ -390,6 +406,46 @@ int bhWorkloadParams(const char* name, int frame, RenderParams* out)
     else return -1;
     *out = P;
     return 0;
+}
+
+int bhWorkloadViews(const char* name, uint64_t seed, int maxViews, RenderParams* out)
+{
+    if (!name || std::strcmp(name, "uneven_views") != 0 || maxViews < 0) return -1;
+    static const int res[4][2] = {{320, 180}, {640, 360}, {1280, 720}, {1920, 1080}};
+    const double deg = 3.14159265358979323846 / 180.0;
+    SplitMix64 rng{seed};
+    for (int v = 0; v < maxViews; ++v)
+    {
+        RenderParams P;
+        const int r = (int)(rng.uniform() * 4.0) & 3;
+        P.width = res[r][0];
+        P.height = res[r][1];
+        const double aStar = rng.uniform(-0.99, 0.99);
+        const double qMax = std::sqrt(std::fmax(0.98 - aStar * aStar, 0.0));
+        const double q = rng.uniform(0.0, qMax);
+        const double dist = std::exp(rng.uniform(std::log(6.0), std::log(40.0)));
+        const double el = rng.uniform(-80.0, 80.0) * deg;
+        const double fov = rng.uniform(40.0, 90.0);
+        const bool disk = rng.uniform() < 0.5;
+        const double t = rng.uniform(0.0, 10.0);
+        float M = 0.5f, a = (float)aStar, qq = (float)q;
+        BHDerived d;
+        bhValidateAndDerive(BH_KERR_NEWMAN, &M, &a, &qq, 0, &d);
+        P.model = BH_KERR_NEWMAN;
+        P.M = M; P.aSpin = d.a; P.Qc = d.Q;
+        P.rPlus = d.rPlus; P.rPhoton = d.rPhoton; P.rErgo = d.rErgo;
+        P.dPhi = 0.012f; P.maxSteps = 2000;
+        P.diskEnabled = disk ? 1 : 0;
+        P.diskInner = (float)bhIscoFP64(P.M, P.aSpin, P.Qc);
+        P.diskOuter = 8.0f;
+        P.diskTime = (float)t;
+        P.hotSpotsEnabled = 1; P.hotSpotStrength = 0.65f;
+        P.bloomEnabled = 1; P.bloomStrength = 0.35f;
+        P.exposure = 1.0f;
+        bhFillCamera(&P, 0.6f, (float)el, (float)dist, (float)fov);
+        out[v] = P;
+    }
+    return maxViews;
 }
 
 }  // extern "C"

@@ -94,6 +94,15 @@ void bhFillCamera(RenderParams* p, float azimuthRad, float elevationRad,
  *   "kn_m07_q04_2160p_high"  Kerr-Newman a*=-0.7 q=0.4 3840x2160, bloom */
 int bhWorkloadParams(const char* name, int frame, RenderParams* out);
 
+/* Seeded batch of views, "uneven_views": resolution uniform in {320x180,
+ * 640x360, 1280x720, 1920x1080}, Kerr-Newman with a* ~ U[-0.99, 0.99] and
+ * q ~ U[0, sqrt(0.98 - a*^2)], camera distance log-uniform in [6, 40],
+ * elevation U[-80, 80] deg, vertical FOV U[40, 90] deg, disk on with
+ * probability 1/2 from the FP64 ISCO to 8, disk time U[0, 10], balanced
+ * preset, bloom on.  SplitMix64 stream from `seed`.  Writes `maxViews` views
+ * and returns that count, or -1 for an unknown name. */
+int bhWorkloadViews(const char* name, uint64_t seed, int maxViews, RenderParams* out);
+
 /* ------------------------------------------------------------------------ */
 /* Probes (verification; follow probeGridKernel/probeOneKernel,             */
 /* test_main.cu:71-98)                                                      */
@@ -124,6 +133,24 @@ typedef struct BhRayRecord
     float trans;        /* remaining transmittance */
 } BhRayRecord;
 bh_error_t bhTraceRecords(BhRayRecord* devOut, BH_PARAMS_IN p, bh_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched views: one persistent launch over many views                     */
+/* ------------------------------------------------------------------------ */
+/* Renders `nViews` views of any sizes and models, each with the full
+ * pipeline of launchRenderPipeline (accumulation, bloom, tonemap) into its
+ * own buffers: outs[v] uchar4[W*H], accums[v] float4[W*H], bloomAs[v] /
+ * bloomBs[v] float4[ceil(W/2)*ceil(H/2)] (the arrays may be NULL when no view
+ * has bloom).  `views` is a host array; each view is copied and sanitized.
+ * Results equal per-view launchRenderPipeline calls bit for bit.  The views
+ * currently run as consecutive pipelines on `stream` (the single-queue
+ * persistent multi-view kernel is disabled in this build). */
+bh_error_t bhRenderBatch(int nViews, const RenderParams* views, bh_uchar4* const* outs,
+                         bh_float4* const* accums, bh_float4* const* bloomAs,
+                         bh_float4* const* bloomBs, bh_stream_t stream);
+/* Per-view completion time (ms after the batch began on the GPU) of the
+ * last bhRenderBatch on `stream`; synchronizes the stream. */
+bh_error_t bhBatchViewLatency(float* msOut, int nViews, bh_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Headless framebuffer (replaces the Vulkan interop image)                 */
