@@ -43,7 +43,25 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-row-step", type=int, default=4)
     ap.add_argument("--math", type=int, default=None, help="0 exact, 1 fast (default: library default)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """Three plain device-to-device copies of `mib` MiB each: under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE this gives the counter reading of
+    a known byte count (mib MiB read + mib MiB written per copy)."""
+    import bhtrace as bt
+
+    n = mib << 20
+    src, dst = bt.DeviceBuffer(n), bt.DeviceBuffer(n)
+    src.fill(1)
+    bt.synchronize()
+    for _ in range(3):
+        bt.check(bt.load().bhCopyDevice(dst.ptr, src.ptr, n), "bhCopyDevice")
+    bt.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
 
 
 def cpu_baseline(P, row_step):
@@ -69,6 +87,9 @@ def main():
         sys.exit(2)
     import bhtrace as bt
 
+    if a.copy_calibration > 0:
+        copy_calibration(a.copy_calibration)
+        return
     lib = bt.load()
     if lib.bhDeviceCount() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
@@ -111,7 +132,7 @@ def main():
         achieved = fl / (avg_trace * 1e-3) / 1e12
         roof = {"bound": "valu_fp32", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "traceKernel", "flops_per_frame": int(fl), "trace_ms": round(avg_trace, 3)}
+                "kernel": "traceTileKernel", "flops_per_frame": int(fl), "trace_ms": round(avg_trace, 3)}
     res = {
         "metric": METRIC, "value": round(mrays, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",

@@ -70,6 +70,13 @@ class BhSchedule(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BhPrecision(C.Structure):
+    _fields_ = [("fp64Band", C.c_int), ("critBand", C.c_float), ("polarBand", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 PROBE_DTYPE = np.dtype([("outcome", "<i4"), ("escDir", "<f4", (3,)), ("maxHviol", "<f4"),
                         ("lum", "<f4"), ("steps", "<i4"), ("trans", "<f4")])
 RECORD_DTYPE = np.dtype([("escDir", "<f4", (3,)), ("flags", "<i4"), ("accum", "<f4", (3,)),
@@ -105,6 +112,9 @@ SIGNATURES = [
     ("bhFramebufferReadback", C.c_int, [_vp, _vp, _vp, _vp]),
     ("bhGetSchedule", None, [C.POINTER(BhSchedule)]),
     ("bhSetSchedule", C.c_int, [C.POINTER(BhSchedule)]),
+    ("bhGetPrecision", None, [C.POINTER(BhPrecision)]),
+    ("bhSetPrecision", C.c_int, [C.POINTER(BhPrecision)]),
+    ("bhLastBandStats", C.c_int, [C.POINTER(C.c_uint), C.POINTER(C.c_float), _vp]),
     ("bhEnableKernelTimers", None, [C.c_int]),
     ("bhLastKernelTimes", C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("bhDeviceAlloc", C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
@@ -112,6 +122,7 @@ SIGNATURES = [
     ("bhCopyToDevice", C.c_int, [_vp, _vp, C.c_size_t]),
     ("bhCopyToHost", C.c_int, [_vp, _vp, C.c_size_t]),
     ("bhDeviceFill", C.c_int, [_vp, C.c_ubyte, C.c_size_t]),
+    ("bhCopyDevice", C.c_int, [_vp, _vp, C.c_size_t]),
     ("bhDeviceSynchronize", C.c_int, []),
     ("bhStreamCreate", C.c_int, [C.POINTER(C.c_void_p)]),
     ("bhStreamDestroy", C.c_int, [_vp]),
@@ -344,6 +355,36 @@ def set_schedule(wg: int | None = None, chunk: int | None = None, math: int | No
 
 def restore_schedule(s: BhSchedule) -> None:
     check(load().bhSetSchedule(C.byref(s)), "bhSetSchedule")
+
+
+def get_precision() -> BhPrecision:
+    p = BhPrecision()
+    load().bhGetPrecision(C.byref(p))
+    return p
+
+
+def set_precision(band: int | None = None, crit: float | None = None, polar: float | None = None) -> BhPrecision:
+    """FP64 critical-band pass on/off and its thresholds (returns the new setting)."""
+    p = get_precision()
+    if band is not None:
+        p.fp64Band = band
+    if crit is not None:
+        p.critBand = crit
+    if polar is not None:
+        p.polarBand = polar
+    check(load().bhSetPrecision(C.byref(p)), "bhSetPrecision")
+    return p
+
+
+def restore_precision(p: BhPrecision) -> None:
+    check(load().bhSetPrecision(C.byref(p)), "bhSetPrecision")
+
+
+def last_band_stats(stream=None):
+    """(band ray count, FP64 band kernel ms or 0) of the last frame on `stream`."""
+    n, ms = C.c_uint(), C.c_float()
+    check(load().bhLastBandStats(C.byref(n), C.byref(ms), stream), "bhLastBandStats")
+    return n.value, ms.value
 
 
 class BatchBuffers:

@@ -1,0 +1,418 @@
+// ---------------------------------------------------------------------------
+// bh_band.h -- FP64 critical-band rays.
+//
+// A ray whose conserved quantities put it close to the critical (photon
+// ring) curve winds around the hole before it escapes or falls in, and a
+// ray whose polar turning point lies close to the spin axis passes the
+// coordinate pole; in both cases FP32 round-off is amplified far beyond
+// the exit-direction tolerance.  Such rays are found before tracing from
+// their FP64 invariants and traced in double precision; the disk radiative
+// transfer along their chords stays in float (as for every other ray).
+//
+// Band tests (all FP64):
+//   critical:  delta = min_r R(r) / (r^2 + a^2)^2 over r in [r+, 5M], with
+//              R(r) = (r^2 + a^2 - a xi)^2 - Delta(r) (eta + (xi - a)^2),
+//              xi = Lz/E, eta = Carter/E^2 (Kerr-Newman radial potential;
+//              Bardeen 1973, de Vries 2000).  For the spherical models this
+//              is the closed form 1 - b^2/b_c^2.  |delta| < critBand.
+//   polar:     sin^2(theta_min) = 1 - u+, u+ the larger root of
+//              a^2 u^2 - (a^2 - eta - xi^2) u - eta = 0 (theta turning
+//              point, u = cos^2 theta).  Kerr models only.  < polarBand.
+//
+// The FP64 integrators follow the reference's traceSpherical
+// (trace.cuh:466-611) and traceKerr (trace.cuh:684-890) operation for
+// operation in double, with float literals widened to double.
+// ---------------------------------------------------------------------------
+#pragma once
+#include "bh_rays.h"
+
+namespace bhk {
+
+struct d3 { double x, y, z; };
+BHK_INL d3 dk3(double x, double y, double z) { return d3{x, y, z}; }
+BHK_INL d3 operator+(d3 a, d3 b) { return dk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+BHK_INL d3 operator-(d3 a, d3 b) { return dk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+BHK_INL d3 operator*(d3 a, double s) { return dk3(a.x * s, a.y * s, a.z * s); }
+BHK_INL double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+BHK_INL double dlen(d3 a) { return sqrt(ddot(a, a)); }
+BHK_INL d3 ddivs(d3 a, double s) { return a * (1.0 / s); }   // V3 / scalar (vec_math.cuh)
+BHK_INL d3 dnormalize(d3 a) { return a * (1.0 / sqrt(fmax(ddot(a, a), (double)1e-20f))); }
+BHK_INL v3 tof3(d3 a) { return mk3((float)a.x, (float)a.y, (float)a.z); }
+#define F2D(x) ((double)(x##f))
+
+// Per-frame FP64 constants: the RenderParams floats widened to double and
+// the derived values evaluated in double (host-built, passed by value).
+struct BandK
+{
+    double M, a, Q, rPlus, rPhoton, dPhi, diskOuter, diskTime;
+    double rCap, rFloor, escR, capR, rPhotonRefine, diskOuterCap, sixM, kRad, kAng;
+    double uHor, uEscape, invNineM, uBudget, hFloor;
+    double bCrit2;           // spherical models: critical impact parameter squared
+    double rScanLo, rScanHi; // Kerr models: radial range of the critical-curve scan
+    double critBand, polarBand;
+    int kerr, maxSteps, disk, pad;
+};
+
+// FP64 invariants of a camera ray: E, Lz from the static-observer tetrad,
+// Carter's constant, plus the Boyer-Lindquist start state.
+struct RayInv64
+{
+    double E, Lz, carter, r0, th0, ph0, pr, pth;
+};
+
+BHK_INL RayInv64 kerrInvariants64(const BandK& B, d3 o, d3 dir)
+{
+#pragma clang fp contract(off)
+    RayInv64 I;
+    const double a = B.a, M = B.M, Q = B.Q;
+    const double r0 = dlen(o);
+    const double cth0 = fmin(fmax(o.y / r0, -1.0), 1.0);
+    const double th0 = acos(cth0);
+    const double ph0 = atan2(-o.z, o.x);
+    const double s = sin(th0), c = cos(th0), sp = sin(ph0), cp = cos(ph0);
+    const d3 er = dk3(s * cp, c, -s * sp);
+    const d3 eth = dk3(c * cp, -s, -c * sp);
+    const d3 eph = dk3(-sp, 0.0, -cp);
+    const double drc = ddot(dir, er), dtc = ddot(dir, eth), dpc = ddot(dir, eph);
+    double s0 = sin(th0);
+    if (s0 < F2D(1e-4)) s0 = F2D(1e-4);
+    const double s02 = s0 * s0;
+    const double Sig0 = r0 * r0 + a * a * cth0 * cth0;
+    const double Del0 = r0 * r0 - 2.0 * M * r0 + a * a + Q * Q;
+    const double m2r0 = 2.0 * M * r0 - Q * Q;
+    double gtt = -(1.0 - m2r0 / Sig0);
+    if (gtt > F2D(-1e-5)) gtt = F2D(-1e-5);
+    const double gtp = -a * m2r0 * s02 / Sig0;
+    const double gpp = (r0 * r0 + a * a + m2r0 * a * a * s02 / Sig0) * s02;
+    const double A = 1.0 / sqrt(fmax(gpp - gtp * gtp / gtt, F2D(1e-8)));
+    const double ptUp = 1.0 / sqrt(-gtt) - dpc * A * gtp / gtt;
+    const double pfUp = dpc * A;
+    I.E = -(gtt * ptUp + gtp * pfUp);
+    I.Lz = gtp * ptUp + gpp * pfUp;
+    I.r0 = r0; I.th0 = th0; I.ph0 = ph0;
+    I.pr = drc * sqrt(Sig0 / fmax(Del0, F2D(1e-6)));
+    I.pth = dtc * sqrt(Sig0);
+    I.carter = I.pth * I.pth + cth0 * cth0 * (I.Lz * I.Lz / s02 - a * a * I.E * I.E);
+    return I;
+}
+
+// R(r) / (r^2 + a^2)^2 for unit energy.
+BHK_INL double radialPotN(const BandK& B, double xi, double eta, double r)
+{
+#pragma clang fp contract(off)
+    const double a = B.a;
+    const double ra = r * r + a * a;
+    const double P = ra - a * xi;
+    const double Del = r * r - 2.0 * B.M * r + a * a + B.Q * B.Q;
+    const double xa = xi - a;
+    return (P * P - Del * (eta + xa * xa)) / (ra * ra);
+}
+
+// min over [rScanLo, rScanHi]: 32-point scan, then 24 golden-section steps
+// in the bracket around the best sample.
+BHK_INL double criticalDelta(const BandK& B, double xi, double eta)
+{
+#pragma clang fp contract(off)
+    const int N = 32;
+    const double lo = B.rScanLo, hi = B.rScanHi, dr = (hi - lo) / (double)(N - 1);
+    int best = 0;
+    double vBest = radialPotN(B, xi, eta, lo);
+    for (int i = 1; i < N; ++i)
+    {
+        const double v = radialPotN(B, xi, eta, lo + dr * (double)i);
+        if (v < vBest) { vBest = v; best = i; }
+    }
+    double x0 = lo + dr * (double)(best > 0 ? best - 1 : 0);
+    double x3 = lo + dr * (double)(best < N - 1 ? best + 1 : N - 1);
+    const double gr = 0.6180339887498949;
+    double x1 = x3 - gr * (x3 - x0), x2 = x0 + gr * (x3 - x0);
+    double f1 = radialPotN(B, xi, eta, x1), f2 = radialPotN(B, xi, eta, x2);
+    for (int it = 0; it < 24; ++it)
+    {
+        if (f1 < f2) { x3 = x2; x2 = x1; f2 = f1; x1 = x3 - gr * (x3 - x0); f1 = radialPotN(B, xi, eta, x1); }
+        else { x0 = x1; x1 = x2; f1 = f2; x2 = x0 + gr * (x3 - x0); f2 = radialPotN(B, xi, eta, x2); }
+    }
+    return fmin(vBest, fmin(f1, f2));
+}
+
+// sin^2 of the polar turning angle.
+BHK_INL double polarSin2Min(double a, double xi, double eta)
+{
+#pragma clang fp contract(off)
+    const double a2 = a * a;
+    const double Bq = a2 - eta - xi * xi;
+    const double D = sqrt(fmax(Bq * Bq + 4.0 * a2 * eta, 0.0));
+    double u;
+    if (Bq <= 0.0) u = (D - Bq > 0.0) ? 2.0 * eta / (D - Bq) : 1.0;
+    else u = (a2 > 0.0) ? (Bq + D) / (2.0 * a2) : 1.0;
+    return 1.0 - u;
+}
+
+// Camera ray of pixel (px, py) in double (trace.cuh:431-438, 910-912).
+BHK_INL d3 cameraDir64(const RenderParams& P, int px, int py, float2 jit)
+{
+#pragma clang fp contract(off)
+    const double fx = ((double)(float)px + (double)jit.x) / (double)(float)P.width;
+    const double fy = ((double)(float)py + (double)jit.y) / (double)(float)P.height;
+    const double nx = fx * 2.0 - 1.0, ny = 1.0 - fy * 2.0;
+    const d3 F = dk3(P.camForward.x, P.camForward.y, P.camForward.z);
+    const d3 R = dk3(P.camRight.x, P.camRight.y, P.camRight.z);
+    const d3 U = dk3(P.camUp.x, P.camUp.y, P.camUp.z);
+    return dnormalize(F + R * (nx * (double)P.tanHalfFov * (double)P.aspect) + U * (ny * (double)P.tanHalfFov));
+}
+
+// true: trace this ray in FP64.
+BHK_INL bool inCriticalBand(const BandK& B, d3 o, d3 dir)
+{
+#pragma clang fp contract(off)
+    if (B.kerr)
+    {
+        const RayInv64 I = kerrInvariants64(B, o, dir);
+        if (!(I.E > 1e-12) || !isfinite(I.E) || !isfinite(I.Lz)) return true;
+        const double xi = I.Lz / I.E, eta = I.carter / (I.E * I.E);
+        if (fabs(criticalDelta(B, xi, eta)) < B.critBand) return true;
+        return polarSin2Min(B.a, xi, eta) < B.polarBand;
+    }
+    const double r0 = dlen(o);
+    const d3 e1 = ddivs(o, r0);
+    const d3 perp = dir - e1 * ddot(dir, e1);
+    const double pl = dlen(perp);
+    if (pl < F2D(1e-6) || !(B.bCrit2 > 0.0)) return false;
+    const double f0 = fmax(1.0 - 2.0 * B.M / r0 + B.Q * B.Q / (r0 * r0), F2D(1e-6));
+    const double b2 = (r0 * pl) * (r0 * pl) / f0;
+    return fabs(1.0 - b2 / B.bCrit2) < B.critBand;
+}
+
+// ------------------------------------------------------------------ spherical, FP64 (trace.cuh:466-611)
+template <class Pol> struct SphRay64
+{
+    double u, du, phi, tFlight, bImpact;
+    d3 e1, e2, prevPos, dir3, escDir;
+    v3 acc;
+    float trans, E, Lz;
+    int step, status;
+
+    BHK_INL void init(const BandK& B, d3 origin, d3 dir)
+    {
+#pragma clang fp contract(off)
+        status = RAY_RUNNING;
+        step = 0;
+        escDir = dir;
+        acc = mk3(0.f, 0.f, 0.f);
+        trans = 1.0f;
+        {
+            // photon E, Lz in double, handed to the float disk shading (trace.cuh:213-247)
+            const RayInv64 I = kerrInvariants64(B, origin, dir);
+            double Ed = I.E, Ld = I.Lz;
+            if (!(Ed > F2D(1e-6)) || !isfinite(Ed)) { Ed = 1.0; Ld = 0.0; }
+            E = (float)Ed;
+            Lz = (float)Ld;
+        }
+        const double r0 = dlen(origin);
+        e1 = ddivs(origin, r0);
+        const double ddr = ddot(dir, e1);
+        const d3 perp = dir - e1 * ddr;
+        const double pl = dlen(perp);
+        if (pl < F2D(1e-6))
+        {
+            status = (ddot(dir, origin) < 0.0) ? RAY_CAPTURED : RAY_ESCAPED;
+            return;
+        }
+        e2 = ddivs(perp, pl);
+        const double f0 = fmax(1.0 - 2.0 * B.M / r0 + B.Q * B.Q / (r0 * r0), F2D(1e-6));
+        bImpact = fmax(r0 * pl / sqrt(f0), F2D(1e-6));
+        u = 1.0 / r0;
+        du = -sqrt(f0) * ddr / (r0 * pl);
+        phi = 0.0;
+        tFlight = 0.0;
+        prevPos = origin;
+        dir3 = dir;
+    }
+
+    BHK_INL float diskE() const { return E; }
+    BHK_INL float diskLz() const { return Lz; }
+
+    BHK_INL double accel(const BandK& B, double uu) const
+    {
+#pragma clang fp contract(off)
+        return 3.0 * B.M * uu * uu - 2.0 * (B.Q * B.Q) * uu * uu * uu - uu;
+    }
+
+    BHK_INL bool advance(const BandK& B, const DiskParams& D, Chord& ch)
+    {
+#pragma clang fp contract(off)
+        ch.n = 0;
+        if (step >= B.maxSteps)
+        {
+            if (u > B.uBudget) status = RAY_CAPTURED;
+            else { status = RAY_ESCAPED; escDir = dir3; }
+            return true;
+        }
+        const double hMax = B.dPhi * fmin(fmax(B.invNineM / u, 1.0), 6.0);
+        double h = fmin(hMax, F2D(0.10) / fmax(fabs(du), F2D(1e-6)));
+        if (h < B.hFloor) h = B.hFloor;
+        const double k1u = du, k1v = accel(B, u);
+        const double u2 = u + F2D(0.5) * h * k1u, v2 = du + F2D(0.5) * h * k1v;
+        const double k2u = v2, k2v = accel(B, u2);
+        const double u3 = u + F2D(0.5) * h * k2u, w3 = du + F2D(0.5) * h * k2v;
+        const double k3u = w3, k3v = accel(B, u3);
+        const double u4 = u + h * k3u, v4 = du + h * k3v;
+        const double k4u = v4, k4v = accel(B, u4);
+        u = u + (h / 6.0) * (k1u + 2.0 * k2u + 2.0 * k3u + k4u);
+        du = du + (h / 6.0) * (k1v + 2.0 * k2v + 2.0 * k3v + k4v);
+        phi = phi + h;
+        if (!isfinite(u) || !isfinite(du)) { status = RAY_CAPTURED; return true; }
+        if (u >= B.uHor) { status = RAY_CAPTURED; return true; }
+        if (u < F2D(1e-4)) { status = RAY_ESCAPED; escDir = dir3; return true; }
+        const double r = 1.0 / u;
+        const double cp = cos(phi), sp = sin(phi);
+        const d3 m = e1 * cp + e2 * sp;
+        const d3 mp = e1 * (-sp) + e2 * cp;
+        const double drdphi = -du / (u * u);
+        const d3 pos = m * r;
+        dir3 = dnormalize(m * drdphi + mp * r);
+        const double fR = fmax(1.0 - 2.0 * B.M / r + B.Q * B.Q / (r * r), F2D(1e-4));
+        tFlight = tFlight + h * (r * r) / (bImpact * fR);
+        if (B.disk && trans > 0.01f)
+            chordPrepare<Pol>(D, tof3(prevPos), tof3(pos), (float)(B.diskTime - tFlight), ch);
+        prevPos = pos;
+        if (u < B.uEscape && du < 0.0) { status = RAY_ESCAPED; escDir = dir3; return true; }
+        ++step;
+        return false;
+    }
+};
+
+// ------------------------------------------------------------------ Kerr-Newman, FP64 (trace.cuh:616-890)
+struct KD64 { double r, th, ph, pr, pth; };
+
+BHK_INL KD64 kerrRHS64(const BandK& B, double E, double Lz, double r, double th, double pr, double pth)
+{
+#pragma clang fp contract(off)
+    double s = sin(th);
+    const double cth = cos(th);
+    if (s < F2D(1e-4)) s = F2D(1e-4);
+    const double s2 = s * s, r2 = r * r, a = B.a, a2 = a * a;
+    const double Sig = r2 + a2 * cth * cth;
+    double Del = r2 - 2.0 * B.M * r + a2 + B.Q * B.Q;
+    if (Del < F2D(1e-6)) Del = F2D(1e-6);
+    const double Pp = E * (r2 + a2) - a * Lz;
+    const double Bq = Lz / s - a * E * s;
+    const double W = Bq * Bq - Pp * Pp / Del;
+    const double K = Del * pr * pr + pth * pth + W;
+    const double dDel = 2.0 * r - 2.0 * B.M;
+    const double dP = 2.0 * r * E;
+    const double Wr = -(2.0 * Pp * dP * Del - Pp * Pp * dDel) / (Del * Del);
+    const double dB = -Lz * cth / s2 - a * E * cth;
+    const double Wth = 2.0 * Bq * dB;
+    const double invS = 1.0 / Sig, invS2 = invS * invS;
+    KD64 d;
+    d.r = Del * pr * invS;
+    d.th = pth * invS;
+    d.ph = ((Lz / s2 - a * E) + a * Pp / Del) * invS;
+    d.pr = F2D(-0.5) * invS * (dDel * pr * pr + Wr) + r * invS2 * K;
+    d.pth = F2D(-0.5) * invS * Wth - a2 * s * cth * invS2 * K;
+    return d;
+}
+
+template <class Pol> struct KerrRay64
+{
+    double r, th, ph, pr, pth, E, Lz, tFlight;
+    KD64 dLast;
+    d3 prevPos, dir3, escDir;
+    v3 acc;
+    float trans, Ef, Lzf;
+    int step, status, haveDeriv;
+
+    BHK_INL void init(const BandK& B, d3 origin, d3 dir)
+    {
+        status = RAY_RUNNING;
+        step = 0;
+        haveDeriv = 0;
+        escDir = dir;
+        acc = mk3(0.f, 0.f, 0.f);
+        trans = 1.0f;
+        const RayInv64 I = kerrInvariants64(B, origin, dir);
+        E = I.E; Lz = I.Lz;
+        r = I.r0; th = I.th0; ph = I.ph0; pr = I.pr; pth = I.pth;
+        Ef = (float)E; Lzf = (float)Lz;
+        tFlight = 0.0;
+        prevPos = origin;
+        dir3 = dir;
+    }
+
+    BHK_INL float diskE() const { return Ef; }
+    BHK_INL float diskLz() const { return Lzf; }
+
+    BHK_INL d3 bodyDir(const KD64& d) const
+    {
+#pragma clang fp contract(off)
+        const double s = sin(th), c = cos(th), sp = sin(ph), cp = cos(ph);
+        const d3 e1b = dk3(s * cp, c, -s * sp), e2b = dk3(c * cp, -s, -c * sp), e3b = dk3(-sp, 0.0, -cp);
+        return dnormalize(e1b * d.r + e2b * (r * d.th) + e3b * (r * s * d.ph));
+    }
+
+    BHK_INL bool advance(const BandK& B, const DiskParams& D, Chord& ch)
+    {
+#pragma clang fp contract(off)
+        ch.n = 0;
+        if (step >= B.maxSteps)
+        {
+            if (r < B.sixM) status = RAY_CAPTURED;
+            else { status = RAY_ESCAPED; escDir = haveDeriv ? bodyDir(dLast) : dir3; }
+            return true;
+        }
+        const KD64 d1 = kerrRHS64(B, E, Lz, r, th, pr, pth);
+        if (!isfinite(r) || !isfinite(pr) || !isfinite(pth) || !isfinite(d1.r) || !isfinite(d1.pr))
+        { status = RAY_CAPTURED; return true; }
+        if (r <= B.capR) { status = RAY_CAPTURED; return true; }
+        if (r >= B.escR && d1.r > 0.0) { escDir = bodyDir(d1); status = RAY_ESCAPED; return true; }
+        double h = 1.0 / (fabs(d1.r) / (B.kRad * fmax(r - B.rFloor, F2D(0.02))) +
+                          (fabs(d1.th) + fabs(d1.ph)) / B.kAng + F2D(1e-5));
+        if (r < B.rPhotonRefine) h = h * F2D(0.55);
+        const double spd = fabs(d1.r) + r * (fabs(d1.th) + fabs(d1.ph)) + F2D(1e-6);
+        double capLen;
+        if (B.disk && r < B.diskOuterCap) capLen = (r * fabs(cos(th)) < F2D(1.8)) ? F2D(1.2) : F2D(2.2);
+        else capLen = fmax(F2D(1.2), F2D(0.22) * r);
+        h = fmin(h, capLen / spd);
+        const double hh = F2D(0.5) * h;
+        const KD64 d2 = kerrRHS64(B, E, Lz, r + hh * d1.r, th + hh * d1.th, pr + hh * d1.pr, pth + hh * d1.pth);
+        const KD64 d3_ = kerrRHS64(B, E, Lz, r + hh * d2.r, th + hh * d2.th, pr + hh * d2.pr, pth + hh * d2.pth);
+        const KD64 d4 = kerrRHS64(B, E, Lz, r + h * d3_.r, th + h * d3_.th, pr + h * d3_.pr, pth + h * d3_.pth);
+        const double h6 = h / 6.0;
+        r = r + h6 * (d1.r + 2.0 * d2.r + 2.0 * d3_.r + d4.r);
+        th = th + h6 * (d1.th + 2.0 * d2.th + 2.0 * d3_.th + d4.th);
+        ph = ph + h6 * (d1.ph + 2.0 * d2.ph + 2.0 * d3_.ph + d4.ph);
+        pr = pr + h6 * (d1.pr + 2.0 * d2.pr + 2.0 * d3_.pr + d4.pr);
+        pth = pth + h6 * (d1.pth + 2.0 * d2.pth + 2.0 * d3_.pth + d4.pth);
+        const double pi = (double)kPi;
+        if (th < F2D(1e-3)) { th = F2D(1e-3); pth = fabs(pth); }
+        else if (th > pi - F2D(1e-3)) { th = pi - F2D(1e-3); pth = -fabs(pth); }
+        dLast = d4;
+        haveDeriv = 1;
+        {
+            double s = sin(th);
+            const double cth = cos(th);
+            if (s < F2D(1e-4)) s = F2D(1e-4);
+            const double a = B.a;
+            const double Sig = r * r + a * a * cth * cth;
+            const double Del = fmax(r * r - 2.0 * B.M * r + a * a + B.Q * B.Q, F2D(1e-5));
+            const double Pp = E * (r * r + a * a) - a * Lz;
+            const double dtdl = (a * (Lz - a * E * s * s) + (r * r + a * a) * Pp / Del) / fmax(Sig, F2D(1e-8));
+            tFlight = tFlight + h * fabs(dtdl);
+        }
+        if (B.disk)
+        {
+            const double s = sin(th), c = cos(th), sp = sin(ph), cp = cos(ph);
+            const d3 pos = dk3(r * s * cp, r * c, -r * s * sp);
+            const d3 stepv = pos - prevPos;
+            const double sl = dlen(stepv);
+            if (sl > F2D(1e-6)) dir3 = stepv * (1.0 / sl);
+            if (trans > 0.01f) chordPrepare<Pol>(D, tof3(prevPos), tof3(pos), (float)(B.diskTime - tFlight), ch);
+            prevPos = pos;
+        }
+        ++step;
+        return false;
+    }
+};
+
+}  // namespace bhk

@@ -44,6 +44,31 @@ struct MathExact
     BHK_INL static float acos_(float x) { return acosf(x); }
 };
 
+// atan2 for the fast policy: atan(a) = a P(a^2) on a = min/max in [0, 1]
+// (degree-7 polynomial in a^2, least-squares fit with iterative minimax
+// reweighting; |error| <= 1.3e-7 rad in float arithmetic), then the octant
+// and quadrant reflections of atan2.
+BHK_INL float atan2Poly(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    float a = mn * __builtin_amdgcn_rcpf(mx);
+    if (mx == 0.f) a = 0.f;
+    const float s = a * a;
+    float p = -4.054606427e-03f;
+    p = p * s + 2.186309919e-02f;
+    p = p * s - 5.591253191e-02f;
+    p = p * s + 9.642212093e-02f;
+    p = p * s - 1.390863508e-01f;
+    p = p * s + 1.994656622e-01f;
+    p = p * s - 3.332986236e-01f;
+    p = p * s + 9.999993443e-01f;
+    float r = a * p;
+    if (ay > ax) r = 1.57079632679489662f - r;
+    if (__builtin_signbit(x)) r = 3.14159265358979324f - r;
+    return __builtin_copysignf(r, y);
+}
+
 struct MathFast
 {
     static constexpr bool fast = true;
@@ -55,9 +80,10 @@ struct MathFast
     BHK_INL static float sin_(float x) { return __sinf(x); }
     BHK_INL static float cos_(float x) { return __cosf(x); }
     BHK_INL static float exp_(float x) { return __expf(x); }
-    BHK_INL static float log_(float x) { return __logf(x); }
-    BHK_INL static float pow_(float x, float y) { return __powf(x, y); }
-    BHK_INL static float atan2_(float y, float x) { return atan2f(y, x); }
+    // hardware log2 / exp2 (x > 0 or x == 0 at every call site)
+    BHK_INL static float log_(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+    BHK_INL static float pow_(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+    BHK_INL static float atan2_(float y, float x) { return atan2Poly(y, x); }
     BHK_INL static float acos_(float x) { return acosf(x); }
 };
 
@@ -160,7 +186,7 @@ BHK_HD float captureRadius(const RenderParams& P)
 // ------------------------------------------------------------------ blackbody (trace.cuh:101-120)
 template <class M> BHK_INL v3 blackbody(float kelvin)
 {
-    float t = M::div(clampf(kelvin, 1200.f, 40000.f), 100.f);
+    float t = M::divc(clampf(kelvin, 1200.f, 40000.f), 100.f, 0.01f);
     float r, g, b;
     if (t <= 66.f) r = 255.f;
     else r = 329.698727446f * M::pow_(t - 60.f, -0.1332047592f);
@@ -169,8 +195,9 @@ template <class M> BHK_INL v3 blackbody(float kelvin)
     if (t >= 66.f) b = 255.f;
     else if (t <= 19.f) b = 0.f;
     else b = 138.5177312231f * M::log_(t - 10.f) - 305.0447927307f;
-    v3 srgb = mk3(M::div(clampf(r, 0.f, 255.f), 255.f), M::div(clampf(g, 0.f, 255.f), 255.f),
-                  M::div(clampf(b, 0.f, 255.f), 255.f));
+    constexpr float i255 = 1.f / 255.f;
+    v3 srgb = mk3(M::divc(clampf(r, 0.f, 255.f), 255.f, i255), M::divc(clampf(g, 0.f, 255.f), 255.f, i255),
+                  M::divc(clampf(b, 0.f, 255.f), 255.f, i255));
     return pow3<M>(srgb, 2.2f);
 }
 
@@ -258,9 +285,20 @@ template <class M> BHK_INL float emitterG(const DiskParams& D, float r, float E,
     const float a = D.a;
     float r2 = r * r;
     float m2r = D.twoM * r - D.Q2;
-    float gtt = -(1.f - M::div(m2r, r2));
-    float gtp = M::div(-a * m2r, r2);
-    float gpp = r2 + a * a + M::div(m2r * a * a, r2);
+    float gtt, gtp, gpp;
+    if constexpr (M::fast)
+    {
+        const float ir2 = M::rcp(r2);
+        gtt = -(1.f - m2r * ir2);
+        gtp = -a * m2r * ir2;
+        gpp = r2 + a * a + m2r * a * a * ir2;
+    }
+    else
+    {
+        gtt = -(1.f - M::div(m2r, r2));
+        gtp = M::div(-a * m2r, r2);
+        gpp = r2 + a * a + M::div(m2r * a * a, r2);
+    }
     float den = -(gtt + 2.f * Om * gtp + Om * Om * gpp);
     if (den < 1e-4f) return 0.f;
     float ut = M::rcp(M::sqrt_(den));

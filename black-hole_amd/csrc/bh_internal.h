@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include "../../include/bh/bhtrace.h"
 #include "bh_rays.h"
+#include "bh_band.h"
 
 namespace bhk {
 
@@ -21,6 +22,21 @@ struct TraceArgs
     int refillMin;          // idle lanes that trigger a refill
     int coop;               // 1: wavefront-cooperative disk sampling
     float2 jitter;          // sub-pixel sample position
+    const unsigned char* bandMask;  // [W*H] 1: traced by the FP64 band kernel (or null)
+};
+
+// FP64 critical-band pass: classification of every pixel, then the listed
+// rays traced in double precision.
+struct BandArgs
+{
+    RenderParams P;         // sanitized
+    BandK B;                // FP64 frame constants
+    DiskParams D;           // float disk constants (as in the FP32 trace)
+    BhRayRecord* rec;       // [W*H]
+    unsigned char* mask;    // [W*H]
+    unsigned* list;         // [W*H] band pixel indices
+    unsigned* count;        // list length (zero before classification)
+    float2 jitter;
 };
 
 struct ProbeArgs
@@ -69,6 +85,9 @@ hipError_t launchTraceExact(const TraceArgs& A, const LaunchShape& S, hipStream_
 hipError_t launchTraceFast(const TraceArgs& A, const LaunchShape& S, hipStream_t st);
 int traceOccupancyExact(int kerr, int wgSize, int coop, int tiles);
 int traceOccupancyFast(int kerr, int wgSize, int coop, int tiles);
+hipError_t launchBandClassify(const BandArgs& A, hipStream_t st);
+hipError_t launchBandTraceExact(const BandArgs& A, int blocks, hipStream_t st);
+hipError_t launchBandTraceFast(const BandArgs& A, int blocks, hipStream_t st);
 hipError_t launchProbeExact(const ProbeArgs& A, hipStream_t st);
 hipError_t launchProbeFast(const ProbeArgs& A, hipStream_t st);
 hipError_t launchTraceBatchExact(const BatchArgs& A, const LaunchShape& S, hipStream_t st);

@@ -183,6 +183,29 @@ BHK_INL KDeriv kerrRHS(const FrameK& K, float E, float Lz, float r, float pr, fl
     float Del = r2 - K.twoM * r + a2 + K.Q2;
     if (Del < 1e-6f) Del = 1e-6f;
     float Pp = E * (r2 + a2) - a * Lz;
+    KDeriv d;
+    if constexpr (M::fast)
+    {
+        // three reciprocals (1/s, 1/Delta, 1/Sigma) instead of five divisions
+        const float is = M::rcp(s), is2 = is * is;
+        const float iDel = M::rcp(Del), iDel2 = iDel * iDel;
+        const float invS = M::rcp(Sig), invS2 = invS * invS;
+        float B = Lz * is - a * E * s;
+        float W = B * B - Pp * Pp * iDel;
+        float Kc = Del * pr * pr + pth * pth + W;
+        if (kWant) Kout = Kc;
+        float dDel = 2.f * r - K.twoM;
+        float dP = 2.f * r * E;
+        float Wr = -(2.f * Pp * dP * Del - Pp * Pp * dDel) * iDel2;
+        float dB = -Lz * cth * is2 - a * E * cth;
+        float Wth = 2.f * B * dB;
+        d.r = Del * pr * invS;
+        d.th = pth * invS;
+        d.ph = ((Lz * is2 - a * E) + a * Pp * iDel) * invS;
+        d.pr = -0.5f * invS * (dDel * pr * pr + Wr) + r * invS2 * Kc;
+        d.pth = -0.5f * invS * Wth - a2 * s * cth * invS2 * Kc;
+        return d;
+    }
     float B = M::div(Lz, s) - a * E * s;
     float W = B * B - M::div(Pp * Pp, Del);
     float Kc = Del * pr * pr + pth * pth + W;
@@ -194,7 +217,6 @@ BHK_INL KDeriv kerrRHS(const FrameK& K, float E, float Lz, float r, float pr, fl
     float Wth = 2.f * B * dB;
     float invS = M::rcp(Sig);
     float invS2 = invS * invS;
-    KDeriv d;
     d.r = Del * pr * invS;
     d.th = pth * invS;
     d.ph = ((M::div(Lz, s2) - a * E) + M::div(a * Pp, Del)) * invS;

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void traceKernel(TraceArgs A)
                 {
                     int px, py;
                     pixelOfId(id, A.tilesX, px, py);
-                    if (px < P.width && py < P.height)
+                    if (px < P.width && py < P.height && !(A.bandMask && A.bandMask[py * P.width + px]))
                     {
                         const float fx = Pol::div((float)px + jit.x, (float)P.width);
                         const float fy = Pol::div((float)py + jit.y, (float)P.height);
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void traceKernel(TraceArgs A)
 // single-view launch when raysPerRefill == 64 and by the batched launch.
 template <class Ray, bool kCoop>
 BHK_INL void traceTile(const RenderParams& P, const FrameK& K, BhRayRecord* rec, int tilesX, float2 jit,
-                       unsigned tile, unsigned lane)
+                       unsigned tile, unsigned lane, const unsigned char* bandMask = nullptr)
 {
     const int px = (int)((tile % (unsigned)tilesX) * 8u + (lane & 7u));
     const int py = (int)((tile / (unsigned)tilesX) * 8u + (lane >> 3));
@@ -137,7 +137,7 @@ BHK_INL void traceTile(const RenderParams& P, const FrameK& K, BhRayRecord* rec,
     ch.n = 0;
     bool active = false;
     int pix = 0;
-    if (px < P.width && py < P.height)
+    if (px < P.width && py < P.height && !(bandMask && bandMask[py * P.width + px]))
     {
         const float fx = Pol::div((float)px + jit.x, (float)P.width);
         const float fy = Pol::div((float)py + jit.y, (float)P.height);
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Pol::fast ?
     {
         const unsigned t = nextTile(A.counter, lane);
         if (t >= nTiles) break;
-        traceTile<Ray, kCoop>(A.P, A.K, A.rec, A.tilesX, A.jitter, t, lane);
+        traceTile<Ray, kCoop>(A.P, A.K, A.rec, A.tilesX, A.jitter, t, lane, A.bandMask);
     }
 }
 
@@ -214,6 +214,35 @@ __global__ __launch_bounds__(256) void traceBatchKernel(const ViewDesc* __restri
             const unsigned d = atomicAdd(&A.viewDone[v], 1u) + 1u;
             if (d == (unsigned)V.tileCount) A.stamps[v] = (unsigned long long)wall_clock64();
         }
+    }
+}
+
+// ---------------------------------------------------------------- FP64 band rays
+// One band ray per thread over the classification list (grid-stride); the
+// geometry is double precision, the disk chords are sampled in float with
+// the object's policy, exactly as for the FP32 rays.
+template <class Ray>
+__global__ __launch_bounds__(64) void traceBandKernel(BandArgs A)
+{
+    const unsigned n = *A.count;
+    const d3 o = dk3(A.P.camPos.x, A.P.camPos.y, A.P.camPos.z);
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const unsigned pix = A.list[i];
+        const int px = (int)(pix % (unsigned)A.P.width), py = (int)(pix / (unsigned)A.P.width);
+        Ray R;
+        R.init(A.B, o, cameraDir64(A.P, px, py, A.jitter));
+        Chord ch;
+        bool done = (R.status != RAY_RUNNING);
+        while (!done)
+        {
+            done = R.advance(A.B, A.D, ch);
+            if (ch.n > 0) diskChordSerial<Pol>(A.D, R.diskE(), R.diskLz(), ch, R.acc, R.trans);
+        }
+        float4* out = reinterpret_cast<float4*>(A.rec + pix);
+        const int flags = (R.status == RAY_ESCAPED ? 1 : 2) | (R.step << 8);
+        out[0] = make_float4((float)R.escDir.x, (float)R.escDir.y, (float)R.escDir.z, __int_as_float(flags));
+        out[1] = make_float4(R.acc.x, R.acc.y, R.acc.z, R.trans);
     }
 }
 
@@ -307,6 +336,13 @@ int BHK_SUFFIX(traceBatchOccupancy)(int wgSize)
                                                      0) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
+}
+
+hipError_t BHK_SUFFIX(launchBandTrace)(const BandArgs& A, int blocks, hipStream_t st)
+{
+    if (A.B.kerr) hipLaunchKernelGGL(traceBandKernel<KerrRay64<Pol>>, dim3(blocks), dim3(64), 0, st, A);
+    else hipLaunchKernelGGL(traceBandKernel<SphRay64<Pol>>, dim3(blocks), dim3(64), 0, st, A);
+    return hipGetLastError();
 }
 
 hipError_t BHK_SUFFIX(launchProbe)(const ProbeArgs& Ain, hipStream_t st)

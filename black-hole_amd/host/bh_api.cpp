@@ -43,11 +43,18 @@ struct Scratch
     int viewCap = 0;
     int lastViews = 0;
     std::vector<hipEvent_t> viewEvents;   // [0] batch start, [v+1] view v done
+    // FP64 band pass: per-pixel mask, band pixel list (count in counter[16])
+    unsigned char* bandMask = nullptr;
+    unsigned* bandList = nullptr;
+    size_t bandCap = 0;
+    hipEvent_t bandEv[2] = {nullptr, nullptr};   // timing: band pass start / end
 };
 
 std::mutex g_mu;
 std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 BhSchedule g_sched = {256, 4096, 64, 1, 0};   // wgSize, stepsPerChunk, raysPerRefill, mathMode, diskCoop
+BhPrecision g_prec = {0, 3e-2f, 1e-3f};
+constexpr int kBandCountSlot = 16;            // counter[16]: band list length (own 64-byte line)
 std::atomic<int> g_timers{0};
 std::pair<int, hipStream_t> g_lastTimed{-1, nullptr};
 int g_numCU[64] = {0};
@@ -130,6 +137,81 @@ BhSchedule currentSchedule()
     return g_sched;
 }
 
+BhPrecision currentPrecision()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_prec;
+}
+
+hipError_t getBandScratch(Scratch* S, size_t n, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n <= S->bandCap) return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (S->bandMask)
+    {
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        (void)hipFree(S->bandMask);
+        (void)hipFree(S->bandList);
+        S->bandMask = nullptr; S->bandList = nullptr; S->bandCap = 0;
+    }
+    e = hipMalloc(&S->bandMask, n);
+    if (e == hipSuccess) e = hipMalloc(&S->bandList, n * sizeof(unsigned));
+    if (e != hipSuccess) return e;
+    S->bandCap = n;
+    return hipSuccess;
+}
+
+// FP64 frame constants: RenderParams floats widened, derived values in
+// double with the reference's float literals widened (trace.cuh:466-890).
+bhk::BandK makeBandK(const RenderParams& P, const BhPrecision& pr)
+{
+    bhk::BandK B{};
+    const bool kerr = (P.model == BH_KERR || P.model == BH_KERR_NEWMAN);
+    B.kerr = kerr ? 1 : 0;
+    B.M = P.M;
+    B.a = kerr ? (double)P.aSpin : 0.0;
+    B.Q = P.Qc;
+    B.rPlus = P.rPlus;
+    B.rPhoton = P.rPhoton;
+    B.dPhi = P.dPhi;
+    B.diskOuter = P.diskOuter;
+    B.diskTime = P.diskTime;
+    B.rCap = (B.rPlus > (double)1e-6f) ? B.rPlus : (double)bhk::kSingCutOverM * B.M;
+    B.rFloor = std::max(B.rPlus, B.rCap);
+    B.escR = (double)120.f * B.M;
+    B.capR = B.rCap * (double)1.002f + (double)1e-3f;
+    B.rPhotonRefine = (double)1.5f * B.rPhoton;
+    B.diskOuterCap = B.diskOuter * (double)1.3f;
+    B.sixM = 6.0 * B.M;
+    B.kRad = (double)8.0f * B.dPhi;
+    B.kAng = (double)2.2f * B.dPhi;
+    B.uHor = 1.0 / std::max(B.rCap, (double)1e-6f);
+    B.uEscape = 1.0 / B.escR;
+    B.invNineM = 1.0 / (9.0 * B.M);
+    B.uBudget = 1.0 / std::max((double)1.5f * B.rPhoton, 3.0 * B.M);
+    B.hFloor = B.dPhi * (double)0.03f;
+    B.maxSteps = P.maxSteps;
+    B.disk = P.diskEnabled != 0;
+    // spherical models: b_c^2 = r_ph^2 / f(r_ph), r_ph = (3M + sqrt(9M^2 - 8Q^2)) / 2
+    const double Q2 = B.Q * B.Q, disc = 9.0 * B.M * B.M - 8.0 * Q2;
+    B.bCrit2 = 0.0;
+    if (!kerr && disc >= 0.0)
+    {
+        const double rph = 0.5 * (3.0 * B.M + std::sqrt(disc));
+        const double f = 1.0 - 2.0 * B.M / rph + Q2 / (rph * rph);
+        if (f > 0.0) B.bCrit2 = rph * rph / f;
+    }
+    // Kerr models: scan the radial potential over [r+ (1 + 1e-3), 5M]
+    const double rp = bhHorizonOuterFP64(B.M, B.a, B.Q);
+    B.rScanLo = rp > 0.0 ? rp * 1.001 : (double)bhk::kSingCutOverM * B.M;
+    B.rScanHi = std::max(5.0 * B.M, B.rScanLo * 1.5);
+    B.critBand = pr.critBand;
+    B.polarBand = pr.polarBand;
+    return B;
+}
+
 inline bool isKerr(int model) { return model == BH_KERR || model == BH_KERR_NEWMAN; }
 
 float2 jitterOf(int accumMode, int sampleIndex)
@@ -141,10 +223,32 @@ float2 jitterOf(int accumMode, int sampleIndex)
     return make_float2(x - std::floor(x), y - std::floor(y));
 }
 
-hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, unsigned* counter, const BhSchedule& sc,
+hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, const BhSchedule& sc,
                         hipStream_t st)
 {
+    unsigned* counter = Sc->counter;
+    const BhPrecision pr = currentPrecision();
+    const bool band = pr.fp64Band != 0 && P.width > 0 && P.height > 0;
+    bhk::BandArgs BA;
+    if (band)
+    {
+        const size_t n = (size_t)P.width * (size_t)P.height;
+        hipError_t e = getBandScratch(Sc, n, st);
+        if (e != hipSuccess) return e;
+        BA.P = P;
+        BA.B = makeBandK(P, pr);
+        BA.D = bhk::makeDisk(P, BA.B.kerr ? P.aSpin : 0.f, P.Qc);
+        BA.rec = rec;
+        BA.mask = Sc->bandMask;
+        BA.list = Sc->bandList;
+        BA.count = counter + kBandCountSlot;
+        BA.jitter = jitterOf(P.accumMode, P.sampleIndex);
+        e = bhk::launchResetCounter(BA.count, st);
+        if (e == hipSuccess) e = bhk::launchBandClassify(BA, st);
+        if (e != hipSuccess) return e;
+    }
     bhk::TraceArgs A;
+    A.bandMask = band ? Sc->bandMask : nullptr;
     A.P = P;
     A.rec = rec;
     A.counter = counter;
@@ -169,7 +273,19 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, unsigned* count
     if (S.blocks < 1) S.blocks = 1;
     hipError_t e = bhk::launchResetCounter(counter, st);
     if (e != hipSuccess) return e;
-    return sc.mathMode ? bhk::launchTraceFast(A, S, st) : bhk::launchTraceExact(A, S, st);
+    e = sc.mathMode ? bhk::launchTraceFast(A, S, st) : bhk::launchTraceExact(A, S, st);
+    if (e != hipSuccess || !band) return e;
+    const bool timed = g_timers.load() != 0;
+    if (timed && !Sc->bandEv[0])
+    {
+        if (hipEventCreate(&Sc->bandEv[0]) != hipSuccess || hipEventCreate(&Sc->bandEv[1]) != hipSuccess)
+            return hipErrorOutOfMemory;
+    }
+    if (timed) (void)hipEventRecord(Sc->bandEv[0], st);
+    const int blocks = 4 * numCUs(dev);
+    e = sc.mathMode ? bhk::launchBandTraceFast(BA, blocks, st) : bhk::launchBandTraceExact(BA, blocks, st);
+    if (timed) (void)hipEventRecord(Sc->bandEv[1], st);
+    return e;
 }
 
 void recordEvent(Scratch* S, int i, hipStream_t st)
@@ -198,7 +314,7 @@ hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bl
     const BhSchedule sc = currentSchedule();
     const bool timed = g_timers.load() != 0;
     if (timed) recordEvent(S, 0, st);
-    e = enqueueTrace(P, S->rec, S->counter, sc, st);
+    e = enqueueTrace(P, S->rec, S, sc, st);
     if (e != hipSuccess) return e;
     if (timed) recordEvent(S, 1, st);
     if (singlePass)
@@ -402,7 +518,7 @@ bh_error_t bhTraceRecords(BhRayRecord* devOut, const RenderParams& p, bh_stream_
     Scratch* S = nullptr;
     hipError_t e = getScratch(st, 0, &S);
     if (e != hipSuccess) return e;
-    e = enqueueTrace(P, devOut, S->counter, currentSchedule(), st);
+    e = enqueueTrace(P, devOut, S, currentSchedule(), st);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -535,6 +651,48 @@ bh_error_t bhSetSchedule(const BhSchedule* s)
 
 void bhEnableKernelTimers(int enable) { g_timers.store(enable ? 1 : 0); }
 
+void bhGetPrecision(BhPrecision* p)
+{
+    if (p) *p = currentPrecision();
+}
+bh_error_t bhSetPrecision(const BhPrecision* p)
+{
+    if (!p) return hipErrorInvalidValue;
+    if (p->fp64Band != 0 && p->fp64Band != 1) return hipErrorInvalidValue;
+    if (!(p->critBand >= 0.f) || !(p->polarBand >= 0.f) || !std::isfinite(p->critBand) ||
+        !std::isfinite(p->polarBand))
+        return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prec = *p;
+    return hipSuccess;
+}
+bh_error_t bhLastBandStats(unsigned* count, float* bandMs, bh_stream_t stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    Scratch* S = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_scratch.find({dev, st});
+        if (it == g_scratch.end() || !it->second.counter) return hipErrorNotReady;
+        S = &it->second;
+    }
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (count)
+    {
+        e = hipMemcpy(count, S->counter + kBandCountSlot, sizeof(unsigned), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return e;
+    }
+    if (bandMs)
+    {
+        *bandMs = 0.f;
+        if (S->bandEv[0]) e = hipEventElapsedTime(bandMs, S->bandEv[0], S->bandEv[1]);
+    }
+    return e;
+}
 bh_error_t bhLastKernelTimes(float* traceMs, float* shadeMs, float* postMs)
 {
     Scratch* S = nullptr;
@@ -562,6 +720,7 @@ bh_error_t bhDeviceFree(void* ptr) { return hipFree(ptr); }
 bh_error_t bhCopyToDevice(void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); }
 bh_error_t bhCopyToHost(void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost); }
 bh_error_t bhDeviceFill(void* dst, unsigned char value, size_t bytes) { return hipMemset(dst, value, bytes); }
+bh_error_t bhCopyDevice(void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice); }
 bh_error_t bhDeviceSynchronize(void) { return hipDeviceSynchronize(); }
 bh_error_t bhStreamCreate(bh_stream_t* s)
 {

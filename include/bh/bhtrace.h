@@ -183,6 +183,28 @@ void bhGetSchedule(BhSchedule* s);
 bh_error_t bhSetSchedule(const BhSchedule* s);   /* -1 (hipErrorInvalidValue-like) on bad values */
 
 /* ------------------------------------------------------------------------ */
+/* Precision: FP64 critical-band rays                                        */
+/* ------------------------------------------------------------------------ */
+/* With fp64Band set, every frame first classifies each pixel's ray from its
+ * FP64 invariants: rays near the critical (photon-ring) curve, |delta| <
+ * critBand with delta = min_r R(r)/(r^2+a^2)^2 of the radial potential
+ * (1 - b^2/b_c^2 for the spherical models), and Kerr rays whose polar
+ * turning point has sin^2(theta) < polarBand, are traced in double
+ * precision on a second stream while the FP32 kernel traces the rest.
+ * No counterpart in the reference (which is FP32 throughout). */
+typedef struct BhPrecision
+{
+    int   fp64Band;     /* 0 off (pure FP32, the reference's arithmetic), 1 on */
+    float critBand;     /* default 3e-2 */
+    float polarBand;    /* default 1e-3 */
+} BhPrecision;
+void bhGetPrecision(BhPrecision* p);
+bh_error_t bhSetPrecision(const BhPrecision* p);   /* rejects negative or non-finite bands */
+/* Band rays of the last frame traced on `stream` and, when kernel timers
+ * are on, the FP64 band kernel's time in ms (synchronizes the stream). */
+bh_error_t bhLastBandStats(unsigned* count, float* bandMs, bh_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Timing (HIP events recorded on the launch stream)                        */
 /* ------------------------------------------------------------------------ */
 /* When enabled, launchRenderPipeline/launchRenderKernel record events around
@@ -200,6 +222,7 @@ bh_error_t  bhDeviceFree(void* ptr);
 bh_error_t  bhCopyToDevice(void* dst, const void* src, size_t bytes);
 bh_error_t  bhCopyToHost(void* dst, const void* src, size_t bytes);
 bh_error_t  bhDeviceFill(void* dst, unsigned char value, size_t bytes);
+bh_error_t  bhCopyDevice(void* dst, const void* src, size_t bytes);   /* device to device */
 bh_error_t  bhDeviceSynchronize(void);
 bh_error_t  bhStreamCreate(bh_stream_t* s);
 bh_error_t  bhStreamDestroy(bh_stream_t s);

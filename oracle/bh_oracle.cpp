@@ -317,8 +317,65 @@ void bho_validate_derive(int model, float* M, float* aStar, float* q, int allowN
 
 float bho_isco_numeric(float M, float a, float Q) { return hIscoNumeric(M, a, Q); }
 
+// FP64 critical-band switch for the float paths (mode 0 of bho_trace_frame,
+// bho_render_pipeline, bho_render_kernel): band rays are traced in double.
+static BandCfg g_band = {false, 3e-2, 1e-3};
+
+void bho_set_band(int on, double crit, double polar)
+{
+    g_band.on = on != 0;
+    g_band.crit = crit;
+    g_band.polar = polar;
+}
+
+static bool pixelInBand(const RenderParams& P, int px, int py, float jx, float jy)
+{
+    if (!g_band.on) return false;
+    V3<double> cam = mk3((double)P.camPos.x, (double)P.camPos.y, (double)P.camPos.z);
+    return rayInBand(P, cam, cameraDirD(P, px, py, (double)jx, (double)jy), g_band);
+}
+
+// HDR sample of the float path with the band rule applied.
+static V3<float> sampleBanded(const RenderParams& P, int px, int py, float jx, float jy)
+{
+    if (pixelInBand(P, px, py, jx, jy)) return renderSampleHDR<double>(P, px, py, (double)jx, (double)jy);
+    return renderSampleHDR<float>(P, px, py, jx, jy);
+}
+
+// Band statistics for given (M, a, Q) and photon constants xi = Lz/E,
+// eta = Carter/E^2 (tests of the band restatement).
+double bho_band_delta(float M, float a, float Q, double xi, double eta)
+{
+    RenderParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.model = BH_KERR_NEWMAN;
+    P.M = M; P.aSpin = a; P.Qc = Q;
+    return bandCriticalDelta(P, xi, eta);
+}
+double bho_band_polar(double a, double xi, double eta) { return bandPolarSin2(a, xi, eta); }
+
+// Band classification of a frame: out[W*H] = 1 for band pixels; delta and
+// polar (may be NULL) receive the two band statistics (polar 2.0 when not
+// applicable).
+void bho_band_mask(const RenderParams* Pp, double crit, double polar, int nthreads, unsigned char* out,
+                   double* delta, double* pol)
+{
+    const RenderParams P = *Pp;
+    BandCfg cfg = {true, crit, polar};
+    V2<float> j = subpixelJitter(P.accumMode, P.sampleIndex);
+    V3<double> cam = mk3((double)P.camPos.x, (double)P.camPos.y, (double)P.camPos.z);
+    forTiles(P.width, 0, 1, P.height, nthreads, [&](int px, int py, int jr) {
+        size_t o = (size_t)jr * P.width + px;
+        double d = 0.0, pl = 2.0;
+        out[o] = rayInBand(P, cam, cameraDirD(P, px, py, (double)j.x, (double)j.y), cfg, &d, &pl) ? 1 : 0;
+        if (delta) delta[o] = d;
+        if (pol) pol[o] = pl;
+    });
+}
+
 // Trace a frame (or every rowStep-th row from rowStart) with the sample
-// position given by accumMode/sampleIndex.  mode 0 float, 1 double.
+// position given by accumMode/sampleIndex.  mode 0 float (band rays in
+// double when the band switch is on), 1 double.
 //   hdr4  [nRows*W*4]: HDR rgb + outcome (1 escaped, 0 captured)
 //   esc3  [nRows*W*3] double exit directions (may be NULL)
 //   steps [nRows*W]   (may be NULL)
@@ -338,7 +395,7 @@ void bho_trace_frame(const RenderParams* Pp, int mode, int rowStart, int rowStep
         int st;
         V3<float> acc;
         float tr;
-        if (mode == 1)
+        if (mode == 1 || (mode == 0 && pixelInBand(P, px, py, j.x, j.y)))
         {
             TraceResult<double> t;
             hdr = renderSampleHDR<double>(P, px, py, (double)j.x, (double)j.y, &t);
@@ -416,7 +473,7 @@ void bho_render_pipeline(const RenderParams* Pp, float* accum, unsigned char* ou
     const int W = P.width, H = P.height;
     V2<float> j = subpixelJitter(P.accumMode, P.sampleIndex);
     forTiles(W, 0, 1, H, nthreads, [&](int px, int py, int) {
-        V3<float> cur = renderSampleHDR<float>(P, px, py, j.x, j.y);
+        V3<float> cur = sampleBanded(P, px, py, j.x, j.y);
         size_t idx = (size_t)py * W + px;
         float* pv = accum + 4 * idx;
         V3<float> prev = mk3(pv[0], pv[1], pv[2]);
@@ -434,7 +491,7 @@ void bho_render_kernel(const RenderParams* Pp, unsigned char* out, int nthreads)
     hSanitize(P);
     P.bloomEnabled = 0;
     forTiles(P.width, 0, 1, P.height, nthreads, [&](int px, int py, int) {
-        V3<float> hdr = renderSampleHDR<float>(P, px, py, 0.5f, 0.5f);
+        V3<float> hdr = sampleBanded(P, px, py, 0.5f, 0.5f);
         finalizePixel<float>(hdr, mk3(0.f, 0.f, 0.f), P, px, py, out + 4 * ((size_t)py * P.width + px));
     });
 }

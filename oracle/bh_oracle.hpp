@@ -871,6 +871,142 @@ inline V3<Shade<T>> renderSampleHDR(const RenderParams& P, int px, int py, T jx,
     return hdr;
 }
 
+// ---------------------------------------------------------------- FP64 critical band (no reference counterpart)
+// Restates the library's band test: a ray is traced in double when its FP64
+// invariants put it near the critical curve, |min_r R(r)/(r^2+a^2)^2| <
+// crit over r in [r+ (1+1e-3), max(5M, 1.5 r_lo)] (radial potential of
+// Kerr-Newman at unit energy, Bardeen 1973), or, for the Kerr models, when
+// the polar turning point has sin^2(theta) < polar.  Spherical models use
+// the closed form 1 - b^2/b_c^2, b_c^2 = r_ph^2 / f(r_ph).
+struct BandCfg { bool on; double crit, polar; };
+
+struct BandInv { double E, Lz, carter; bool ok; };
+
+inline BandInv bandInvariants(const RenderParams& Pr, V3<double> o, V3<double> dir)
+{
+    const bool kerr = (Pr.model == BH_KERR || Pr.model == BH_KERR_NEWMAN);
+    const double M = Pr.M, a = kerr ? (double)Pr.aSpin : 0.0, Q = Pr.Qc;
+    const double r0 = len3(o);
+    const double cth0 = clampf(o.y / r0, -1.0, 1.0);
+    const double th0 = std::acos(cth0), ph0 = std::atan2(-o.z, o.x);
+    V3<double> er, eth, eph;
+    kerrBasis(th0, ph0, er, eth, eph);
+    const double dtc = dot(dir, eth), dpc = dot(dir, eph);
+    double s0 = std::sin(th0);
+    if (s0 < (double)1e-4f) s0 = (double)1e-4f;
+    const double s02 = s0 * s0;
+    const double Sig0 = r0 * r0 + a * a * cth0 * cth0;
+    const double m2r0 = 2.0 * M * r0 - Q * Q;
+    double gtt = -(1.0 - m2r0 / Sig0);
+    if (gtt > (double)-1e-5f) gtt = (double)-1e-5f;
+    const double gtp = -a * m2r0 * s02 / Sig0;
+    const double gpp = (r0 * r0 + a * a + m2r0 * a * a * s02 / Sig0) * s02;
+    const double A = 1.0 / std::sqrt(std::max(gpp - gtp * gtp / gtt, (double)1e-8f));
+    const double ptUp = 1.0 / std::sqrt(-gtt) - dpc * A * gtp / gtt;
+    const double pfUp = dpc * A;
+    BandInv I;
+    I.E = -(gtt * ptUp + gtp * pfUp);
+    I.Lz = gtp * ptUp + gpp * pfUp;
+    const double pth = dtc * std::sqrt(Sig0);
+    I.carter = pth * pth + cth0 * cth0 * (I.Lz * I.Lz / s02 - a * a * I.E * I.E);
+    I.ok = (I.E > 1e-12) && std::isfinite(I.E) && std::isfinite(I.Lz);
+    return I;
+}
+
+inline double bandRadialPot(double M, double a, double Q, double xi, double eta, double r)
+{
+    const double ra = r * r + a * a;
+    const double P = ra - a * xi;
+    const double Del = r * r - 2.0 * M * r + a * a + Q * Q;
+    const double xa = xi - a;
+    return (P * P - Del * (eta + xa * xa)) / (ra * ra);
+}
+
+// min of the normalised radial potential: 32 samples, then 24 golden-section
+// steps on the bracket of the smallest sample.
+inline double bandCriticalDelta(const RenderParams& Pr, double xi, double eta)
+{
+    const double M = Pr.M, a = Pr.aSpin, Q = Pr.Qc;
+    const double disc = M * M - a * a - Q * Q;
+    const double rp = disc >= 0.0 ? M + std::sqrt(disc) : 0.0;
+    const double lo = rp > 0.0 ? rp * 1.001 : (double)SING_CUT_OVER_M * M;
+    const double hi = std::max(5.0 * M, lo * 1.5);
+    const int N = 32;
+    const double dr = (hi - lo) / (double)(N - 1);
+    int best = 0;
+    double vBest = bandRadialPot(M, a, Q, xi, eta, lo);
+    for (int i = 1; i < N; ++i)
+    {
+        double v = bandRadialPot(M, a, Q, xi, eta, lo + dr * (double)i);
+        if (v < vBest) { vBest = v; best = i; }
+    }
+    double x0 = lo + dr * (double)(best > 0 ? best - 1 : 0);
+    double x3 = lo + dr * (double)(best < N - 1 ? best + 1 : N - 1);
+    const double gr = 0.6180339887498949;
+    double x1 = x3 - gr * (x3 - x0), x2 = x0 + gr * (x3 - x0);
+    double f1 = bandRadialPot(M, a, Q, xi, eta, x1), f2 = bandRadialPot(M, a, Q, xi, eta, x2);
+    for (int it = 0; it < 24; ++it)
+    {
+        if (f1 < f2) { x3 = x2; x2 = x1; f2 = f1; x1 = x3 - gr * (x3 - x0); f1 = bandRadialPot(M, a, Q, xi, eta, x1); }
+        else { x0 = x1; x1 = x2; f1 = f2; x2 = x0 + gr * (x3 - x0); f2 = bandRadialPot(M, a, Q, xi, eta, x2); }
+    }
+    return std::min(vBest, std::min(f1, f2));
+}
+
+// sin^2 of the polar turning angle: larger root u = cos^2 theta of
+// a^2 u^2 - (a^2 - eta - xi^2) u - eta = 0.
+inline double bandPolarSin2(double a, double xi, double eta)
+{
+    const double a2 = a * a, B = a2 - eta - xi * xi;
+    const double D = std::sqrt(std::max(B * B + 4.0 * a2 * eta, 0.0));
+    double u;
+    if (B <= 0.0) u = (D - B > 0.0) ? 2.0 * eta / (D - B) : 1.0;
+    else u = (a2 > 0.0) ? (B + D) / (2.0 * a2) : 1.0;
+    return 1.0 - u;
+}
+
+inline bool rayInBand(const RenderParams& Pr, V3<double> o, V3<double> dir, const BandCfg& cfg, double* deltaOut = nullptr,
+                      double* polarOut = nullptr)
+{
+    if (Pr.model == BH_KERR || Pr.model == BH_KERR_NEWMAN)
+    {
+        BandInv I = bandInvariants(Pr, o, dir);
+        if (!I.ok) return true;
+        const double xi = I.Lz / I.E, eta = I.carter / (I.E * I.E);
+        const double d = bandCriticalDelta(Pr, xi, eta), pol = bandPolarSin2(Pr.aSpin, xi, eta);
+        if (deltaOut) *deltaOut = d;
+        if (polarOut) *polarOut = pol;
+        return std::fabs(d) < cfg.crit || pol < cfg.polar;
+    }
+    const double M = Pr.M, Q2 = (double)Pr.Qc * (double)Pr.Qc;
+    const double disc = 9.0 * M * M - 8.0 * Q2;
+    double bc2 = 0.0;
+    if (disc >= 0.0)
+    {
+        const double rph = 0.5 * (3.0 * M + std::sqrt(disc));
+        const double f = 1.0 - 2.0 * M / rph + Q2 / (rph * rph);
+        if (f > 0.0) bc2 = rph * rph / f;
+    }
+    const double r0 = len3(o);
+    V3<double> e1 = o / r0;
+    V3<double> perp = dir - e1 * dot(dir, e1);
+    const double pl = len3(perp);
+    if (pl < (double)1e-6f || !(bc2 > 0.0)) return false;
+    const double f0 = std::max(1.0 - 2.0 * M / r0 + Q2 / (r0 * r0), (double)1e-6f);
+    const double b2 = (r0 * pl) * (r0 * pl) / f0;
+    const double d = 1.0 - b2 / bc2;
+    if (deltaOut) *deltaOut = d;
+    return std::fabs(d) < cfg.crit;
+}
+
+// Camera ray of a pixel in double (trace.cuh:910-912).
+inline V3<double> cameraDirD(const RenderParams& P, int px, int py, double jx, double jy)
+{
+    double fx = (double(float(px)) + jx) / double(float(P.width));
+    double fy = (double(float(py)) + jy) / double(float(P.height));
+    return cameraRayDir<double>(P, fx, fy);
+}
+
 // ---------------------------------------------------------------- post process (post_process.cuh)
 template <class S> inline V3<S> acesToneMap(V3<S> x)
 {

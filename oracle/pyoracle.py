@@ -35,6 +35,10 @@ class Oracle:
             "bho_isco_numeric": (C.c_float, [C.c_float, C.c_float, C.c_float]),
             "bho_trace_frame": (None, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
             "bho_count_flops": (None, [_P, C.c_int, C.c_int, C.c_int, _vp]),
+            "bho_set_band": (None, [C.c_int, C.c_double, C.c_double]),
+            "bho_band_delta": (C.c_double, [C.c_float, C.c_float, C.c_float, C.c_double, C.c_double]),
+            "bho_band_polar": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+            "bho_band_mask": (None, [_P, C.c_double, C.c_double, C.c_int, _vp, _vp, _vp]),
             "bho_render_pipeline": (None, [_P, _vp, _vp, C.c_int]),
             "bho_render_kernel": (None, [_P, _vp, C.c_int]),
             "bho_post": (None, [_P, _vp, _vp]),
@@ -75,6 +79,18 @@ class Oracle:
         self.L.bho_trace_frame(C.byref(P), mode, row_start, row_step, threads, _ptr(hdr), _ptr(esc), _ptr(steps),
                                _ptr(disk))
         return hdr, esc, steps, disk
+
+    def set_band(self, on, crit=3e-2, polar=1e-3):
+        """FP64 band rule for the float paths (trace_frame mode 0, render_*)."""
+        self.L.bho_set_band(int(on), C.c_double(crit), C.c_double(polar))
+
+    def band_mask(self, P, crit=3e-2, polar=1e-3, threads=0):
+        W, H = P.width, P.height
+        m = np.zeros((H, W), np.uint8)
+        d = np.zeros((H, W), np.float64)
+        pol = np.zeros((H, W), np.float64)
+        self.L.bho_band_mask(C.byref(P), C.c_double(crit), C.c_double(polar), threads, _ptr(m), _ptr(d), _ptr(pol))
+        return m, d, pol
 
     def count_flops(self, P, row_start=0, row_step=1, threads=0):
         out = np.zeros(5, np.uint64)
