@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-row-step", type=int, default=4)
     ap.add_argument("--math", type=int, default=None, help="0 exact, 1 fast (default: library default)")
+    ap.add_argument("--band", type=int, default=None, help="FP64 critical-band pass 0/1 (default: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -96,7 +97,10 @@ def main():
         sys.exit(1)
     if a.math is not None:
         bt.set_schedule(math=a.math)
+    if a.band is not None:
+        bt.set_precision(band=a.band)
     sched = bt.get_schedule()
+    prec = bt.get_precision()
     frames = [bt.workload(WORKLOAD, f) for f in range(3)]
     W, H = frames[0].width, frames[0].height
     fb = bt.Framebuffer(W, H)
@@ -121,6 +125,7 @@ def main():
         per_frame.append(f)
     bt.synchronize()
     total = time.perf_counter() - t0
+    band_rays, _ = bt.last_band_stats()
     lib.bhEnableKernelTimers(0)
 
     ms_per_step = total * 1e3 / a.steps
@@ -138,7 +143,11 @@ def main():
         "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": WORKLOAD, "width": W, "height": H, "disk_times": [0.0, 0.5, 1.0],
-                   "path": "plain FP32 RK4 (no surrogate)", "math_mode": "fast" if sched.mathMode else "exact",
+                   "path": "plain RK4: FP32 bulk + FP64 critical-band rays (no surrogate)" if prec.fp64Band
+                   else "plain FP32 RK4 (no surrogate)",
+                   "fp64_band": {"on": prec.fp64Band, "crit": prec.critBand, "polar": prec.polarBand,
+                                 "band_rays": band_rays},
+                   "math_mode": "fast" if sched.mathMode else "exact",
                    "wg_size": sched.wgSize, "steps_per_chunk": sched.stepsPerChunk},
         "kernel_ms": {"trace": round(avg_trace, 3), "shade": round(sum(shade_ms) / len(shade_ms), 3),
                       "bloom_tonemap": round(sum(post_ms) / len(post_ms), 3)},

@@ -112,6 +112,17 @@ SIGNATURES = [
     ("bhFramebufferReadback", C.c_int, [_vp, _vp, _vp, _vp]),
     ("bhGetSchedule", None, [C.POINTER(BhSchedule)]),
     ("bhSetSchedule", C.c_int, [C.POINTER(BhSchedule)]),
+    ("bhAutotune", C.c_int, [_P, C.c_int, C.POINTER(BhSchedule), C.POINTER(C.c_float), _vp]),
+    ("bhAutotuneClear", None, []),
+    ("bhAccumCreate", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("bhAccumDestroy", None, [_vp]),
+    ("bhAccumColor", _vp, [_vp]),
+    ("bhAccumHDR", _vp, [_vp]),
+    ("bhAccumReset", C.c_int, [_vp, _vp]),
+    ("bhAccumStep", C.c_int, [_vp, _P, _vp]),
+    ("bhAccumCapture", C.c_int, [_vp, _P, _vp]),
+    ("bhAccumReplay", C.c_int, [_vp, C.c_int, _vp]),
+    ("bhAccumSampleIndex", C.c_int, [_vp, C.POINTER(C.c_int), _vp]),
     ("bhGetPrecision", None, [C.POINTER(BhPrecision)]),
     ("bhSetPrecision", C.c_int, [C.POINTER(BhPrecision)]),
     ("bhLastBandStats", C.c_int, [C.POINTER(C.c_uint), C.POINTER(C.c_float), _vp]),
@@ -355,6 +366,61 @@ def set_schedule(wg: int | None = None, chunk: int | None = None, math: int | No
 
 def restore_schedule(s: BhSchedule) -> None:
     check(load().bhSetSchedule(C.byref(s)), "bhSetSchedule")
+
+
+def autotune(P: RenderParams, reps: int = 2):
+    """Run the library's autotuner on P; returns (chosen BhSchedule, best trace ms)."""
+    s, ms = BhSchedule(), C.c_float()
+    check(load().bhAutotune(C.byref(P), reps, C.byref(s), C.byref(ms), None), "bhAutotune")
+    return s, ms.value
+
+
+class Accumulator:
+    """Progressive accumulation of one view (device sample index, graph replay)."""
+
+    def __init__(self, w: int, h: int):
+        self.w, self.h = w, h
+        p = C.c_void_p()
+        check(load().bhAccumCreate(w, h, C.byref(p)), "bhAccumCreate")
+        self.ptr = p
+
+    def reset(self):
+        check(load().bhAccumReset(self.ptr, None), "bhAccumReset")
+
+    def step(self, P: RenderParams):
+        check(load().bhAccumStep(self.ptr, C.byref(P), None), "bhAccumStep")
+
+    def capture(self, P: RenderParams):
+        check(load().bhAccumCapture(self.ptr, C.byref(P), None), "bhAccumCapture")
+
+    def replay(self, frames: int):
+        check(load().bhAccumReplay(self.ptr, frames, None), "bhAccumReplay")
+
+    def sample_index(self) -> int:
+        i = C.c_int()
+        check(load().bhAccumSampleIndex(self.ptr, C.byref(i), None), "bhAccumSampleIndex")
+        return i.value
+
+    def readback(self):
+        lib = load()
+        n = self.w * self.h
+        rgba = np.empty((self.h, self.w, 4), np.uint8)
+        hdr = np.empty((self.h, self.w, 4), np.float32)
+        synchronize()
+        check(lib.bhCopyToHost(rgba.ctypes.data_as(C.c_void_p), lib.bhAccumColor(self.ptr), n * 4), "color")
+        check(lib.bhCopyToHost(hdr.ctypes.data_as(C.c_void_p), lib.bhAccumHDR(self.ptr), n * 16), "hdr")
+        return rgba, hdr
+
+    def close(self):
+        if self.ptr:
+            load().bhAccumDestroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def get_precision() -> BhPrecision:

@@ -40,6 +40,60 @@ BHK_INL d3 dnormalize(d3 a) { return a * (1.0 / sqrt(fmax(ddot(a, a), (double)1e
 BHK_INL v3 tof3(d3 a) { return mk3((float)a.x, (float)a.y, (float)a.z); }
 #define F2D(x) ((double)(x##f))
 
+// FP64 arithmetic policies.  Exact: IEEE division and OCML sin/cos, the
+// operations of the FP64 oracle.  Fast: reciprocals by the hardware estimate
+// plus two Newton steps, and sin/cos together by Cody-Waite reduction modulo
+// pi/2 (two-term constant with fused multiply-adds) and Taylor polynomials
+// to degree 15 / 16 on |r| <= pi/4 (truncation below 5e-17).  Both are
+// accurate to a few ulp in double; the fast one has no branches and about a
+// third of the instructions.
+struct D64Exact
+{
+    static constexpr bool fast = false;
+    BHK_INL static double rcp(double x) { return 1.0 / x; }
+    BHK_INL static double div(double a, double b) { return a / b; }
+    BHK_INL static void sincos_(double x, double& s, double& c) { s = sin(x); c = cos(x); }
+};
+struct D64Fast
+{
+    static constexpr bool fast = true;
+    BHK_INL static double rcp(double x)
+    {
+        double r = __builtin_amdgcn_rcp(x);
+        r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+        return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    }
+    BHK_INL static double div(double a, double b) { return a * rcp(b); }
+    BHK_INL static void sincos_(double x, double& s, double& c)
+    {
+        const double k = __builtin_rint(x * 0.63661977236758134308);
+        double r = __builtin_fma(-k, 1.5707963267948966, x);
+        r = __builtin_fma(-k, 6.123233995736766e-17, r);
+        const double r2 = r * r;
+        double ps = -7.647163731819816e-13;           // -1/15!
+        ps = __builtin_fma(ps, r2, 1.6059043836821613e-10);   // 1/13!
+        ps = __builtin_fma(ps, r2, -2.505210838544172e-08);   // -1/11!
+        ps = __builtin_fma(ps, r2, 2.7557319223985893e-06);   // 1/9!
+        ps = __builtin_fma(ps, r2, -1.984126984126984e-04);   // -1/7!
+        ps = __builtin_fma(ps, r2, 8.333333333333333e-03);    // 1/5!
+        ps = __builtin_fma(ps, r2, -1.6666666666666666e-01);  // -1/3!
+        const double sr = __builtin_fma(r * r2, ps, r);
+        double pc = 4.779477332387385e-14;             // 1/16!
+        pc = __builtin_fma(pc, r2, -1.1470745597729725e-11);  // -1/14!
+        pc = __builtin_fma(pc, r2, 2.08767569878681e-09);     // 1/12!
+        pc = __builtin_fma(pc, r2, -2.755731922398589e-07);   // -1/10!
+        pc = __builtin_fma(pc, r2, 2.48015873015873e-05);     // 1/8!
+        pc = __builtin_fma(pc, r2, -1.388888888888889e-03);   // -1/6!
+        pc = __builtin_fma(pc, r2, 4.1666666666666664e-02);   // 1/4!
+        pc = __builtin_fma(pc, r2, -0.5);
+        const double cr = __builtin_fma(r2, pc, 1.0);
+        const int q = (int)(long long)k & 3;
+        const double s0 = (q & 1) ? cr : sr, c0 = (q & 1) ? sr : cr;
+        s = (q & 2) ? -s0 : s0;
+        c = ((q + 1) & 2) ? -c0 : c0;
+    }
+};
+
 // Per-frame FP64 constants: the RenderParams floats widened to double and
 // the derived values evaluated in double (host-built, passed by value).
 struct BandK
@@ -184,13 +238,17 @@ BHK_INL bool inCriticalBand(const BandK& B, d3 o, d3 dir)
 }
 
 // ------------------------------------------------------------------ spherical, FP64 (trace.cuh:466-611)
-template <class Pol> struct SphRay64
+// Pol: float policy of the disk shading; D: FP64 arithmetic policy.
+template <class Pol, class D> struct SphRay64
 {
     double u, du, phi, tFlight, bImpact;
     d3 e1, e2, prevPos, dir3, escDir;
     v3 acc;
     float trans, E, Lz;
     int step, status;
+
+    BHK_INL float diskE() const { return E; }
+    BHK_INL float diskLz() const { return Lz; }
 
     BHK_INL void init(const BandK& B, d3 origin, d3 dir)
     {
@@ -229,16 +287,13 @@ template <class Pol> struct SphRay64
         dir3 = dir;
     }
 
-    BHK_INL float diskE() const { return E; }
-    BHK_INL float diskLz() const { return Lz; }
-
     BHK_INL double accel(const BandK& B, double uu) const
     {
 #pragma clang fp contract(off)
         return 3.0 * B.M * uu * uu - 2.0 * (B.Q * B.Q) * uu * uu * uu - uu;
     }
 
-    BHK_INL bool advance(const BandK& B, const DiskParams& D, Chord& ch)
+    BHK_INL bool advance(const BandK& B, const DiskParams& Dk, Chord& ch)
     {
 #pragma clang fp contract(off)
         ch.n = 0;
@@ -248,8 +303,8 @@ template <class Pol> struct SphRay64
             else { status = RAY_ESCAPED; escDir = dir3; }
             return true;
         }
-        const double hMax = B.dPhi * fmin(fmax(B.invNineM / u, 1.0), 6.0);
-        double h = fmin(hMax, F2D(0.10) / fmax(fabs(du), F2D(1e-6)));
+        const double hMax = B.dPhi * fmin(fmax(D::div(B.invNineM, u), 1.0), 6.0);
+        double h = fmin(hMax, D::div(F2D(0.10), fmax(fabs(du), F2D(1e-6))));
         if (h < B.hFloor) h = B.hFloor;
         const double k1u = du, k1v = accel(B, u);
         const double u2 = u + F2D(0.5) * h * k1u, v2 = du + F2D(0.5) * h * k1v;
@@ -258,23 +313,25 @@ template <class Pol> struct SphRay64
         const double k3u = w3, k3v = accel(B, u3);
         const double u4 = u + h * k3u, v4 = du + h * k3v;
         const double k4u = v4, k4v = accel(B, u4);
-        u = u + (h / 6.0) * (k1u + 2.0 * k2u + 2.0 * k3u + k4u);
-        du = du + (h / 6.0) * (k1v + 2.0 * k2v + 2.0 * k3v + k4v);
+        const double h6 = D::fast ? h * (1.0 / 6.0) : h / 6.0;
+        u = u + h6 * (k1u + 2.0 * k2u + 2.0 * k3u + k4u);
+        du = du + h6 * (k1v + 2.0 * k2v + 2.0 * k3v + k4v);
         phi = phi + h;
         if (!isfinite(u) || !isfinite(du)) { status = RAY_CAPTURED; return true; }
         if (u >= B.uHor) { status = RAY_CAPTURED; return true; }
         if (u < F2D(1e-4)) { status = RAY_ESCAPED; escDir = dir3; return true; }
-        const double r = 1.0 / u;
-        const double cp = cos(phi), sp = sin(phi);
+        const double r = D::rcp(u);
+        double sp, cp;
+        D::sincos_(phi, sp, cp);
         const d3 m = e1 * cp + e2 * sp;
         const d3 mp = e1 * (-sp) + e2 * cp;
-        const double drdphi = -du / (u * u);
+        const double drdphi = D::div(-du, u * u);
         const d3 pos = m * r;
         dir3 = dnormalize(m * drdphi + mp * r);
-        const double fR = fmax(1.0 - 2.0 * B.M / r + B.Q * B.Q / (r * r), F2D(1e-4));
-        tFlight = tFlight + h * (r * r) / (bImpact * fR);
+        const double fR = fmax(1.0 - D::div(2.0 * B.M, r) + D::div(B.Q * B.Q, r * r), F2D(1e-4));
+        tFlight = tFlight + D::div(h * (r * r), bImpact * fR);
         if (B.disk && trans > 0.01f)
-            chordPrepare<Pol>(D, tof3(prevPos), tof3(pos), (float)(B.diskTime - tFlight), ch);
+            chordPrepare<Pol>(Dk, tof3(prevPos), tof3(pos), (float)(B.diskTime - tFlight), ch);
         prevPos = pos;
         if (u < B.uEscape && du < 0.0) { status = RAY_ESCAPED; escDir = dir3; return true; }
         ++step;
@@ -285,27 +342,44 @@ template <class Pol> struct SphRay64
 // ------------------------------------------------------------------ Kerr-Newman, FP64 (trace.cuh:616-890)
 struct KD64 { double r, th, ph, pr, pth; };
 
-BHK_INL KD64 kerrRHS64(const BandK& B, double E, double Lz, double r, double th, double pr, double pth)
+// Hamiltonian right-hand side at (r, th) with s, cth = sin, cos of th.
+template <class D>
+BHK_INL KD64 kerrRHS64(const BandK& B, double E, double Lz, double r, double pr, double pth, double s, double cth)
 {
 #pragma clang fp contract(off)
-    double s = sin(th);
-    const double cth = cos(th);
     if (s < F2D(1e-4)) s = F2D(1e-4);
     const double s2 = s * s, r2 = r * r, a = B.a, a2 = a * a;
     const double Sig = r2 + a2 * cth * cth;
     double Del = r2 - 2.0 * B.M * r + a2 + B.Q * B.Q;
     if (Del < F2D(1e-6)) Del = F2D(1e-6);
     const double Pp = E * (r2 + a2) - a * Lz;
+    const double dDel = 2.0 * r - 2.0 * B.M;
+    const double dP = 2.0 * r * E;
+    KD64 d;
+    if constexpr (D::fast)
+    {
+        const double is = D::rcp(s), is2 = is * is;
+        const double iDel = D::rcp(Del);
+        const double invS = D::rcp(Sig), invS2 = invS * invS;
+        const double Bq = Lz * is - a * E * s;
+        const double W = Bq * Bq - Pp * Pp * iDel;
+        const double K = Del * pr * pr + pth * pth + W;
+        const double Wr = -(2.0 * Pp * dP * Del - Pp * Pp * dDel) * (iDel * iDel);
+        const double Wth = 2.0 * Bq * (-Lz * cth * is2 - a * E * cth);
+        d.r = Del * pr * invS;
+        d.th = pth * invS;
+        d.ph = ((Lz * is2 - a * E) + a * Pp * iDel) * invS;
+        d.pr = F2D(-0.5) * invS * (dDel * pr * pr + Wr) + r * invS2 * K;
+        d.pth = F2D(-0.5) * invS * Wth - a2 * s * cth * invS2 * K;
+        return d;
+    }
     const double Bq = Lz / s - a * E * s;
     const double W = Bq * Bq - Pp * Pp / Del;
     const double K = Del * pr * pr + pth * pth + W;
-    const double dDel = 2.0 * r - 2.0 * B.M;
-    const double dP = 2.0 * r * E;
     const double Wr = -(2.0 * Pp * dP * Del - Pp * Pp * dDel) / (Del * Del);
     const double dB = -Lz * cth / s2 - a * E * cth;
     const double Wth = 2.0 * Bq * dB;
     const double invS = 1.0 / Sig, invS2 = invS * invS;
-    KD64 d;
     d.r = Del * pr * invS;
     d.th = pth * invS;
     d.ph = ((Lz / s2 - a * E) + a * Pp / Del) * invS;
@@ -314,14 +388,17 @@ BHK_INL KD64 kerrRHS64(const BandK& B, double E, double Lz, double r, double th,
     return d;
 }
 
-template <class Pol> struct KerrRay64
+template <class Pol, class D> struct KerrRay64
 {
-    double r, th, ph, pr, pth, E, Lz, tFlight;
+    double r, th, ph, pr, pth, sth, cth, E, Lz, tFlight;
     KD64 dLast;
     d3 prevPos, dir3, escDir;
     v3 acc;
     float trans, Ef, Lzf;
     int step, status, haveDeriv;
+
+    BHK_INL float diskE() const { return Ef; }
+    BHK_INL float diskLz() const { return Lzf; }
 
     BHK_INL void init(const BandK& B, d3 origin, d3 dir)
     {
@@ -334,24 +411,23 @@ template <class Pol> struct KerrRay64
         const RayInv64 I = kerrInvariants64(B, origin, dir);
         E = I.E; Lz = I.Lz;
         r = I.r0; th = I.th0; ph = I.ph0; pr = I.pr; pth = I.pth;
+        D::sincos_(th, sth, cth);
         Ef = (float)E; Lzf = (float)Lz;
         tFlight = 0.0;
         prevPos = origin;
         dir3 = dir;
     }
 
-    BHK_INL float diskE() const { return Ef; }
-    BHK_INL float diskLz() const { return Lzf; }
-
     BHK_INL d3 bodyDir(const KD64& d) const
     {
 #pragma clang fp contract(off)
-        const double s = sin(th), c = cos(th), sp = sin(ph), cp = cos(ph);
-        const d3 e1b = dk3(s * cp, c, -s * sp), e2b = dk3(c * cp, -s, -c * sp), e3b = dk3(-sp, 0.0, -cp);
-        return dnormalize(e1b * d.r + e2b * (r * d.th) + e3b * (r * s * d.ph));
+        double sp, cp;
+        D::sincos_(ph, sp, cp);
+        const d3 e1b = dk3(sth * cp, cth, -sth * sp), e2b = dk3(cth * cp, -sth, -cth * sp), e3b = dk3(-sp, 0.0, -cp);
+        return dnormalize(e1b * d.r + e2b * (r * d.th) + e3b * (r * sth * d.ph));
     }
 
-    BHK_INL bool advance(const BandK& B, const DiskParams& D, Chord& ch)
+    BHK_INL bool advance(const BandK& B, const DiskParams& Dk, Chord& ch)
     {
 #pragma clang fp contract(off)
         ch.n = 0;
@@ -361,24 +437,31 @@ template <class Pol> struct KerrRay64
             else { status = RAY_ESCAPED; escDir = haveDeriv ? bodyDir(dLast) : dir3; }
             return true;
         }
-        const KD64 d1 = kerrRHS64(B, E, Lz, r, th, pr, pth);
+        const KD64 d1 = kerrRHS64<D>(B, E, Lz, r, pr, pth, sth, cth);
         if (!isfinite(r) || !isfinite(pr) || !isfinite(pth) || !isfinite(d1.r) || !isfinite(d1.pr))
         { status = RAY_CAPTURED; return true; }
         if (r <= B.capR) { status = RAY_CAPTURED; return true; }
         if (r >= B.escR && d1.r > 0.0) { escDir = bodyDir(d1); status = RAY_ESCAPED; return true; }
-        double h = 1.0 / (fabs(d1.r) / (B.kRad * fmax(r - B.rFloor, F2D(0.02))) +
-                          (fabs(d1.th) + fabs(d1.ph)) / B.kAng + F2D(1e-5));
+        double h = D::rcp(D::div(fabs(d1.r), B.kRad * fmax(r - B.rFloor, F2D(0.02))) +
+                          D::div(fabs(d1.th) + fabs(d1.ph), B.kAng) + F2D(1e-5));
         if (r < B.rPhotonRefine) h = h * F2D(0.55);
         const double spd = fabs(d1.r) + r * (fabs(d1.th) + fabs(d1.ph)) + F2D(1e-6);
         double capLen;
-        if (B.disk && r < B.diskOuterCap) capLen = (r * fabs(cos(th)) < F2D(1.8)) ? F2D(1.2) : F2D(2.2);
+        if (B.disk && r < B.diskOuterCap) capLen = (r * fabs(cth) < F2D(1.8)) ? F2D(1.2) : F2D(2.2);
         else capLen = fmax(F2D(1.2), F2D(0.22) * r);
-        h = fmin(h, capLen / spd);
+        h = fmin(h, D::div(capLen, spd));
         const double hh = F2D(0.5) * h;
-        const KD64 d2 = kerrRHS64(B, E, Lz, r + hh * d1.r, th + hh * d1.th, pr + hh * d1.pr, pth + hh * d1.pth);
-        const KD64 d3_ = kerrRHS64(B, E, Lz, r + hh * d2.r, th + hh * d2.th, pr + hh * d2.pr, pth + hh * d2.pth);
-        const KD64 d4 = kerrRHS64(B, E, Lz, r + h * d3_.r, th + h * d3_.th, pr + h * d3_.pr, pth + h * d3_.pth);
-        const double h6 = h / 6.0;
+        double ss, cs;
+        double yth = th + hh * d1.th;
+        D::sincos_(yth, ss, cs);
+        const KD64 d2 = kerrRHS64<D>(B, E, Lz, r + hh * d1.r, pr + hh * d1.pr, pth + hh * d1.pth, ss, cs);
+        yth = th + hh * d2.th;
+        D::sincos_(yth, ss, cs);
+        const KD64 d3_ = kerrRHS64<D>(B, E, Lz, r + hh * d2.r, pr + hh * d2.pr, pth + hh * d2.pth, ss, cs);
+        yth = th + h * d3_.th;
+        D::sincos_(yth, ss, cs);
+        const KD64 d4 = kerrRHS64<D>(B, E, Lz, r + h * d3_.r, pr + h * d3_.pr, pth + h * d3_.pth, ss, cs);
+        const double h6 = D::fast ? h * (1.0 / 6.0) : h / 6.0;
         r = r + h6 * (d1.r + 2.0 * d2.r + 2.0 * d3_.r + d4.r);
         th = th + h6 * (d1.th + 2.0 * d2.th + 2.0 * d3_.th + d4.th);
         ph = ph + h6 * (d1.ph + 2.0 * d2.ph + 2.0 * d3_.ph + d4.ph);
@@ -389,25 +472,25 @@ template <class Pol> struct KerrRay64
         else if (th > pi - F2D(1e-3)) { th = pi - F2D(1e-3); pth = -fabs(pth); }
         dLast = d4;
         haveDeriv = 1;
+        D::sincos_(th, sth, cth);
         {
-            double s = sin(th);
-            const double cth = cos(th);
-            if (s < F2D(1e-4)) s = F2D(1e-4);
+            const double s = sth < F2D(1e-4) ? F2D(1e-4) : sth;
             const double a = B.a;
             const double Sig = r * r + a * a * cth * cth;
             const double Del = fmax(r * r - 2.0 * B.M * r + a * a + B.Q * B.Q, F2D(1e-5));
             const double Pp = E * (r * r + a * a) - a * Lz;
-            const double dtdl = (a * (Lz - a * E * s * s) + (r * r + a * a) * Pp / Del) / fmax(Sig, F2D(1e-8));
+            const double dtdl = D::div(a * (Lz - a * E * s * s) + D::div((r * r + a * a) * Pp, Del), fmax(Sig, F2D(1e-8)));
             tFlight = tFlight + h * fabs(dtdl);
         }
         if (B.disk)
         {
-            const double s = sin(th), c = cos(th), sp = sin(ph), cp = cos(ph);
-            const d3 pos = dk3(r * s * cp, r * c, -r * s * sp);
+            double sp, cp;
+            D::sincos_(ph, sp, cp);
+            const d3 pos = dk3(r * sth * cp, r * cth, -r * sth * sp);
             const d3 stepv = pos - prevPos;
             const double sl = dlen(stepv);
             if (sl > F2D(1e-6)) dir3 = stepv * (1.0 / sl);
-            if (trans > 0.01f) chordPrepare<Pol>(D, tof3(prevPos), tof3(pos), (float)(B.diskTime - tFlight), ch);
+            if (trans > 0.01f) chordPrepare<Pol>(Dk, tof3(prevPos), tof3(pos), (float)(B.diskTime - tFlight), ch);
             prevPos = pos;
         }
         ++step;

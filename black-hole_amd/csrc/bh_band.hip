@@ -20,7 +20,7 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
     {
         const int px = i % W, py = i / W;
         const d3 o = dk3(A.P.camPos.x, A.P.camPos.y, A.P.camPos.z);
-        band = inCriticalBand(A.B, o, cameraDir64(A.P, px, py, A.jitter));
+        band = inCriticalBand(A.B, o, cameraDir64(A.P, px, py, launchJitter(A.P, A.jitter, A.dynIndex)));
         A.mask[i] = band ? 1 : 0;
     }
     const unsigned long long m = __ballot(band);

@@ -562,6 +562,16 @@ BHK_INL v2 subpixelJitter(int accumMode, int sampleIndex)
     return mk2(fract(0.7548776662f * n + 0.5f), fract(0.5698402909f * n + 0.5f));
 }
 
+// Sample position of a launch: the host-computed jitter, or, when the
+// sample index lives in device memory (graph-replayed accumulation), the R2
+// point of that index.
+BHK_INL float2 launchJitter(const RenderParams& P, float2 hostJit, const int* dynIndex)
+{
+    if (!dynIndex) return hostJit;
+    const v2 j = subpixelJitter(P.accumMode, *dynIndex);
+    return make_float2(j.x, j.y);
+}
+
 // ------------------------------------------------------------------ sky (trace.cuh:131-195), exact arithmetic
 BHK_INL v3 starLayer(v3 d, float cellScale, float intensity, float coreSharp)
 {

@@ -23,6 +23,7 @@ struct TraceArgs
     int coop;               // 1: wavefront-cooperative disk sampling
     float2 jitter;          // sub-pixel sample position
     const unsigned char* bandMask;  // [W*H] 1: traced by the FP64 band kernel (or null)
+    const int* dynIndex;    // device sample index overriding P.sampleIndex (or null)
 };
 
 // FP64 critical-band pass: classification of every pixel, then the listed
@@ -37,6 +38,7 @@ struct BandArgs
     unsigned* list;         // [W*H] band pixel indices
     unsigned* count;        // list length (zero before classification)
     float2 jitter;
+    const int* dynIndex;    // device sample index overriding P.sampleIndex (or null)
 };
 
 struct ProbeArgs
@@ -98,7 +100,12 @@ int traceBatchOccupancyFast(int wgSize);
 // post chain (bh_post.hip)
 hipError_t launchResetCounter(unsigned* counter, hipStream_t st);
 // records -> HDR sample -> blend into accum (pipeline) ...
-hipError_t launchShadeBlend(const RenderParams& P, const BhRayRecord* rec, float4* accum, hipStream_t st);
+hipError_t launchShadeBlend(const RenderParams& P, const BhRayRecord* rec, float4* accum, const int* dynIndex,
+                            hipStream_t st);
+// dynIndex[0] += 1 (end of a graph-replayed accumulation step)
+hipError_t launchBumpIndex(int* dynIndex, hipStream_t st);
+// dynIndex[0] = value
+hipError_t launchSetIndex(int* dynIndex, int value, hipStream_t st);
 // ... or records -> HDR sample -> tonemapped pixel (single pass, no bloom)
 hipError_t launchShadeFinalize(const RenderParams& P, const BhRayRecord* rec, uchar4* out, hipStream_t st);
 // accum -> half-res bright pass + horizontal blur (into dstH)

@@ -36,29 +36,34 @@ BHK_INL v3 sampleFromRecord(const BhRayRecord* rec, int idx, float gCam)
     return hdr;
 }
 
-BHK_INL void shadeBlendAt(const RenderParams& P, const BhRayRecord* rec, float4* accum, int idx, float gCam)
+BHK_INL void shadeBlendAt(const RenderParams& P, const BhRayRecord* rec, float4* accum, int idx, float gCam,
+                          int sampleIndex)
 {
     v3 cur = sampleFromRecord(rec, idx, gCam);
     float4 pv = accum[idx];
     v3 prev = mk3(pv.x, pv.y, pv.z);
     if (!finite3(prev)) prev = cur;
-    v3 c = accumBlend(prev, cur, P.accumMode, P.sampleIndex);
+    v3 c = accumBlend(prev, cur, P.accumMode, sampleIndex);
     accum[idx] = make_float4(c.x, c.y, c.z, 1.f);
 }
 
-__global__ __launch_bounds__(256) void shadeBlendKernel(RenderParams P, const BhRayRecord* rec, float4* accum)
+__global__ __launch_bounds__(256) void shadeBlendKernel(RenderParams P, const BhRayRecord* rec, float4* accum,
+                                                        const int* dynIndex)
 {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= P.width * P.height) return;
-    shadeBlendAt(P, rec, accum, idx, cameraEnergyFactor(P));
+    shadeBlendAt(P, rec, accum, idx, cameraEnergyFactor(P), dynIndex ? *dynIndex : P.sampleIndex);
 }
+
+__global__ void bumpIndexKernel(int* i) { i[0] += 1; }
+__global__ void setIndexKernel(int* i, int v) { i[0] = v; }
 
 __global__ __launch_bounds__(256) void shadeBlendBatchKernel(const ViewDesc* views)
 {
     const ViewDesc& V = views[blockIdx.z];
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= V.P.width * V.P.height) return;
-    shadeBlendAt(V.P, V.rec, V.accum, idx, cameraEnergyFactor(V.P));
+    shadeBlendAt(V.P, V.rec, V.accum, idx, cameraEnergyFactor(V.P), V.P.sampleIndex);
 }
 
 __global__ __launch_bounds__(256) void shadeFinalizeKernel(RenderParams P, const BhRayRecord* rec, uchar4* out)
@@ -252,10 +257,23 @@ hipError_t launchResetCounter(unsigned* counter, hipStream_t st)
 
 static inline unsigned blocksFor(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-hipError_t launchShadeBlend(const RenderParams& P, const BhRayRecord* rec, float4* accum, hipStream_t st)
+hipError_t launchShadeBlend(const RenderParams& P, const BhRayRecord* rec, float4* accum, const int* dynIndex,
+                            hipStream_t st)
 {
     long long n = (long long)P.width * P.height;
-    hipLaunchKernelGGL(shadeBlendKernel, dim3(blocksFor(n, 256)), dim3(256), 0, st, P, rec, accum);
+    hipLaunchKernelGGL(shadeBlendKernel, dim3(blocksFor(n, 256)), dim3(256), 0, st, P, rec, accum, dynIndex);
+    return hipGetLastError();
+}
+
+hipError_t launchBumpIndex(int* dynIndex, hipStream_t st)
+{
+    hipLaunchKernelGGL(bumpIndexKernel, dim3(1), dim3(1), 0, st, dynIndex);
+    return hipGetLastError();
+}
+
+hipError_t launchSetIndex(int* dynIndex, int value, hipStream_t st)
+{
+    hipLaunchKernelGGL(setIndexKernel, dim3(1), dim3(1), 0, st, dynIndex, value);
     return hipGetLastError();
 }
 

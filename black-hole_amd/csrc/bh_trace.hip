@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void traceKernel(TraceArgs A)
     const unsigned total = (unsigned)A.totalIds;
     const int chunk = A.chunk;
     const int refillMin = A.refillMin;
-    const float2 jit = A.jitter;
+    const float2 jit = launchJitter(P, A.jitter, A.dynIndex);
 
     Ray R;
     Chord ch;
@@ -174,11 +174,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Pol::fast ?
 {
     const unsigned lane = __lane_id();
     const unsigned nTiles = (unsigned)A.totalIds / 64u;
+    const float2 jit = launchJitter(A.P, A.jitter, A.dynIndex);
     for (;;)
     {
         const unsigned t = nextTile(A.counter, lane);
         if (t >= nTiles) break;
-        traceTile<Ray, kCoop>(A.P, A.K, A.rec, A.tilesX, A.jitter, t, lane, A.bandMask);
+        traceTile<Ray, kCoop>(A.P, A.K, A.rec, A.tilesX, jit, t, lane, A.bandMask);
     }
 }
 
@@ -226,12 +227,13 @@ __global__ __launch_bounds__(64) void traceBandKernel(BandArgs A)
 {
     const unsigned n = *A.count;
     const d3 o = dk3(A.P.camPos.x, A.P.camPos.y, A.P.camPos.z);
+    const float2 jit = launchJitter(A.P, A.jitter, A.dynIndex);
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
         const unsigned pix = A.list[i];
         const int px = (int)(pix % (unsigned)A.P.width), py = (int)(pix / (unsigned)A.P.width);
         Ray R;
-        R.init(A.B, o, cameraDir64(A.P, px, py, A.jitter));
+        R.init(A.B, o, cameraDir64(A.P, px, py, jit));
         Chord ch;
         bool done = (R.status != RAY_RUNNING);
         while (!done)
@@ -340,8 +342,11 @@ int BHK_SUFFIX(traceBatchOccupancy)(int wgSize)
 
 hipError_t BHK_SUFFIX(launchBandTrace)(const BandArgs& A, int blocks, hipStream_t st)
 {
-    if (A.B.kerr) hipLaunchKernelGGL(traceBandKernel<KerrRay64<Pol>>, dim3(blocks), dim3(64), 0, st, A);
-    else hipLaunchKernelGGL(traceBandKernel<SphRay64<Pol>>, dim3(blocks), dim3(64), 0, st, A);
+    using D = typename std::conditional<Pol::fast, D64Fast, D64Exact>::type;
+    using KR = KerrRay64<Pol, D>;
+    using SR = SphRay64<Pol, D>;
+    if (A.B.kerr) hipLaunchKernelGGL(traceBandKernel<KR>, dim3(blocks), dim3(64), 0, st, A);
+    else hipLaunchKernelGGL(traceBandKernel<SR>, dim3(blocks), dim3(64), 0, st, A);
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@
 #include <atomic>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -48,12 +49,17 @@ struct Scratch
     unsigned* bandList = nullptr;
     size_t bandCap = 0;
     hipEvent_t bandEv[2] = {nullptr, nullptr};   // timing: band pass start / end
+    // the FP32 tile trace runs on `aux` concurrently with the FP64 band
+    // kernel on the caller's stream: fork after classification, join before
+    // shading
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 std::mutex g_mu;
 std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 BhSchedule g_sched = {256, 4096, 64, 1, 0};   // wgSize, stepsPerChunk, raysPerRefill, mathMode, diskCoop
-BhPrecision g_prec = {0, 3e-2f, 1e-3f};
+BhPrecision g_prec = {1, 3e-2f, 1e-3f};
 constexpr int kBandCountSlot = 16;            // counter[16]: band list length (own 64-byte line)
 std::atomic<int> g_timers{0};
 std::pair<int, hipStream_t> g_lastTimed{-1, nullptr};
@@ -160,6 +166,13 @@ hipError_t getBandScratch(Scratch* S, size_t n, hipStream_t st)
     if (e == hipSuccess) e = hipMalloc(&S->bandList, n * sizeof(unsigned));
     if (e != hipSuccess) return e;
     S->bandCap = n;
+    if (!S->aux)
+    {
+        e = hipStreamCreateWithFlags(&S->aux, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&S->fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&S->join, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
     return hipSuccess;
 }
 
@@ -224,7 +237,7 @@ float2 jitterOf(int accumMode, int sampleIndex)
 }
 
 hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, const BhSchedule& sc,
-                        hipStream_t st)
+                        hipStream_t st, const int* dynIndex = nullptr, bool allowTimers = true)
 {
     unsigned* counter = Sc->counter;
     const BhPrecision pr = currentPrecision();
@@ -243,6 +256,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.list = Sc->bandList;
         BA.count = counter + kBandCountSlot;
         BA.jitter = jitterOf(P.accumMode, P.sampleIndex);
+        BA.dynIndex = dynIndex;
         e = bhk::launchResetCounter(BA.count, st);
         if (e == hipSuccess) e = bhk::launchBandClassify(BA, st);
         if (e != hipSuccess) return e;
@@ -259,6 +273,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     A.refillMin = sc.raysPerRefill;
     A.coop = sc.diskCoop;
     A.jitter = jitterOf(P.accumMode, P.sampleIndex);
+    A.dynIndex = dynIndex;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     const int kerr = isKerr(P.model) ? 1 : 0;
@@ -273,19 +288,30 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     if (S.blocks < 1) S.blocks = 1;
     hipError_t e = bhk::launchResetCounter(counter, st);
     if (e != hipSuccess) return e;
-    e = sc.mathMode ? bhk::launchTraceFast(A, S, st) : bhk::launchTraceExact(A, S, st);
-    if (e != hipSuccess || !band) return e;
-    const bool timed = g_timers.load() != 0;
+    if (!band) return sc.mathMode ? bhk::launchTraceFast(A, S, st) : bhk::launchTraceExact(A, S, st);
+    // The band kernel goes first on the caller's stream so its few, long
+    // FP64 wavefronts are resident before the persistent FP32 tile grid
+    // fills the machine; the tile kernel follows on the auxiliary stream.
+    const bool timed = allowTimers && g_timers.load() != 0;
     if (timed && !Sc->bandEv[0])
     {
         if (hipEventCreate(&Sc->bandEv[0]) != hipSuccess || hipEventCreate(&Sc->bandEv[1]) != hipSuccess)
             return hipErrorOutOfMemory;
     }
+    e = hipEventRecord(Sc->fork, st);
+    if (e != hipSuccess) return e;
     if (timed) (void)hipEventRecord(Sc->bandEv[0], st);
     const int blocks = 4 * numCUs(dev);
     e = sc.mathMode ? bhk::launchBandTraceFast(BA, blocks, st) : bhk::launchBandTraceExact(BA, blocks, st);
+    if (e != hipSuccess) return e;
     if (timed) (void)hipEventRecord(Sc->bandEv[1], st);
-    return e;
+    e = hipStreamWaitEvent(Sc->aux, Sc->fork, 0);
+    if (e != hipSuccess) return e;
+    e = sc.mathMode ? bhk::launchTraceFast(A, S, Sc->aux) : bhk::launchTraceExact(A, S, Sc->aux);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(Sc->join, Sc->aux);
+    if (e != hipSuccess) return e;
+    return hipStreamWaitEvent(st, Sc->join, 0);
 }
 
 void recordEvent(Scratch* S, int i, hipStream_t st)
@@ -300,7 +326,7 @@ void recordEvent(Scratch* S, int i, hipStream_t st)
 }
 
 hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bloomB, const RenderParams& p,
-                          hipStream_t st, bool singlePass)
+                          hipStream_t st, bool singlePass, const int* dynIndex = nullptr, bool allowTimers = true)
 {
     RenderParams P = p;
     bhSanitizeRenderParams(&P);
@@ -312,9 +338,9 @@ hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bl
     hipError_t e = getScratch(st, n, &S);
     if (e != hipSuccess) return e;
     const BhSchedule sc = currentSchedule();
-    const bool timed = g_timers.load() != 0;
+    const bool timed = allowTimers && g_timers.load() != 0;
     if (timed) recordEvent(S, 0, st);
-    e = enqueueTrace(P, S->rec, S, sc, st);
+    e = enqueueTrace(P, S->rec, S, sc, st, dynIndex, allowTimers);
     if (e != hipSuccess) return e;
     if (timed) recordEvent(S, 1, st);
     if (singlePass)
@@ -324,7 +350,7 @@ hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bl
     }
     else
     {
-        e = bhk::launchShadeBlend(P, S->rec, accum, st);
+        e = bhk::launchShadeBlend(P, S->rec, accum, dynIndex, st);
         if (e != hipSuccess) return e;
         if (timed) recordEvent(S, 2, st);
         if (P.bloomEnabled)
@@ -493,6 +519,31 @@ struct BhFramebuffer
     float4* bloomB;
 };
 
+struct BhAccumulator
+{
+    BhFramebuffer* fb = nullptr;
+    int* index = nullptr;              // device sample index of the next frame
+    hipStream_t capStream = nullptr;   // private stream the step graph is captured on
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Autotune cache: schedule chosen per workload class.
+struct TuneKey
+{
+    int kerr, w, h, disk, maxSteps, band, math;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(kerr, w, h, disk, maxSteps, band, math) <
+               std::tie(o.kerr, o.w, o.h, o.disk, o.maxSteps, o.band, o.math);
+    }
+};
+std::map<TuneKey, std::pair<BhSchedule, float>> g_tuned;
+
+}  // namespace
+
 extern "C" {
 
 bh_error_t launchRenderPipeline(bh_uchar4* out, bh_float4* accum, bh_float4* bloomA, bh_float4* bloomB,
@@ -628,6 +679,195 @@ bh_error_t bhFramebufferReadback(BhFramebuffer* fb, bh_uchar4* hostColor, bh_flo
     if (e == hipSuccess && hostColor) e = hipMemcpy(hostColor, fb->color, n * sizeof(uchar4), hipMemcpyDeviceToHost);
     if (e == hipSuccess && hostHdr) e = hipMemcpy(hostHdr, fb->hdr, n * sizeof(float4), hipMemcpyDeviceToHost);
     return e;
+}
+
+// ---------------------------------------------------------------- progressive accumulation
+bh_error_t bhAccumCreate(int width, int height, BhAccumulator** acc)
+{
+    if (!acc) return hipErrorInvalidValue;
+    *acc = nullptr;
+    BhAccumulator* A = new (std::nothrow) BhAccumulator;
+    if (!A) return hipErrorOutOfMemory;
+    hipError_t e = (hipError_t)bhFramebufferCreate(width, height, &A->fb);
+    if (e == hipSuccess) e = hipMalloc(&A->index, sizeof(int) * 64);
+    if (e == hipSuccess) e = hipMemset(A->index, 0, sizeof(int) * 64);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&A->capStream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+    {
+        bhAccumDestroy(A);
+        return e;
+    }
+    *acc = A;
+    return hipSuccess;
+}
+
+void bhAccumDestroy(BhAccumulator* acc)
+{
+    if (!acc) return;
+    if (acc->exec) (void)hipGraphExecDestroy(acc->exec);
+    if (acc->graph) (void)hipGraphDestroy(acc->graph);
+    if (acc->capStream)
+    {
+        (void)hipStreamSynchronize(acc->capStream);
+        (void)hipStreamDestroy(acc->capStream);
+    }
+    (void)hipFree(acc->index);
+    bhFramebufferDestroy(acc->fb);
+    delete acc;
+}
+
+bh_uchar4* bhAccumColor(BhAccumulator* acc) { return acc ? bhFramebufferColor(acc->fb) : nullptr; }
+bh_float4* bhAccumHDR(BhAccumulator* acc) { return acc ? bhFramebufferHDR(acc->fb) : nullptr; }
+
+bh_error_t bhAccumReset(BhAccumulator* acc, bh_stream_t stream)
+{
+    if (!acc) return hipErrorInvalidValue;
+    return bhk::launchSetIndex(acc->index, 0, (hipStream_t)stream);
+}
+
+static hipError_t accumStepOn(BhAccumulator* acc, const RenderParams& p, hipStream_t st, bool allowTimers)
+{
+    BhFramebuffer* F = acc->fb;
+    if (p.width != F->w || p.height != F->h) return hipErrorInvalidValue;
+    hipError_t e = renderPipeline(F->color, F->hdr, F->bloomA, F->bloomB, p, st, false, acc->index, allowTimers);
+    if (e != hipSuccess) return e;
+    return bhk::launchBumpIndex(acc->index, st);
+}
+
+bh_error_t bhAccumStep(BhAccumulator* acc, const RenderParams& p, bh_stream_t stream)
+{
+    if (!acc) return hipErrorInvalidValue;
+    return accumStepOn(acc, p, (hipStream_t)stream, true);
+}
+
+bh_error_t bhAccumCapture(BhAccumulator* acc, const RenderParams& p, bh_stream_t stream)
+{
+    if (!acc) return hipErrorInvalidValue;
+    BhFramebuffer* F = acc->fb;
+    if (p.width != F->w || p.height != F->h) return hipErrorInvalidValue;
+    hipStream_t cs = acc->capStream;
+    // work already queued by the caller comes first
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return e;
+    // scratch of the capture stream is allocated before capture (no
+    // allocation may happen inside it)
+    RenderParams P = p;
+    bhSanitizeRenderParams(&P);
+    const size_t n = (size_t)P.width * (size_t)P.height;
+    Scratch* S = nullptr;
+    e = getScratch(cs, n, &S);
+    if (e == hipSuccess && currentPrecision().fp64Band) e = getBandScratch(S, n, cs);
+    if (e != hipSuccess) return e;
+    if (acc->exec) { (void)hipGraphExecDestroy(acc->exec); acc->exec = nullptr; }
+    if (acc->graph) { (void)hipGraphDestroy(acc->graph); acc->graph = nullptr; }
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = accumStepOn(acc, p, cs, false);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    if (le != hipSuccess) { if (g) (void)hipGraphDestroy(g); return le; }
+    if (e != hipSuccess) return e;
+    acc->graph = g;
+    e = hipGraphInstantiate(&acc->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) { acc->exec = nullptr; return e; }
+    return hipSuccess;
+}
+
+bh_error_t bhAccumReplay(BhAccumulator* acc, int frames, bh_stream_t stream)
+{
+    if (!acc || frames < 0) return hipErrorInvalidValue;
+    if (!acc->exec) return hipErrorNotReady;
+    for (int i = 0; i < frames; ++i)
+    {
+        hipError_t e = hipGraphLaunch(acc->exec, (hipStream_t)stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+bh_error_t bhAccumSampleIndex(BhAccumulator* acc, int* index, bh_stream_t stream)
+{
+    if (!acc || !index) return hipErrorInvalidValue;
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(index, acc->index, sizeof(int), hipMemcpyDeviceToHost);
+}
+
+// ---------------------------------------------------------------- autotuner
+bh_error_t bhAutotune(const RenderParams& p, int reps, BhSchedule* chosen, float* bestMs, bh_stream_t stream)
+{
+    if (reps < 1 || reps > 64) return hipErrorInvalidValue;
+    RenderParams P = p;
+    bhSanitizeRenderParams(&P);
+    hipStream_t st = (hipStream_t)stream;
+    const BhSchedule cur = currentSchedule();
+    const BhPrecision pr = currentPrecision();
+    const TuneKey key{isKerr(P.model) ? 1 : 0, P.width, P.height, P.diskEnabled, P.maxSteps, pr.fp64Band,
+                      cur.mathMode};
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end())
+        {
+            g_sched = it->second.first;
+            if (chosen) *chosen = it->second.first;
+            if (bestMs) *bestMs = it->second.second;
+            return hipSuccess;
+        }
+    }
+    const size_t n = (size_t)P.width * (size_t)P.height;
+    Scratch* S = nullptr;
+    hipError_t e = getScratch(st, n, &S);
+    if (e != hipSuccess) return e;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    if (e != hipSuccess) return e;
+    // candidates: workgroup size x (whole tiles | lane refill at two chunk
+    // lengths) x cooperative disk sampling
+    const int wgs[3] = {256, 128, 64};
+    const int modes[3][2] = {{64, 4096}, {32, 256}, {16, 64}};
+    BhSchedule best = cur;
+    float bestT = 1e30f;
+    for (int w = 0; w < 3 && e == hipSuccess; ++w)
+        for (int m = 0; m < 3 && e == hipSuccess; ++m)
+            for (int coop = 0; coop < 2 && e == hipSuccess; ++coop)
+            {
+                BhSchedule c = cur;
+                c.wgSize = wgs[w];
+                c.raysPerRefill = modes[m][0];
+                c.stepsPerChunk = modes[m][1];
+                c.diskCoop = coop;
+                float tMin = 1e30f;
+                for (int r = 0; r <= reps && e == hipSuccess; ++r)   // first run warms up
+                {
+                    e = hipEventRecord(t0, st);
+                    if (e == hipSuccess) e = enqueueTrace(P, S->rec, S, c, st, nullptr, false);
+                    if (e == hipSuccess) e = hipEventRecord(t1, st);
+                    if (e == hipSuccess) e = hipEventSynchronize(t1);
+                    float ms = 0.f;
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+                    if (e == hipSuccess && r > 0) tMin = std::min(tMin, ms);
+                }
+                if (e == hipSuccess && tMin < bestT) { bestT = tMin; best = c; }
+            }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (e != hipSuccess) return e;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_sched = best;
+        g_tuned[key] = {best, bestT};
+    }
+    if (chosen) *chosen = best;
+    if (bestMs) *bestMs = bestT;
+    return hipSuccess;
+}
+
+void bhAutotuneClear(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_tuned.clear();
 }
 
 // ---------------------------------------------------------------- schedule / timers

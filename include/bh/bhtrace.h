@@ -168,6 +168,30 @@ bh_error_t bhFramebufferReadback(BhFramebuffer* fb, bh_uchar4* hostColor,
                                  bh_float4* hostHdr, bh_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Progressive accumulation, graph-replayed                                 */
+/* ------------------------------------------------------------------------ */
+/* Frames of one steady view accumulated into one HDR buffer, as in the
+ * reference's progressive anti-aliasing loop (main.cpp:439-478): frame k
+ * takes sample index k (R2 sub-pixel jitter, running mean in accumMode 1,
+ * EMA in accumMode 2; p.sampleIndex is ignored).  The index lives in device
+ * memory and the step increments it, so one captured step (a hipGraph of
+ * trace, shade, bloom and composite) replays for every later frame. */
+typedef struct BhAccumulator BhAccumulator;
+bh_error_t bhAccumCreate(int width, int height, BhAccumulator** acc);
+void       bhAccumDestroy(BhAccumulator* acc);
+bh_uchar4* bhAccumColor(BhAccumulator* acc);   /* device uchar4[W*H] */
+bh_float4* bhAccumHDR(BhAccumulator* acc);     /* device float4[W*H] */
+bh_error_t bhAccumReset(BhAccumulator* acc, bh_stream_t stream);   /* next frame is sample 0 */
+/* One frame with eager launches on `stream`. */
+bh_error_t bhAccumStep(BhAccumulator* acc, BH_PARAMS_IN p, bh_stream_t stream);
+/* Capture one step for `p` as a graph (synchronizes `stream` first; nothing
+ * is executed).  Replays run the captured step `frames` times on `stream`. */
+bh_error_t bhAccumCapture(BhAccumulator* acc, BH_PARAMS_IN p, bh_stream_t stream);
+bh_error_t bhAccumReplay(BhAccumulator* acc, int frames, bh_stream_t stream);
+/* Sample index of the next frame (synchronizes `stream`). */
+bh_error_t bhAccumSampleIndex(BhAccumulator* acc, int* index, bh_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Scheduling                                                               */
 /* ------------------------------------------------------------------------ */
 typedef struct BhSchedule
@@ -181,6 +205,14 @@ typedef struct BhSchedule
 } BhSchedule;
 void bhGetSchedule(BhSchedule* s);
 bh_error_t bhSetSchedule(const BhSchedule* s);   /* -1 (hipErrorInvalidValue-like) on bad values */
+/* Times the trace kernel of `p` under 18 candidate schedules (workgroup
+ * size 64/128/256 x whole tiles / lane refill of 32 rays every 256 steps /
+ * 16 rays every 64 steps x cooperative disk sampling off/on; `reps` timed
+ * frames each after one warm-up) on `stream`, makes the fastest the current
+ * schedule and caches it per (model class, size, disk, step budget, band,
+ * math mode).  Output does not depend on the schedule. */
+bh_error_t bhAutotune(BH_PARAMS_IN p, int reps, BhSchedule* chosen, float* bestMs, bh_stream_t stream);
+void bhAutotuneClear(void);
 
 /* ------------------------------------------------------------------------ */
 /* Precision: FP64 critical-band rays                                        */

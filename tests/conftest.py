@@ -21,9 +21,14 @@ def bt():
 
 
 @pytest.fixture(scope="session")
-def oracle():
+def oracle(bt):
+    """The CPU oracle, with its FP64 band rule set like the library's default
+    precision (tests that change one change the other)."""
     import pyoracle
-    return pyoracle.Oracle()
+    O = pyoracle.Oracle()
+    p = bt.get_precision()
+    O.set_band(p.fp64Band, p.critBand, p.polarBand)
+    return O
 
 
 @pytest.fixture(scope="session")
