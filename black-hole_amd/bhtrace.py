@@ -77,6 +77,9 @@ class BhPrecision(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+MLP_TARGET_DTYPE = np.dtype([("d0", "<f4"), ("d1", "<f4"), ("disk", "<f4"), ("hitR", "<f4"), ("escaped", "<f4"),
+                             ("pad", "<f4", (3,))])
+
 PROBE_DTYPE = np.dtype([("outcome", "<i4"), ("escDir", "<f4", (3,)), ("maxHviol", "<f4"),
                         ("lum", "<f4"), ("steps", "<i4"), ("trans", "<f4")])
 RECORD_DTYPE = np.dtype([("escDir", "<f4", (3,)), ("flags", "<i4"), ("accum", "<f4", (3,)),
@@ -128,6 +131,20 @@ SIGNATURES = [
     ("bhLastBandStats", C.c_int, [C.POINTER(C.c_uint), C.POINTER(C.c_float), _vp]),
     ("bhEnableKernelTimers", None, [C.c_int]),
     ("bhLastKernelTimes", C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("bhSurrogateParamCount", C.c_int, []),
+    ("bhSurrogateCreate", C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("bhSurrogateDestroy", None, [_vp]),
+    ("bhSurrogateSetParams", C.c_int, [_vp, _vp, C.c_int]),
+    ("bhSurrogateGetParams", C.c_int, [_vp, _vp, C.c_int]),
+    ("bhSurrogateConfigs", C.c_int, [C.c_uint64, C.c_int, _P]),
+    ("bhSurrogateDataset", C.c_int, [_vp, C.c_int, C.c_int, C.c_uint64, _vp]),
+    ("bhSurrogateDatasetView", C.c_int, [_vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                         C.POINTER(C.c_int)]),
+    ("bhSurrogateTrain", C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _vp]),
+    ("bhSurrogateForward", C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
+    ("bhSurrogateLossGrad", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp]),
+    ("bhMlpGemmTest", C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
+                                C.c_int, C.c_int, _vp]),
     ("bhDeviceAlloc", C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     ("bhDeviceFree", C.c_int, [_vp]),
     ("bhCopyToDevice", C.c_int, [_vp, _vp, C.c_size_t]),
@@ -421,6 +438,92 @@ class Accumulator:
             self.close()
         except Exception:
             pass
+
+
+class Surrogate:
+    """The learned surrogate (network parameters, training set, training)."""
+
+    def __init__(self, seed: int = 2):
+        p = C.c_void_p()
+        check(load().bhSurrogateCreate(seed, C.byref(p)), "bhSurrogateCreate")
+        self.ptr = p
+        self.nparams = load().bhSurrogateParamCount()
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, np.float32)
+        check(load().bhSurrogateGetParams(self.ptr, out.ctypes.data_as(C.c_void_p), self.nparams), "get params")
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        a = np.ascontiguousarray(p, np.float32)
+        check(load().bhSurrogateSetParams(self.ptr, a.ctypes.data_as(C.c_void_p), a.size), "set params")
+
+    def dataset(self, n_cfg: int, rays_per_cfg: int, seed: int = 2) -> None:
+        check(load().bhSurrogateDataset(self.ptr, n_cfg, rays_per_cfg, seed, None), "bhSurrogateDataset")
+
+    def dataset_view(self):
+        """(inputs [n,6], targets [n] MLP_TARGET_DTYPE, escDir [n,3]) copied to host."""
+        x, t, e, n = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_int()
+        check(load().bhSurrogateDatasetView(self.ptr, C.byref(x), C.byref(t), C.byref(e), C.byref(n)), "view")
+        n = n.value
+        lib = load()
+        X = np.empty((n, 6), np.float32)
+        T = np.empty(n, MLP_TARGET_DTYPE)
+        E = np.empty((n, 3), np.float32)
+        synchronize()
+        for dst, src in ((X, x), (T, t), (E, e)):
+            if n:
+                check(lib.bhCopyToHost(dst.ctypes.data_as(C.c_void_p), src, dst.nbytes), "download")
+        return X, T, E
+
+    def dataset_device(self):
+        x, t, e, n = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_int()
+        check(load().bhSurrogateDatasetView(self.ptr, C.byref(x), C.byref(t), C.byref(e), C.byref(n)), "view")
+        return x, t, e, n.value
+
+    def train(self, steps: int, batch: int = 65536, lr: float = 1e-3) -> np.ndarray:
+        loss = np.zeros((steps, 3), np.float32)
+        check(load().bhSurrogateTrain(self.ptr, steps, batch, lr, loss.ctypes.data_as(C.c_void_p), None), "train")
+        return loss
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
+        n = a.shape[0]
+        xi, yo = DeviceBuffer(a.nbytes), DeviceBuffer(n * 5 * 4)
+        xi.upload(a)
+        check(load().bhSurrogateForward(self.ptr, xi.ptr, n, yo.ptr, None), "forward")
+        synchronize()
+        return yo.download(np.float32).reshape(n, 5)
+
+    def loss_grad(self, x: np.ndarray, t: np.ndarray):
+        a = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
+        tt = np.ascontiguousarray(t, MLP_TARGET_DTYPE)
+        n = a.shape[0]
+        xi, ti = DeviceBuffer(a.nbytes), DeviceBuffer(tt.nbytes)
+        xi.upload(a)
+        ti.upload(tt.view(np.uint8))
+        loss = np.zeros(3, np.float32)
+        grad = np.zeros(self.nparams, np.float32)
+        check(load().bhSurrogateLossGrad(self.ptr, xi.ptr, ti.ptr, n, loss.ctypes.data_as(C.c_void_p),
+                                         grad.ctypes.data_as(C.c_void_p), None), "loss_grad")
+        return loss, grad
+
+    def close(self):
+        if self.ptr:
+            load().bhSurrogateDestroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def surrogate_configs(seed: int, n: int):
+    arr = (RenderParams * n)()
+    check(load().bhSurrogateConfigs(seed, n, arr), "bhSurrogateConfigs")
+    return [arr[i] for i in range(n)]
 
 
 def get_precision() -> BhPrecision:

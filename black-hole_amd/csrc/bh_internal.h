@@ -7,6 +7,7 @@
 #include "../../include/bh/bhtrace.h"
 #include "bh_rays.h"
 #include "bh_band.h"
+#include "bh_mlp.h"
 
 namespace bhk {
 
@@ -77,6 +78,20 @@ struct BatchArgs
     unsigned long long* stamps;   // [nViews + 1]: completion per view, [nViews] = start
 };
 
+// Surrogate training set: rays of many configurations traced by the plain
+// FP32 Kerr integrator (fast policy), labelled for the network.
+struct DatasetArgs
+{
+    const FrameK* frames;   // [nCfg] per-configuration frame constants
+    const RenderParams* params;   // [nCfg] (camera position, M, a, Q)
+    int nCfg, raysPerCfg;   // raysPerCfg: multiple of 64
+    unsigned long long seed;
+    float psiMax;           // largest angle between a ray and the direction to the hole
+    float* inputs;          // [n x 6] normalised network inputs
+    MlpTarget* targets;     // [n]
+    float* escDir;          // [n x 3] exit directions of the plain trace (or null)
+};
+
 struct LaunchShape
 {
     int wgSize;
@@ -88,6 +103,7 @@ hipError_t launchTraceFast(const TraceArgs& A, const LaunchShape& S, hipStream_t
 int traceOccupancyExact(int kerr, int wgSize, int coop, int tiles);
 int traceOccupancyFast(int kerr, int wgSize, int coop, int tiles);
 hipError_t launchBandClassify(const BandArgs& A, hipStream_t st);
+hipError_t launchSurrogateDataset(const DatasetArgs& A, hipStream_t st);
 hipError_t launchBandTraceExact(const BandArgs& A, int blocks, hipStream_t st);
 hipError_t launchBandTraceFast(const BandArgs& A, int blocks, hipStream_t st);
 hipError_t launchProbeExact(const ProbeArgs& A, hipStream_t st);

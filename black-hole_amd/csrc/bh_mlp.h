@@ -1,0 +1,69 @@
+// ---------------------------------------------------------------------------
+// bh_mlp.h -- shapes and launch descriptors of the surrogate network
+// (kernels in bh_mlp.hip, training and routing in host/bh_surrogate.cpp).
+// ---------------------------------------------------------------------------
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace bhk {
+
+#define BHK_MLP_INL __device__ __forceinline__
+
+constexpr int kMlpInputs = 6;                         // b, alpha, a*, q, r_cam, inclination (normalised)
+constexpr int kMlpFreqs = 16;                         // Fourier frequencies per input
+constexpr int kMlpFF = kMlpInputs * kMlpFreqs * 2;    // 192 sin/cos features
+constexpr int kMlpIn = 224;                           // 192 + 6 raw inputs, zero-padded to 7 x 32
+constexpr int kMlpHidden = 128;                       // width of the 4 hidden layers
+constexpr int kMlpLayers = 5;                         // 4 hidden + output
+constexpr int kMlpOut = 5;                            // deflection x2, disk logit, hit radius, log-variance
+constexpr int kMlpOutPad = 16;
+constexpr float kMlpSigma0 = 1e-3f;                   // scale of the predicted exit-direction spread (rad)
+
+// Frequencies 1.25^f * pi / 2, f = 0..15 (pi/2 .. 44.6 per unit input).
+__host__ __device__ inline float mlpFreq(int f)
+{
+    float w = 1.5707963267948966f;
+    for (int i = 0; i < f; ++i) w *= 1.25f;
+    return w;
+}
+
+// Layer l: out x in weights (row-major), in = kMlpIn for l = 0 else kMlpHidden;
+// out = kMlpHidden for l < 4 else kMlpOutPad.
+__host__ __device__ inline int mlpLayerIn(int l) { return l == 0 ? kMlpIn : kMlpHidden; }
+__host__ __device__ inline int mlpLayerOut(int l) { return l == kMlpLayers - 1 ? kMlpOutPad : kMlpHidden; }
+
+struct MlpTarget
+{
+    float d0, d1;       // exit deflection components (radians) minus the analytic base
+    float disk;         // 1 when the plain trace sampled the disk
+    float hitR;         // cylindrical radius of the first disk chord (M units of the params)
+    float escaped;      // 1 escaped, 0 captured
+    float pad[3];
+};
+
+struct GemmArgs
+{
+    const __bf16* A;
+    const __bf16* B;
+    float* C;
+    int M, N, K;
+    int lda, ldb, ldc;
+    int transA, transB;
+    int accumulate;
+};
+
+hipError_t launchGemm(const GemmArgs& g, int splitK, hipStream_t st);
+hipError_t launchFeatures(const float* x, int M, __bf16* X0, hipStream_t st);
+hipError_t launchBiasAct(const float* Y, const float* bias, int M, int N, float* Z, __bf16* A, int silu,
+                         hipStream_t st);
+hipError_t launchSiluBack(const float* dA, const float* Z, int M, int N, __bf16* dZ, float* db, hipStream_t st);
+hipError_t launchLossHead(const float* Y, const float* bias, const MlpTarget* T, int M, __bf16* dOut,
+                          float* lossOut, float* dbHead, hipStream_t st);
+hipError_t launchHeadOut(const float* Y, const float* bias, int M, float* out, hipStream_t st);
+hipError_t launchAdam(float* p, const float* g, float* m, float* v, __bf16* pb, int n, float lr, int t,
+                      hipStream_t st);
+hipError_t launchToBf16(const float* x, __bf16* y, int n, hipStream_t st);
+hipError_t launchZero(float* x, size_t n, hipStream_t st);
+
+}  // namespace bhk

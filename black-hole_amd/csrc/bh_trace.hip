@@ -20,6 +20,7 @@
 // ---------------------------------------------------------------------------
 #include "bh_rays.h"
 #include "bh_internal.h"
+#include "bh_surrogate_geom.h"
 
 #if BHK_FAST
 #define BHK_POLICY MathFast
@@ -339,6 +340,92 @@ int BHK_SUFFIX(traceBatchOccupancy)(int wgSize)
         return 1;
     return nb > 0 ? nb : 1;
 }
+
+#if BHK_FAST
+// ---------------------------------------------------------------- surrogate training set
+// One ray per thread; rays of a configuration are contiguous (raysPerCfg is
+// a multiple of 64, so a wavefront shares one configuration).  psi is
+// uniform in (0, psiMax], alpha uniform in (-pi, pi], from a counter-based
+// hash of (seed, ray index).  The ray is traced by the plain FP32 Kerr
+// integrator with the configuration's disk geometry; its chords are
+// detected (the disk-hit label) but not sampled.
+BHK_INL float uhash(unsigned long long seed, unsigned long long i, unsigned k)
+{
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (i * 4ull + k + 1ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (float)((z >> 40) + 0.5) * (1.0f / 16777216.0f);
+}
+
+__global__ __launch_bounds__(256) void surrogateDatasetKernel(DatasetArgs A)
+{
+    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long n = (unsigned long long)A.nCfg * (unsigned long long)A.raysPerCfg;
+    if (i >= n) return;
+    const int cfg = __builtin_amdgcn_readfirstlane((int)(i / (unsigned long long)A.raysPerCfg));
+    const FrameK& K = A.frames[cfg];
+    const RenderParams& P = A.params[cfg];
+    const v3 o = mk3(P.camPos.x, P.camPos.y, P.camPos.z);
+    v3 er, eth, eph;
+    float rc, thc;
+    camBasis(o, er, eth, eph, rc, thc);
+    const float psi = A.psiMax * uhash(A.seed, i, 0);
+    const float alpha = 3.14159265358979f * (2.f * uhash(A.seed, i, 1) - 1.f);
+    const float sp = sinf(psi), cpsi = cosf(psi), sa = sinf(alpha), ca = cosf(alpha);
+    const v3 dir = unit3(add3(scale3(er, -cpsi), add3(scale3(eth, -sp * ca), scale3(eph, sp * sa))));
+    const RayFrame F = rayFrame(dir, er, eth, eph, rc);
+    float x[kMlpInputs];
+    surrogateInputs(F, P.aSpin / P.M, P.Qc / P.M, rc, thc, x);
+    for (int k = 0; k < kMlpInputs; ++k) A.inputs[i * kMlpInputs + k] = x[k];
+
+    KerrRay<MathFast, false> R;
+    R.init(K, o, dir);
+    Chord ch;
+    bool done = (R.status != RAY_RUNNING);
+    float disk = 0.f, hitR = 0.f;
+    while (!done)
+    {
+        done = R.advance(K, ch);
+        if (ch.n > 0 && disk == 0.f)
+        {
+            disk = 1.f;
+            const v3 m = lerp3(ch.a, ch.b, 0.5f);
+            hitR = sqrtf(m.x * m.x + m.z * m.z);
+        }
+    }
+    MlpTarget t;
+    t.escaped = (R.status == RAY_ESCAPED) ? 1.f : 0.f;
+    t.disk = disk;
+    t.hitR = hitR;
+    t.d0 = 0.f;
+    t.d1 = 0.f;
+    t.pad[0] = t.pad[1] = t.pad[2] = 0.f;
+    if (t.escaped > 0.5f)
+    {
+        float dA, dB;
+        exitDeflection(F, R.escDir, dA, dB);
+        t.d0 = dA - weakDeflection(P.M, F.b, F.psi);
+        t.d1 = dB;
+    }
+    A.targets[i] = t;
+    if (A.escDir)
+    {
+        A.escDir[i * 3 + 0] = R.escDir.x;
+        A.escDir[i * 3 + 1] = R.escDir.y;
+        A.escDir[i * 3 + 2] = R.escDir.z;
+    }
+}
+
+hipError_t launchSurrogateDataset(const DatasetArgs& A, hipStream_t st)
+{
+    const long long n = (long long)A.nCfg * A.raysPerCfg;
+    if (n <= 0) return hipSuccess;
+    if (A.raysPerCfg % 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(surrogateDatasetKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A);
+    return hipGetLastError();
+}
+#endif
 
 hipError_t BHK_SUFFIX(launchBandTrace)(const BandArgs& A, int blocks, hipStream_t st)
 {

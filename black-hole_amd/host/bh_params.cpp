@@ -408,6 +408,39 @@ int bhWorkloadParams(const char* name, int frame, RenderParams* out)
     return 0;
 }
 
+int bhSurrogateConfigs(uint64_t seed, int n, RenderParams* out)
+{
+    if (!out || n < 0) return -1;
+    const double deg = 3.14159265358979323846 / 180.0;
+    SplitMix64 rng{seed};
+    for (int c = 0; c < n; ++c)
+    {
+        double aStar, q;
+        do
+        {
+            aStar = rng.uniform(-0.99, 0.99);
+            q = rng.uniform(0.0, 0.6);
+        } while (aStar * aStar + q * q > 0.995);
+        const double dist = std::exp(rng.uniform(std::log(6.0), std::log(40.0)));
+        const double el = rng.uniform(-80.0, 80.0) * deg;
+        RenderParams P;
+        P.width = 64; P.height = 64;
+        float M = 0.5f, a = (float)aStar, qq = (float)q;
+        BHDerived d;
+        bhValidateAndDerive(BH_KERR_NEWMAN, &M, &a, &qq, 0, &d);
+        P.model = BH_KERR_NEWMAN; P.M = M; P.aSpin = d.a; P.Qc = d.Q;
+        P.rPlus = d.rPlus; P.rPhoton = d.rPhoton; P.rErgo = d.rErgo;
+        P.dPhi = 0.007f; P.maxSteps = 3500;
+        P.diskEnabled = 1;
+        P.diskInner = (float)bhIscoFP64(P.M, P.aSpin, P.Qc);
+        P.diskOuter = 20.0f;
+        P.diskTime = 0.f;
+        bhFillCamera(&P, 0.f, (float)el, (float)dist, 50.f);
+        out[c] = P;
+    }
+    return 0;
+}
+
 int bhWorkloadViews(const char* name, uint64_t seed, int maxViews, RenderParams* out)
 {
     if (!name || std::strcmp(name, "uneven_views") != 0 || maxViews < 0) return -1;

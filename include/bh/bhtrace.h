@@ -247,6 +247,57 @@ void bhEnableKernelTimers(int enable);
 bh_error_t bhLastKernelTimes(float* traceMs, float* shadeMs, float* postMs);
 
 /* ------------------------------------------------------------------------ */
+/* Learned surrogate (no counterpart in the reference)                      */
+/* ------------------------------------------------------------------------ */
+/* A small MLP predicts, for a camera ray, its exit deflection, whether the
+ * plain trace would sample the disk, the radius of that first chord and
+ * the log-variance of its own exit prediction.  Inputs (normalised): screen
+ * impact parameter b, image-plane angle alpha, a*, q, camera radius,
+ * camera inclination -> 16-frequency Fourier features -> 4 x 128 SiLU ->
+ * 5 outputs.  BF16 weights on MFMA with FP32 accumulation; FP32 master
+ * weights trained with Adam by the library's own HIP kernels on rays traced
+ * by the plain FP32 kernel. */
+typedef struct BhSurrogate BhSurrogate;
+typedef struct BhMlpTarget
+{
+    float d0, d1;     /* exit deflection in the ray frame, minus the weak-field term */
+    float disk;       /* 1 when the plain trace sampled the disk */
+    float hitR;       /* cylindrical radius of the first disk chord */
+    float escaped;    /* 1 escaped, 0 captured */
+    float pad[3];
+} BhMlpTarget;
+int        bhSurrogateParamCount(void);
+bh_error_t bhSurrogateCreate(uint64_t seed, BhSurrogate** s);
+void       bhSurrogateDestroy(BhSurrogate* s);
+bh_error_t bhSurrogateSetParams(BhSurrogate* s, const float* host, int count);   /* resets Adam */
+bh_error_t bhSurrogateGetParams(BhSurrogate* s, float* host, int count);
+/* Training configurations: KN black holes a* ~ U[-0.99, 0.99], q ~ U[0, 0.6]
+ * with a*^2 + q^2 <= 0.995, camera radius log-uniform in [6, 40] rs,
+ * elevation uniform in [-80, 80] deg, disk ISCO..20 rs, 'high' steps. */
+int        bhSurrogateConfigs(uint64_t seed, int n, RenderParams* out);
+/* Traces nCfg x raysPerCfg rays (raysPerCfg a multiple of 64) with the plain
+ * FP32 kernel into the training set (replacing any previous one). */
+bh_error_t bhSurrogateDataset(BhSurrogate* s, int nCfg, int raysPerCfg, uint64_t seed, bh_stream_t stream);
+/* Device pointers of the training set: inputs float[n*6], targets
+ * BhMlpTarget[n], plain-trace exit directions float[n*3]. */
+bh_error_t bhSurrogateDatasetView(BhSurrogate* s, float** inputs, void** targets, float** escDir, int* n);
+/* `steps` Adam steps on consecutive batches of the training set (batch a
+ * multiple of 64), cosine learning-rate decay from lr to lr/20.  lossHost
+ * (may be NULL) receives per-step [regression, disk BCE, hit radius] means. */
+bh_error_t bhSurrogateTrain(BhSurrogate* s, int steps, int batch, float lr, float* lossHost, bh_stream_t stream);
+/* Network outputs for n rows of normalised inputs (device float[n*6] ->
+ * device float[n*5]). */
+bh_error_t bhSurrogateForward(BhSurrogate* s, const float* devInputs, int n, float* devOut, bh_stream_t stream);
+/* Loss terms (sums over rows) and the gradient of the mean loss with respect
+ * to every parameter for n rows (host outputs; synchronizes). */
+bh_error_t bhSurrogateLossGrad(BhSurrogate* s, const float* devInputs, const void* devTargets, int n,
+                               float* lossHost, float* gradHost, bh_stream_t stream);
+/* The MFMA GEMM used by the network, exposed for tests: C (+)= op(A) op(B)
+ * with bf16 operands (device) and f32 C (device; zero it when splitK > 1). */
+bh_error_t bhMlpGemmTest(const void* A, int lda, int transA, const void* B, int ldb, int transB, float* C, int ldc,
+                         int M, int N, int K, int splitK, bh_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Device memory and stream utilities (plumbing for callers without HIP)    */
 /* ------------------------------------------------------------------------ */
 bh_error_t  bhDeviceAlloc(void** ptr, size_t bytes);

@@ -1,0 +1,154 @@
+"""CPU restatement of the surrogate network (test infrastructure only).
+
+Restates black-hole_amd/csrc/bh_mlp.hip and host/bh_surrogate.cpp in numpy
+float64: Fourier features, 4 x 128 SiLU layers, 5-output head, the loss and
+its analytic backward pass.  With `bf16=True` the operands the GPU stores in
+bfloat16 (features, activations, weights, output gradients, hidden-layer
+gradients) are rounded to bfloat16 (round to nearest even) at the same
+points, so GPU results can be compared at tight tolerances; with
+`bf16=False` it is a plain FP64 network whose backward pass is pinned by
+finite differences (tests/test_mlp_oracle.py).
+
+The surrogate has no counterpart in the reference.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+N_IN, N_FREQ, N_FF, K_IN, HID, N_LAYERS, N_OUT, OUT_PAD = 6, 16, 192, 224, 128, 5, 5, 16
+SIGMA0 = np.float32(1e-3)
+
+
+def layer_in(l):
+    return K_IN if l == 0 else HID
+
+
+def layer_out(l):
+    return OUT_PAD if l == N_LAYERS - 1 else HID
+
+
+def layout():
+    """(W offsets, b offsets, total) in the library's parameter vector."""
+    w, b, o = [], [], 0
+    for l in range(N_LAYERS):
+        w.append(o)
+        o += layer_out(l) * layer_in(l)
+        b.append(o)
+        o += layer_out(l)
+    return w, b, o
+
+
+W_OFF, B_OFF, N_PARAMS = layout()
+
+
+def unpack(p):
+    Ws = [np.asarray(p[W_OFF[l]:W_OFF[l] + layer_out(l) * layer_in(l)], np.float64).reshape(layer_out(l), layer_in(l))
+          for l in range(N_LAYERS)]
+    bs = [np.asarray(p[B_OFF[l]:B_OFF[l] + layer_out(l)], np.float64) for l in range(N_LAYERS)]
+    return Ws, bs
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    a = np.ascontiguousarray(np.asarray(x, np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def freqs():
+    """1.25^f * pi/2 evaluated in float32 as the kernels do."""
+    out = []
+    w = np.float32(1.5707963267948966)
+    for _ in range(N_FREQ):
+        out.append(w)
+        w = np.float32(w * np.float32(1.25))
+    return np.array(out, np.float32)
+
+
+def features(x):
+    x = np.asarray(x, np.float32)
+    n = x.shape[0]
+    F = np.zeros((n, K_IN), np.float64)
+    fr = freqs()
+    for d in range(N_IN):
+        arg = (fr[None, :] * x[:, d:d + 1]).astype(np.float32).astype(np.float64)
+        F[:, (d * N_FREQ) * 2:(d * N_FREQ + N_FREQ) * 2:2] = np.sin(arg)
+        F[:, (d * N_FREQ) * 2 + 1:(d * N_FREQ + N_FREQ) * 2:2] = np.cos(arg)
+    F[:, N_FF:N_FF + N_IN] = x
+    return F
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def forward(p, x, bf=False):
+    """Returns head outputs y [n, 16] (bias included) and the cache."""
+    r = bf16 if bf else (lambda a: np.asarray(a, np.float64))
+    Ws, bs = unpack(p)
+    Wq = [r(W) for W in Ws]
+    X = r(features(x))
+    cache = {"X": [X], "Z": []}
+    for l in range(N_LAYERS - 1):
+        Z = X @ Wq[l].T + bs[l]
+        X = r(silu(Z))
+        cache["Z"].append(Z)
+        cache["X"].append(X)
+    y = X @ Wq[-1].T + bs[-1]
+    cache["Wq"] = Wq
+    return y, cache
+
+
+def loss_grad(p, x, t, bf=False):
+    """Loss sums [regression, disk BCE, hit radius] over rows and the gradient
+    of the mean loss w.r.t. the parameter vector (library layout)."""
+    r = bf16 if bf else (lambda a: np.asarray(a, np.float64))
+    y, c = forward(p, x, bf)
+    n = y.shape[0]
+    g = np.zeros_like(y)
+    esc = t["escaped"] > 0.5
+    disk = t["disk"].astype(np.float64)
+    y2 = y[:, 2]
+    l_bce = np.maximum(y2, 0) - y2 * disk + np.log1p(np.exp(-np.abs(y2)))
+    g[:, 2] = (1.0 / (1.0 + np.exp(-y2)) - disk) / n
+    inv0 = 1.0 / float(SIGMA0) ** 2
+    e0 = y[:, 0] - t["d0"]
+    e1 = y[:, 1] - t["d1"]
+    s = np.clip(y[:, 4], -20, 20)
+    w = np.exp(-s) * inv0
+    l_reg = np.where(esc, 0.5 * w * (e0 ** 2 + e1 ** 2) + s, 0.0)
+    g[:, 0] = np.where(esc, w * e0 / n, 0.0)
+    g[:, 1] = np.where(esc, w * e1 / n, 0.0)
+    inside = (y[:, 4] > -20) & (y[:, 4] < 20)
+    g[:, 4] = np.where(esc & inside, (1.0 - 0.5 * w * (e0 ** 2 + e1 ** 2)) / n, 0.0)
+    hit = esc & (disk > 0.5)
+    eh = y[:, 3] - t["hitR"] * np.float32(0.1)
+    l_hit = np.where(hit, eh ** 2, 0.0)
+    g[:, 3] = np.where(hit, 2.0 * eh / n, 0.0)
+    g[:, N_OUT:] = 0.0
+    dY = r(g)
+    grad = np.zeros(N_PARAMS, np.float64)
+    Ws, _ = unpack(p)
+    Wq = c["Wq"]
+    L = N_LAYERS - 1
+    grad[W_OFF[L]:W_OFF[L] + OUT_PAD * HID] = (dY.T @ c["X"][L]).ravel()
+    grad[B_OFF[L]:B_OFF[L] + OUT_PAD] = dY.sum(0)
+    dA = dY @ Wq[L]
+    for l in range(L - 1, -1, -1):
+        Z = c["Z"][l]
+        sg = 1.0 / (1.0 + np.exp(-Z))
+        dZf = dA * (sg * (1.0 + Z * (1.0 - sg)))
+        grad[B_OFF[l]:B_OFF[l] + HID] = dZf.sum(0)
+        dZ = r(dZf)
+        grad[W_OFF[l]:W_OFF[l] + HID * layer_in(l)] = (dZ.T @ c["X"][l]).ravel()
+        if l > 0:
+            dA = dZ @ Wq[l]
+    losses = np.array([l_reg.sum(), l_bce.sum(), l_hit.sum()])
+    return losses, grad
+
+
+def mean_loss(p, x, t):
+    """Mean total loss of the FP64 network (for finite differences)."""
+    l, _ = loss_grad(p, x, t, bf=False)
+    return l.sum() / x.shape[0]

@@ -1,0 +1,166 @@
+"""The surrogate network on the GPU against its CPU restatement.
+
+* the MFMA GEMM (every transpose combination, ragged sizes, split K) equals
+  the float64 product of the same bfloat16 operands;
+* the forward pass and the loss gradients of the HIP kernels equal the CPU
+  network (oracle/mlp_oracle.py) evaluated with the same bfloat16 rounding
+  points; the CPU network's own backward pass is pinned by finite
+  differences in tests/test_mlp_oracle.py;
+* training-set labels from the plain FP32 kernel agree with the FP64 oracle
+  trace of the same rays;
+* training with the HIP kernels lowers the loss.
+"""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def to_bf16_bits(a):
+    x = np.ascontiguousarray(np.asarray(a, np.float32))
+    u = x.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return u
+
+
+def bits_to_f64(u):
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("M,N,K", [(37, 45, 70), (16, 16, 32), (130, 16, 224), (64, 128, 128)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("split", [1, 3])
+def test_gemm_layouts(gpu, M, N, K, ta, tb, split):
+    rng = np.random.default_rng(M * 131 + N * 7 + K + ta * 3 + tb * 5 + split)
+    A = rng.normal(size=(K, M) if ta else (M, K))
+    B = rng.normal(size=(N, K) if tb else (K, N))
+    Ab, Bb = to_bf16_bits(A), to_bf16_bits(B)
+    ref = (bits_to_f64(Ab).T if ta else bits_to_f64(Ab)) @ (bits_to_f64(Bb).T if tb else bits_to_f64(Bb))
+    dA, dB = gpu.DeviceBuffer(Ab.nbytes), gpu.DeviceBuffer(Bb.nbytes)
+    dA.upload(Ab)
+    dB.upload(Bb)
+    dC = gpu.DeviceBuffer(M * N * 4)
+    dC.fill(0)
+    lda = Ab.shape[1]
+    ldb = Bb.shape[1]
+    gpu.check(gpu.load().bhMlpGemmTest(dA.ptr, lda, ta, dB.ptr, ldb, tb, dC.ptr, N, M, N, K, split, None), "gemm")
+    gpu.synchronize()
+    C = dC.download(np.float32).reshape(M, N)
+    err = np.abs(C - ref).max() / np.abs(ref).max()
+    assert err < 1e-5, err
+
+
+@pytest.fixture(scope="module")
+def net(gpu):
+    s = gpu.Surrogate(seed=7)
+    yield s
+    s.close()
+
+
+def targets(rng, n):
+    t = np.zeros(n, gpu_dtype())
+    t["d0"] = rng.normal(0, 3e-3, n)
+    t["d1"] = rng.normal(0, 3e-3, n)
+    t["disk"] = rng.random(n) < 0.3
+    t["hitR"] = rng.uniform(2, 20, n)
+    t["escaped"] = rng.random(n) < 0.85
+    return t
+
+
+def gpu_dtype():
+    import bhtrace
+    return bhtrace.MLP_TARGET_DTYPE
+
+
+def test_forward_matches_cpu(gpu, net):
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, (777, 6)).astype(np.float32)
+    y = net.forward(x)
+    p = net.params()
+    yr, _ = mo.forward(p, x, bf=True)
+    err = np.abs(y - yr[:, :5]).max() / np.abs(yr[:, :5]).max()
+    say("forward max rel err %.2e" % err)
+    assert err < 5e-3
+
+
+def test_gradients_match_cpu(gpu, net):
+    rng = np.random.default_rng(4)
+    n = 512
+    x = rng.uniform(-1, 1, (n, 6)).astype(np.float32)
+    t = targets(rng, n)
+    p = net.params()
+    # move the log-variance head away from 0 so every loss term is active
+    loss, g = net.loss_grad(x, t)
+    lr, gr = mo.loss_grad(p, x, t, bf=True)
+    say("loss gpu", loss, "cpu", lr)
+    assert np.allclose(loss, lr, rtol=2e-3, atol=1e-3)
+    for l in range(mo.N_LAYERS):
+        for off, cnt, what in ((mo.W_OFF[l], mo.layer_out(l) * mo.layer_in(l), "W"),
+                               (mo.B_OFF[l], mo.layer_out(l), "b")):
+            a, b = g[off:off + cnt], gr[off:off + cnt]
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            say("layer", l, what, "grad rel err %.2e (norm %.3e)" % (rel, np.linalg.norm(b)))
+            assert rel < 2e-2, (l, what, rel)
+
+
+def test_dataset_labels_vs_fp64(gpu, oracle):
+    s = gpu.Surrogate(seed=1)
+    try:
+        n_cfg, per = 6, 128
+        s.dataset(n_cfg, per, seed=5)
+        X, T, E = s.dataset_view()
+    finally:
+        s.close()
+    cfgs = gpu.surrogate_configs(5, n_cfg)
+    bad, tot, worst = 0, 0, 0.0
+    for c, P in enumerate(cfgs):
+        o = np.array([P.camPos.x, P.camPos.y, P.camPos.z], np.float64)
+        r = np.linalg.norm(o)
+        th = np.arccos(np.clip(o[1] / r, -1, 1))
+        ph = np.arctan2(-o[2], o[0])
+        er = np.array([np.sin(th) * np.cos(ph), np.cos(th), -np.sin(th) * np.sin(ph)])
+        eth = np.array([np.cos(th) * np.cos(ph), -np.sin(th), -np.cos(th) * np.sin(ph)])
+        eph = np.array([-np.sin(ph), 0.0, -np.cos(ph)])
+        rows = slice(c * per, (c + 1) * per)
+        b = X[rows, 0].astype(np.float64) * 40.0
+        al = X[rows, 1].astype(np.float64) * np.pi
+        psi = np.arcsin(np.clip(b / r, 0, 1))
+        d = (-np.cos(psi))[:, None] * er + np.sin(psi)[:, None] * (
+            np.cos(al)[:, None] * (-eth) + np.sin(al)[:, None] * eph)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rec = oracle.probe_dirs(P, d.astype(np.float32), mode=1)
+        esc_o = rec[:, 0] == 1
+        esc_g = T["escaped"][rows] > 0.5
+        both = esc_o & esc_g
+        if both.any():
+            a = E[rows][both].astype(np.float64)
+            bdir = rec[both, 1:4]
+            a /= np.linalg.norm(a, axis=1, keepdims=True)
+            bdir /= np.linalg.norm(bdir, axis=1, keepdims=True)
+            ang = 2 * np.arcsin(np.clip(np.linalg.norm(a - bdir, axis=1) / 2, 0, 1))
+            bad += int((ang > 1e-3).sum())
+            worst = max(worst, float(ang.max()))
+        bad += int((esc_o != esc_g).sum())
+        tot += per
+    say("labels vs FP64: %d of %d off (outcome or > 1e-3 rad), worst %.2e rad" % (bad, tot, worst))
+    assert bad <= 0.01 * tot
+
+
+def test_training_reduces_loss(gpu):
+    s = gpu.Surrogate(seed=3)
+    try:
+        s.dataset(64, 2048, seed=9)
+        loss = s.train(300, batch=8192, lr=2e-3)
+    finally:
+        s.close()
+    first, last = loss[:20].mean(0), loss[-20:].mean(0)
+    say("loss first", first, "last", last)
+    assert np.isfinite(loss).all()
+    assert last[0] < first[0] - 1.0          # regression NLL (includes log-variance) falls
+    assert last[1] < 0.8 * first[1]          # disk-hit BCE falls
