@@ -78,7 +78,7 @@ class BhPrecision(C.Structure):
 
 
 MLP_TARGET_DTYPE = np.dtype([("d0", "<f4"), ("d1", "<f4"), ("disk", "<f4"), ("hitR", "<f4"), ("escaped", "<f4"),
-                             ("pad", "<f4", (3,))])
+                             ("band", "<f4"), ("pad", "<f4", (2,))])
 
 PROBE_DTYPE = np.dtype([("outcome", "<i4"), ("escDir", "<f4", (3,)), ("maxHviol", "<f4"),
                         ("lum", "<f4"), ("steps", "<i4"), ("trans", "<f4")])
@@ -143,6 +143,8 @@ SIGNATURES = [
     ("bhSurrogateTrain", C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _vp]),
     ("bhSurrogateForward", C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
     ("bhSurrogateLossGrad", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp]),
+    ("bhSetSurrogate", C.c_int, [_vp, C.c_float, C.c_float]),
+    ("bhLastSurrogateStats", C.c_int, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), _vp]),
     ("bhMlpGemmTest", C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
                                 C.c_int, C.c_int, _vp]),
     ("bhDeviceAlloc", C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
@@ -518,6 +520,17 @@ class Surrogate:
             self.close()
         except Exception:
             pass
+
+
+def set_surrogate(s, sigma_max: float = 1e-4, disk_prob_max: float = 1e-3) -> None:
+    """Attach a Surrogate to the render path (None detaches)."""
+    check(load().bhSetSurrogate(s.ptr if s is not None else None, sigma_max, disk_prob_max), "bhSetSurrogate")
+
+
+def last_surrogate_stats(stream=None):
+    c, a = C.c_uint(), C.c_uint()
+    check(load().bhLastSurrogateStats(C.byref(c), C.byref(a), stream), "bhLastSurrogateStats")
+    return c.value, a.value
 
 
 def surrogate_configs(seed: int, n: int):

@@ -215,26 +215,42 @@ BHK_INL d3 cameraDir64(const RenderParams& P, int px, int py, float2 jit)
     return dnormalize(F + R * (nx * (double)P.tanHalfFov * (double)P.aspect) + U * (ny * (double)P.tanHalfFov));
 }
 
-// true: trace this ray in FP64.
-BHK_INL bool inCriticalBand(const BandK& B, d3 o, d3 dir)
+// Band statistics of a camera ray: delta (critical curve; < 0 means an
+// ingoing ray has a turning point and escapes, > 0 it falls in) and, for
+// the Kerr models, sin^2 of the polar turning angle (2 when not applicable).
+// Returns false when the invariants are not usable (treated as band).
+BHK_INL bool bandStats(const BandK& B, d3 o, d3 dir, double& delta, double& polar)
 {
 #pragma clang fp contract(off)
+    polar = 2.0;
+    delta = 0.0;
     if (B.kerr)
     {
         const RayInv64 I = kerrInvariants64(B, o, dir);
-        if (!(I.E > 1e-12) || !isfinite(I.E) || !isfinite(I.Lz)) return true;
+        if (!(I.E > 1e-12) || !isfinite(I.E) || !isfinite(I.Lz)) return false;
         const double xi = I.Lz / I.E, eta = I.carter / (I.E * I.E);
-        if (fabs(criticalDelta(B, xi, eta)) < B.critBand) return true;
-        return polarSin2Min(B.a, xi, eta) < B.polarBand;
+        delta = criticalDelta(B, xi, eta);
+        polar = polarSin2Min(B.a, xi, eta);
+        return true;
     }
     const double r0 = dlen(o);
     const d3 e1 = ddivs(o, r0);
     const d3 perp = dir - e1 * ddot(dir, e1);
     const double pl = dlen(perp);
-    if (pl < F2D(1e-6) || !(B.bCrit2 > 0.0)) return false;
+    if (pl < F2D(1e-6) || !(B.bCrit2 > 0.0)) { delta = 1.0; return true; }   // radial / no photon sphere
     const double f0 = fmax(1.0 - 2.0 * B.M / r0 + B.Q * B.Q / (r0 * r0), F2D(1e-6));
     const double b2 = (r0 * pl) * (r0 * pl) / f0;
-    return fabs(1.0 - b2 / B.bCrit2) < B.critBand;
+    delta = 1.0 - b2 / B.bCrit2;
+    return true;
+}
+
+// true: trace this ray in FP64.
+BHK_INL bool inCriticalBand(const BandK& B, d3 o, d3 dir)
+{
+    double delta, polar;
+    if (!bandStats(B, o, dir, delta, polar)) return true;
+    if (!B.kerr && delta == 1.0) return false;
+    return fabs(delta) < B.critBand || polar < B.polarBand;
 }
 
 // ------------------------------------------------------------------ spherical, FP64 (trace.cuh:466-611)

@@ -396,6 +396,22 @@ BHK_INL bool diskSubSample(const DiskParams& D, float E, float Lz, v3 p, float t
     return true;
 }
 
+// True when a sub-sample at p passes the geometric gates of diskSubSample
+// (radial range, |dz| <= 2.6 scale heights, flux or hot-spot region): only
+// such sub-samples can change a ray's radiance or transmittance.
+template <class M> BHK_INL bool diskSubSampleCanContribute(const DiskParams& D, v3 p)
+{
+    float rc = M::sqrt_(p.x * p.x + p.z * p.z);
+    if (rc < D.rin || rc > D.rout) return false;
+    float HR = 0.040f + 0.015f * (rc * D.invOut);
+    float H = HR * rc;
+    float dz = M::div(p.y, fmaxf(H, 1e-4f));
+    if (fabsf(dz) > 2.6f) return false;
+    float Fr = fluxWeight<M>(D, rc) * kNtPeakInv;
+    bool wantSpots = D.spots && rc < D.rinSpot;
+    return !(Fr < 1e-6f && !wantSpots);
+}
+
 // A disk chord [a, b] that survived the slab and radial culls
 // (trace.cuh:274-291): n sub-samples of length ds at emission time tEmit.
 struct Chord

@@ -40,6 +40,32 @@ struct BandArgs
     unsigned* count;        // list length (zero before classification)
     float2 jitter;
     const int* dynIndex;    // device sample index overriding P.sampleIndex (or null)
+    // learned surrogate: escaping non-band rays inside the training domain
+    // become candidates (inputs + pixel index appended; null when off)
+    float* candInputs;      // [W*H x 6]
+    unsigned* candList;     // [W*H]
+    unsigned* candCount;
+    float psiMax;
+};
+
+// Surrogate inference and routing over the candidate list: accepted rays get
+// their record (predicted exit direction, no disk) and mask value 2.
+struct RouteArgs
+{
+    RenderParams P;
+    float2 jitter;
+    const int* dynIndex;
+    const float* inputs;    // [cap x 6]
+    const unsigned* list;   // [cap]
+    const unsigned* count;
+    const __bf16* W;        // network parameters (bf16 weights)
+    const float* Pf;        // network parameters (f32, biases)
+    int wOff[kMlpLayers], bOff[kMlpLayers];
+    float sigmaMax;         // accept when the predicted exit spread is below this (rad)
+    float diskProbMax;      // ... and the predicted disk-hit probability below this
+    BhRayRecord* rec;
+    unsigned char* mask;
+    unsigned* accepted;
 };
 
 struct ProbeArgs
@@ -87,6 +113,7 @@ struct DatasetArgs
     int nCfg, raysPerCfg;   // raysPerCfg: multiple of 64
     unsigned long long seed;
     float psiMax;           // largest angle between a ray and the direction to the hole
+    const BandK* bands;     // [nCfg] band constants (critical / polar band rays are flagged)
     float* inputs;          // [n x 6] normalised network inputs
     MlpTarget* targets;     // [n]
     float* escDir;          // [n x 3] exit directions of the plain trace (or null)
@@ -104,6 +131,19 @@ int traceOccupancyExact(int kerr, int wgSize, int coop, int tiles);
 int traceOccupancyFast(int kerr, int wgSize, int coop, int tiles);
 hipError_t launchBandClassify(const BandArgs& A, hipStream_t st);
 hipError_t launchSurrogateDataset(const DatasetArgs& A, hipStream_t st);
+hipError_t launchSurrogateRoute(const RouteArgs& A, int maxRows, int numCU, hipStream_t st);
+
+// Device view of a surrogate's network (host/bh_surrogate.cpp).
+struct SurrogateNet
+{
+    const __bf16* W;
+    const float* P;
+    int wOff[kMlpLayers], bOff[kMlpLayers];
+};
+SurrogateNet surrogateNet(const BhSurrogate* s);
+
+// FP64 frame constants of the band test and the FP64 integrators (host).
+BandK makeBandK(const RenderParams& P, float critBand, float polarBand);
 hipError_t launchBandTraceExact(const BandArgs& A, int blocks, hipStream_t st);
 hipError_t launchBandTraceFast(const BandArgs& A, int blocks, hipStream_t st);
 hipError_t launchProbeExact(const ProbeArgs& A, hipStream_t st);

@@ -39,7 +39,8 @@ struct MlpTarget
     float disk;         // 1 when the plain trace sampled the disk
     float hitR;         // cylindrical radius of the first disk chord (M units of the params)
     float escaped;      // 1 escaped, 0 captured
-    float pad[3];
+    float band;         // 1: critical / polar band ray (no regression target)
+    float pad[2];
 };
 
 struct GemmArgs
@@ -59,7 +60,7 @@ hipError_t launchBiasAct(const float* Y, const float* bias, int M, int N, float*
                          hipStream_t st);
 hipError_t launchSiluBack(const float* dA, const float* Z, int M, int N, __bf16* dZ, float* db, hipStream_t st);
 hipError_t launchLossHead(const float* Y, const float* bias, const MlpTarget* T, int M, __bf16* dOut,
-                          float* lossOut, float* dbHead, hipStream_t st);
+                          __bf16* dOutTrunk, float* lossOut, float* dbHead, hipStream_t st);
 hipError_t launchHeadOut(const float* Y, const float* bias, int M, float* out, hipStream_t st);
 hipError_t launchAdam(float* p, const float* g, float* m, float* v, __bf16* pb, int n, float lr, int t,
                       hipStream_t st);

@@ -163,15 +163,23 @@ hipError_t launchSiluBack(const float* dA, const float* Z, int M, int N, __bf16*
 }
 
 // ------------------------------------------------------------------ loss head
-// Output columns: 0, 1 exit deflection (radians, relative to the analytic
-// base in column order of MlpTarget), 2 disk-hit logit, 3 hit radius / 10,
-// 4 s = log(sigma^2 / sigma0^2).  Loss per valid row:
-//   escaped:   0.5 exp(-s) |e|^2 / sigma0^2 + s  (+ BCE(disk logit))
-//   disk rows: + (hitR_pred - hitR / 10)^2
-//   captured rows: BCE only.
-// dOut (bf16, [M x 16]) gets dL/dY / M; loss terms are summed into lossOut.
+// Output columns: 0, 1 exit deflection minus the spherical base (rad), 2
+// disk-hit logit, 3 hit radius / 10, 4 s = log of the expected squared exit
+// error over sigma0^2.  Per row (mean over the M rows of the batch):
+//   all rows:               BCE(disk logit, disk)
+//   escaped, not band:      kRegW |e|^2 for |e| <= kClip (constant beyond:
+//                           outliers do not pull the prediction), and
+//                           0.1 (s - log(min(|e|, kClip)^2 / sigma0^2 + 1e-6))^2
+//                           with the error term held fixed (s learns the
+//                           error it will make)
+//   escaped, not band, disk: (hitR_pred - hitR / 10)^2
+// dOut gets dL/dY for the head; dOutTrunk the same with the s column zero,
+// so the log-variance head does not shape the hidden layers.
+constexpr float kMlpClip = 0.02f;
+constexpr float kMlpRegW = 1e3f;
+
 __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const float* bias, const MlpTarget* T, int M,
-                                                      __bf16* dOut, float* lossOut)
+                                                      __bf16* dOut, __bf16* dOutTrunk, float* lossOut)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float lReg = 0.f, lBce = 0.f, lHit = 0.f;
@@ -185,20 +193,24 @@ __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const floa
         for (int c = 0; c < kMlpOutPad; ++c) g[c] = 0.f;
         const MlpTarget t = T[i];
         const float invM = 1.f / (float)M;
-        // disk-hit logit (all rows)
         const float p = 1.f / (1.f + __expf(-y[2]));
         lBce = fmaxf(y[2], 0.f) - y[2] * t.disk + __logf(1.f + __expf(-fabsf(y[2])));
         g[2] = (p - t.disk) * invM;
-        if (t.escaped > 0.5f)
+        if (t.escaped > 0.5f && t.band < 0.5f)
         {
-            const float inv0 = 1.f / (kMlpSigma0 * kMlpSigma0);
             const float e0 = y[0] - t.d0, e1 = y[1] - t.d1;
-            const float s = fminf(fmaxf(y[4], -20.f), 20.f);
-            const float w = __expf(-s) * inv0;
-            lReg = 0.5f * w * (e0 * e0 + e1 * e1) + s;
-            g[0] = w * e0 * invM;
-            g[1] = w * e1 * invM;
-            g[4] = (y[4] > -20.f && y[4] < 20.f) ? (1.f - 0.5f * w * (e0 * e0 + e1 * e1)) * invM : 0.f;
+            const float en2 = e0 * e0 + e1 * e1;
+            const float c2 = fminf(en2, kMlpClip * kMlpClip);
+            lReg = kMlpRegW * c2;
+            if (en2 <= kMlpClip * kMlpClip)
+            {
+                g[0] = 2.f * kMlpRegW * e0 * invM;
+                g[1] = 2.f * kMlpRegW * e1 * invM;
+            }
+            const float tgt = __logf(c2 / (kMlpSigma0 * kMlpSigma0) + 1e-6f);
+            const float ds = y[4] - tgt;
+            lReg += 0.1f * ds * ds;
+            g[4] = 0.2f * ds * invM;
             if (t.disk > 0.5f)
             {
                 const float eh = y[3] - t.hitR * 0.1f;
@@ -207,7 +219,11 @@ __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const floa
             }
         }
 #pragma unroll
-        for (int c = 0; c < kMlpOutPad; ++c) dOut[(size_t)i * kMlpOutPad + c] = tobf(g[c]);
+        for (int c = 0; c < kMlpOutPad; ++c)
+        {
+            dOut[(size_t)i * kMlpOutPad + c] = tobf(g[c]);
+            dOutTrunk[(size_t)i * kMlpOutPad + c] = tobf(c == 4 ? 0.f : g[c]);
+        }
     }
     // wavefront reduction of the three loss terms
     for (int off = 32; off > 0; off >>= 1)
@@ -240,10 +256,11 @@ __global__ __launch_bounds__(256) void headBiasGradKernel(const __bf16* dOut, in
 }
 
 hipError_t launchLossHead(const float* Y, const float* bias, const MlpTarget* T, int M, __bf16* dOut,
-                          float* lossOut, float* dbHead, hipStream_t st)
+                          __bf16* dOutTrunk, float* lossOut, float* dbHead, hipStream_t st)
 {
     if (M <= 0) return hipSuccess;
-    hipLaunchKernelGGL(lossHeadKernel, dim3((M + 255) / 256), dim3(256), 0, st, Y, bias, T, M, dOut, lossOut);
+    hipLaunchKernelGGL(lossHeadKernel, dim3((M + 255) / 256), dim3(256), 0, st, Y, bias, T, M, dOut, dOutTrunk,
+                       lossOut);
     hipLaunchKernelGGL(headBiasGradKernel, dim3((M + 255) / 256), dim3(256), 0, st, dOut, M, dbHead);
     return hipGetLastError();
 }

@@ -347,8 +347,9 @@ int BHK_SUFFIX(traceBatchOccupancy)(int wgSize)
 // a multiple of 64, so a wavefront shares one configuration).  psi is
 // uniform in (0, psiMax], alpha uniform in (-pi, pi], from a counter-based
 // hash of (seed, ray index).  The ray is traced by the plain FP32 Kerr
-// integrator with the configuration's disk geometry; its chords are
-// detected (the disk-hit label) but not sampled.
+// integrator with the configuration's disk geometry; the disk-hit label is
+// set when a sub-sample of one of its chords passes the geometric gates of
+// the volumetric sampler (the radiance itself is not evaluated).
 BHK_INL float uhash(unsigned long long seed, unsigned long long i, unsigned k)
 {
     unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (i * 4ull + k + 1ull);
@@ -389,23 +390,39 @@ __global__ __launch_bounds__(256) void surrogateDatasetKernel(DatasetArgs A)
         done = R.advance(K, ch);
         if (ch.n > 0 && disk == 0.f)
         {
-            disk = 1.f;
-            const v3 m = lerp3(ch.a, ch.b, 0.5f);
-            hitR = sqrtf(m.x * m.x + m.z * m.z);
+            // disk-hit label: some sub-sample of the chord passes the
+            // geometric gates (without it the disk cannot touch the ray)
+            for (int j = 0; j < ch.n; ++j)
+            {
+                const v3 p = lerp3(ch.a, ch.b, MathFast::div((float)j + 0.5f, (float)ch.n));
+                if (diskSubSampleCanContribute<MathFast>(K.D, p))
+                {
+                    disk = 1.f;
+                    hitR = sqrtf(p.x * p.x + p.z * p.z);
+                    break;
+                }
+            }
         }
     }
     MlpTarget t;
+    {
+        // band rays (critical curve, polar turning point) carry no regression target
+        double delta, polar;
+        const BandK& Bk = A.bands[cfg];
+        const bool ok = bandStats(Bk, dk3(o.x, o.y, o.z), dk3(dir.x, dir.y, dir.z), delta, polar);
+        t.band = (!ok || fabs(delta) < Bk.critBand || polar < Bk.polarBand) ? 1.f : 0.f;
+    }
     t.escaped = (R.status == RAY_ESCAPED) ? 1.f : 0.f;
     t.disk = disk;
     t.hitR = hitR;
     t.d0 = 0.f;
     t.d1 = 0.f;
-    t.pad[0] = t.pad[1] = t.pad[2] = 0.f;
+    t.pad[0] = t.pad[1] = 0.f;
     if (t.escaped > 0.5f)
     {
         float dA, dB;
         exitDeflection(F, R.escDir, dA, dB);
-        t.d0 = dA - weakDeflection(P.M, F.b, F.psi);
+        t.d0 = dA - sphDeflection(P.M, P.Qc, rc, F.psi, 120.f * P.M);
         t.d1 = dB;
     }
     A.targets[i] = t;

@@ -24,6 +24,59 @@
 #include <vector>
 #include "../csrc/bh_internal.h"
 
+namespace bhk {
+
+// FP64 frame constants: RenderParams floats widened, derived values in
+// double with the reference's float literals widened (trace.cuh:466-890).
+BandK makeBandK(const RenderParams& P, float critBand, float polarBand)
+{
+    BandK B{};
+    const bool kerr = (P.model == BH_KERR || P.model == BH_KERR_NEWMAN);
+    B.kerr = kerr ? 1 : 0;
+    B.M = P.M;
+    B.a = kerr ? (double)P.aSpin : 0.0;
+    B.Q = P.Qc;
+    B.rPlus = P.rPlus;
+    B.rPhoton = P.rPhoton;
+    B.dPhi = P.dPhi;
+    B.diskOuter = P.diskOuter;
+    B.diskTime = P.diskTime;
+    B.rCap = (B.rPlus > (double)1e-6f) ? B.rPlus : (double)kSingCutOverM * B.M;
+    B.rFloor = std::max(B.rPlus, B.rCap);
+    B.escR = (double)120.f * B.M;
+    B.capR = B.rCap * (double)1.002f + (double)1e-3f;
+    B.rPhotonRefine = (double)1.5f * B.rPhoton;
+    B.diskOuterCap = B.diskOuter * (double)1.3f;
+    B.sixM = 6.0 * B.M;
+    B.kRad = (double)8.0f * B.dPhi;
+    B.kAng = (double)2.2f * B.dPhi;
+    B.uHor = 1.0 / std::max(B.rCap, (double)1e-6f);
+    B.uEscape = 1.0 / B.escR;
+    B.invNineM = 1.0 / (9.0 * B.M);
+    B.uBudget = 1.0 / std::max((double)1.5f * B.rPhoton, 3.0 * B.M);
+    B.hFloor = B.dPhi * (double)0.03f;
+    B.maxSteps = P.maxSteps;
+    B.disk = P.diskEnabled != 0;
+    // spherical models: b_c^2 = r_ph^2 / f(r_ph), r_ph = (3M + sqrt(9M^2 - 8Q^2)) / 2
+    const double Q2 = B.Q * B.Q, disc = 9.0 * B.M * B.M - 8.0 * Q2;
+    B.bCrit2 = 0.0;
+    if (!kerr && disc >= 0.0)
+    {
+        const double rph = 0.5 * (3.0 * B.M + std::sqrt(disc));
+        const double f = 1.0 - 2.0 * B.M / rph + Q2 / (rph * rph);
+        if (f > 0.0) B.bCrit2 = rph * rph / f;
+    }
+    // Kerr models: scan the radial potential over [r+ (1 + 1e-3), 5M]
+    const double rp = bhHorizonOuterFP64(B.M, B.a, B.Q);
+    B.rScanLo = rp > 0.0 ? rp * 1.001 : (double)kSingCutOverM * B.M;
+    B.rScanHi = std::max(5.0 * B.M, B.rScanLo * 1.5);
+    B.critBand = critBand;
+    B.polarBand = polarBand;
+    return B;
+}
+
+}  // namespace bhk
+
 namespace {
 
 struct Scratch
@@ -48,6 +101,11 @@ struct Scratch
     unsigned char* bandMask = nullptr;
     unsigned* bandList = nullptr;
     size_t bandCap = 0;
+    // surrogate candidates (inputs, pixel list); counts in counter[32] and
+    // counter[48] (accepted)
+    float* candInputs = nullptr;
+    unsigned* candList = nullptr;
+    size_t candCap = 0;
     hipEvent_t bandEv[2] = {nullptr, nullptr};   // timing: band pass start / end
     // the FP32 tile trace runs on `aux` concurrently with the FP64 band
     // kernel on the caller's stream: fork after classification, join before
@@ -61,6 +119,15 @@ std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 BhSchedule g_sched = {256, 4096, 64, 1, 0};   // wgSize, stepsPerChunk, raysPerRefill, mathMode, diskCoop
 BhPrecision g_prec = {1, 3e-2f, 1e-3f};
 constexpr int kBandCountSlot = 16;            // counter[16]: band list length (own 64-byte line)
+constexpr int kCandCountSlot = 32;            // counter[32]: surrogate candidates
+constexpr int kAcceptSlot = 48;               // counter[48]: rays answered by the surrogate
+// learned surrogate attached to the render path (null: off)
+struct SurrogateRoute
+{
+    BhSurrogate* s = nullptr;
+    float sigmaMax = 1e-4f;
+    float diskProbMax = 1e-3f;
+} g_sur;
 std::atomic<int> g_timers{0};
 std::pair<int, hipStream_t> g_lastTimed{-1, nullptr};
 int g_numCU[64] = {0};
@@ -149,6 +216,28 @@ BhPrecision currentPrecision()
     return g_prec;
 }
 
+hipError_t getCandScratch(Scratch* S, size_t n, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n <= S->candCap) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    (void)hipFree(S->candInputs);
+    (void)hipFree(S->candList);
+    S->candInputs = nullptr; S->candList = nullptr; S->candCap = 0;
+    e = hipMalloc(&S->candInputs, n * 6 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&S->candList, n * sizeof(unsigned));
+    if (e != hipSuccess) return e;
+    S->candCap = n;
+    return hipSuccess;
+}
+
+SurrogateRoute currentSurrogate()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_sur;
+}
+
 hipError_t getBandScratch(Scratch* S, size_t n, hipStream_t st)
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -176,54 +265,6 @@ hipError_t getBandScratch(Scratch* S, size_t n, hipStream_t st)
     return hipSuccess;
 }
 
-// FP64 frame constants: RenderParams floats widened, derived values in
-// double with the reference's float literals widened (trace.cuh:466-890).
-bhk::BandK makeBandK(const RenderParams& P, const BhPrecision& pr)
-{
-    bhk::BandK B{};
-    const bool kerr = (P.model == BH_KERR || P.model == BH_KERR_NEWMAN);
-    B.kerr = kerr ? 1 : 0;
-    B.M = P.M;
-    B.a = kerr ? (double)P.aSpin : 0.0;
-    B.Q = P.Qc;
-    B.rPlus = P.rPlus;
-    B.rPhoton = P.rPhoton;
-    B.dPhi = P.dPhi;
-    B.diskOuter = P.diskOuter;
-    B.diskTime = P.diskTime;
-    B.rCap = (B.rPlus > (double)1e-6f) ? B.rPlus : (double)bhk::kSingCutOverM * B.M;
-    B.rFloor = std::max(B.rPlus, B.rCap);
-    B.escR = (double)120.f * B.M;
-    B.capR = B.rCap * (double)1.002f + (double)1e-3f;
-    B.rPhotonRefine = (double)1.5f * B.rPhoton;
-    B.diskOuterCap = B.diskOuter * (double)1.3f;
-    B.sixM = 6.0 * B.M;
-    B.kRad = (double)8.0f * B.dPhi;
-    B.kAng = (double)2.2f * B.dPhi;
-    B.uHor = 1.0 / std::max(B.rCap, (double)1e-6f);
-    B.uEscape = 1.0 / B.escR;
-    B.invNineM = 1.0 / (9.0 * B.M);
-    B.uBudget = 1.0 / std::max((double)1.5f * B.rPhoton, 3.0 * B.M);
-    B.hFloor = B.dPhi * (double)0.03f;
-    B.maxSteps = P.maxSteps;
-    B.disk = P.diskEnabled != 0;
-    // spherical models: b_c^2 = r_ph^2 / f(r_ph), r_ph = (3M + sqrt(9M^2 - 8Q^2)) / 2
-    const double Q2 = B.Q * B.Q, disc = 9.0 * B.M * B.M - 8.0 * Q2;
-    B.bCrit2 = 0.0;
-    if (!kerr && disc >= 0.0)
-    {
-        const double rph = 0.5 * (3.0 * B.M + std::sqrt(disc));
-        const double f = 1.0 - 2.0 * B.M / rph + Q2 / (rph * rph);
-        if (f > 0.0) B.bCrit2 = rph * rph / f;
-    }
-    // Kerr models: scan the radial potential over [r+ (1 + 1e-3), 5M]
-    const double rp = bhHorizonOuterFP64(B.M, B.a, B.Q);
-    B.rScanLo = rp > 0.0 ? rp * 1.001 : (double)bhk::kSingCutOverM * B.M;
-    B.rScanHi = std::max(5.0 * B.M, B.rScanLo * 1.5);
-    B.critBand = pr.critBand;
-    B.polarBand = pr.polarBand;
-    return B;
-}
 
 inline bool isKerr(int model) { return model == BH_KERR || model == BH_KERR_NEWMAN; }
 
@@ -249,7 +290,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         hipError_t e = getBandScratch(Sc, n, st);
         if (e != hipSuccess) return e;
         BA.P = P;
-        BA.B = makeBandK(P, pr);
+        BA.B = bhk::makeBandK(P, pr.critBand, pr.polarBand);
         BA.D = bhk::makeDisk(P, BA.B.kerr ? P.aSpin : 0.f, P.Qc);
         BA.rec = rec;
         BA.mask = Sc->bandMask;
@@ -257,9 +298,48 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.count = counter + kBandCountSlot;
         BA.jitter = jitterOf(P.accumMode, P.sampleIndex);
         BA.dynIndex = dynIndex;
+        BA.candInputs = nullptr;
+        BA.candList = nullptr;
+        BA.candCount = counter + kCandCountSlot;
+        BA.psiMax = (float)(65.0 * 3.14159265358979323846 / 180.0);
+        const SurrogateRoute sur = currentSurrogate();
+        if (sur.s)
+        {
+            e = getCandScratch(Sc, n, st);
+            if (e != hipSuccess) return e;
+            BA.candInputs = Sc->candInputs;
+            BA.candList = Sc->candList;
+        }
         e = bhk::launchResetCounter(BA.count, st);
+        if (e == hipSuccess) e = bhk::launchResetCounter(counter + kCandCountSlot, st);
+        if (e == hipSuccess) e = bhk::launchResetCounter(counter + kAcceptSlot, st);
         if (e == hipSuccess) e = bhk::launchBandClassify(BA, st);
         if (e != hipSuccess) return e;
+        if (sur.s)
+        {
+            // network inference + routing: accepted rays get their record and
+            // mask 2 before the RK4 tracers start
+            const bhk::SurrogateNet net = bhk::surrogateNet(sur.s);
+            bhk::RouteArgs R;
+            R.P = P;
+            R.jitter = BA.jitter;
+            R.dynIndex = dynIndex;
+            R.inputs = Sc->candInputs;
+            R.list = Sc->candList;
+            R.count = counter + kCandCountSlot;
+            R.W = net.W;
+            R.Pf = net.P;
+            for (int l = 0; l < bhk::kMlpLayers; ++l) { R.wOff[l] = net.wOff[l]; R.bOff[l] = net.bOff[l]; }
+            R.sigmaMax = sur.sigmaMax;
+            R.diskProbMax = sur.diskProbMax;
+            R.rec = rec;
+            R.mask = Sc->bandMask;
+            R.accepted = counter + kAcceptSlot;
+            int dv = 0;
+            if (hipGetDevice(&dv) != hipSuccess) dv = 0;
+            e = bhk::launchSurrogateRoute(R, (int)n, numCUs(dv), st);
+            if (e != hipSuccess) return e;
+        }
     }
     bhk::TraceArgs A;
     A.bandMask = band ? Sc->bandMask : nullptr;
@@ -906,6 +986,40 @@ bh_error_t bhSetPrecision(const BhPrecision* p)
     g_prec = *p;
     return hipSuccess;
 }
+bh_error_t bhSetSurrogate(BhSurrogate* s, float sigmaMax, float diskProbMax)
+{
+    if (s && (!(sigmaMax > 0.f) || !(diskProbMax > 0.f) || diskProbMax > 1.f)) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_sur.s = s;
+    if (s)
+    {
+        g_sur.sigmaMax = sigmaMax;
+        g_sur.diskProbMax = diskProbMax;
+    }
+    return hipSuccess;
+}
+
+bh_error_t bhLastSurrogateStats(unsigned* candidates, unsigned* accepted, bh_stream_t stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    Scratch* S = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_scratch.find({dev, st});
+        if (it == g_scratch.end() || !it->second.counter) return hipErrorNotReady;
+        S = &it->second;
+    }
+    e = hipStreamSynchronize(st);
+    if (e == hipSuccess && candidates)
+        e = hipMemcpy(candidates, S->counter + kCandCountSlot, sizeof(unsigned), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && accepted)
+        e = hipMemcpy(accepted, S->counter + kAcceptSlot, sizeof(unsigned), hipMemcpyDeviceToHost);
+    return e;
+}
+
 bh_error_t bhLastBandStats(unsigned* count, float* bandMs, bh_stream_t stream)
 {
     hipStream_t st = (hipStream_t)stream;

@@ -81,6 +81,7 @@ struct BhSurrogate
     __bf16* A[4] = {};           // activations [cap x 128]
     float* Y5 = nullptr;         // [cap x 16]
     __bf16* dOut = nullptr;      // [cap x 16]
+    __bf16* dOutT = nullptr;     // [cap x 16], log-variance column zero
     float* dA = nullptr;         // [cap x 128]
     __bf16* dZ = nullptr;        // [cap x 128]
     float* loss = nullptr;       // per-step loss terms [lossCap x 3]
@@ -101,6 +102,8 @@ void freeWorkspace(BhSurrogate* s)
     for (int l = 0; l < 4; ++l) { (void)hipFree(s->Z[l]); (void)hipFree(s->A[l]); s->Z[l] = nullptr; s->A[l] = nullptr; }
     (void)hipFree(s->Y5);
     (void)hipFree(s->dOut);
+    (void)hipFree(s->dOutT);
+    s->dOutT = nullptr;
     (void)hipFree(s->dA);
     (void)hipFree(s->dZ);
     s->X0 = nullptr; s->Y5 = nullptr; s->dOut = nullptr; s->dA = nullptr; s->dZ = nullptr;
@@ -122,6 +125,7 @@ hipError_t ensureWorkspace(BhSurrogate* s, int rows, hipStream_t st)
     }
     if (e == hipSuccess) e = dalloc(&s->Y5, n * kMlpOutPad);
     if (e == hipSuccess) e = dalloc(&s->dOut, n * kMlpOutPad);
+    if (e == hipSuccess) e = dalloc(&s->dOutT, n * kMlpOutPad);
     if (e == hipSuccess) e = dalloc(&s->dA, n * kMlpHidden);
     if (e == hipSuccess) e = dalloc(&s->dZ, n * kMlpHidden);
     if (e != hipSuccess) { freeWorkspace(s); return e; }
@@ -165,14 +169,15 @@ hipError_t forward(BhSurrogate* s, const float* x, int rows, hipStream_t st)
 // the caller); loss terms into lossSlot[0..2].
 hipError_t backward(BhSurrogate* s, const bhk::MlpTarget* T, int rows, float* lossSlot, hipStream_t st)
 {
-    hipError_t e = bhk::launchLossHead(s->Y5, s->p + kL.b[4], T, rows, s->dOut, lossSlot, s->g + kL.b[4], st);
+    hipError_t e =
+        bhk::launchLossHead(s->Y5, s->p + kL.b[4], T, rows, s->dOut, s->dOutT, lossSlot, s->g + kL.b[4], st);
     const int split = std::max(1, std::min(64, rows / 2048));
     // head: dW4 = dOut^T A3, dA3 = dOut W4
     if (e == hipSuccess)
         e = bhk::launchGemm(gemm(s->dOut, kMlpOutPad, 1, s->A[3], kMlpHidden, 0, s->g + kL.w[4], kMlpHidden,
                                  kMlpOutPad, kMlpHidden, rows, 1), split, st);
     if (e == hipSuccess)
-        e = bhk::launchGemm(gemm(s->dOut, kMlpOutPad, 0, s->pb + kL.w[4], kMlpHidden, 0, s->dA, kMlpHidden, rows,
+        e = bhk::launchGemm(gemm(s->dOutT, kMlpOutPad, 0, s->pb + kL.w[4], kMlpHidden, 0, s->dA, kMlpHidden, rows,
                                  kMlpHidden, kMlpOutPad, 0), 1, st);
     for (int l = 3; l >= 0 && e == hipSuccess; --l)
     {
@@ -195,6 +200,21 @@ hipError_t refreshBf16(BhSurrogate* s, hipStream_t st)
 }
 
 }  // namespace
+
+namespace bhk {
+SurrogateNet surrogateNet(const BhSurrogate* s)
+{
+    SurrogateNet n;
+    n.W = s->pb;
+    n.P = s->p;
+    for (int l = 0; l < kMlpLayers; ++l)
+    {
+        n.wOff[l] = (int)kL.w[l];
+        n.bOff[l] = (int)kL.b[l];
+    }
+    return n;
+}
+}  // namespace bhk
 
 extern "C" {
 
@@ -270,7 +290,14 @@ bh_error_t bhSurrogateDataset(BhSurrogate* s, int nCfg, int raysPerCfg, uint64_t
     std::vector<RenderParams> P((size_t)nCfg);
     std::vector<bhk::FrameK> K((size_t)nCfg);
     bhSurrogateConfigs(seed, nCfg, P.data());
-    for (int c = 0; c < nCfg; ++c) K[(size_t)c] = bhk::makeFrame(P[(size_t)c]);
+    std::vector<bhk::BandK> Bk((size_t)nCfg);
+    BhPrecision prec;
+    bhGetPrecision(&prec);
+    for (int c = 0; c < nCfg; ++c)
+    {
+        K[(size_t)c] = bhk::makeFrame(P[(size_t)c]);
+        Bk[(size_t)c] = bhk::makeBandK(P[(size_t)c], prec.critBand, prec.polarBand);
+    }
     const size_t n = (size_t)nCfg * (size_t)raysPerCfg;
     std::lock_guard<std::mutex> lk(s->mu);
     hipError_t e = hipStreamSynchronize(st);
@@ -282,6 +309,9 @@ bh_error_t bhSurrogateDataset(BhSurrogate* s, int nCfg, int raysPerCfg, uint64_t
     if (e == hipSuccess) e = dalloc(&s->setEsc, n * 3);
     bhk::FrameK* dK = nullptr;
     RenderParams* dP = nullptr;
+    bhk::BandK* dB = nullptr;
+    if (e == hipSuccess) e = dalloc(&dB, Bk.size());
+    if (e == hipSuccess) e = hipMemcpy(dB, Bk.data(), Bk.size() * sizeof(bhk::BandK), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = dalloc(&dK, K.size());
     if (e == hipSuccess) e = dalloc(&dP, P.size());
     if (e == hipSuccess) e = hipMemcpy(dK, K.data(), K.size() * sizeof(bhk::FrameK), hipMemcpyHostToDevice);
@@ -295,6 +325,7 @@ bh_error_t bhSurrogateDataset(BhSurrogate* s, int nCfg, int raysPerCfg, uint64_t
         A.raysPerCfg = raysPerCfg;
         A.seed = seed * 0x2545F4914F6CDD1Dull + 1ull;
         A.psiMax = (float)(65.0 * 3.14159265358979323846 / 180.0);
+        A.bands = dB;
         A.inputs = s->setX;
         A.targets = s->setT;
         A.escDir = s->setEsc;
@@ -303,6 +334,7 @@ bh_error_t bhSurrogateDataset(BhSurrogate* s, int nCfg, int raysPerCfg, uint64_t
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(dK);
     (void)hipFree(dP);
+    (void)hipFree(dB);
     if (e == hipSuccess) s->setN = (int)n;
     return e;
 }

@@ -264,7 +264,8 @@ typedef struct BhMlpTarget
     float disk;       /* 1 when the plain trace sampled the disk */
     float hitR;       /* cylindrical radius of the first disk chord */
     float escaped;    /* 1 escaped, 0 captured */
-    float pad[3];
+    float band;       /* 1 critical / polar band ray: no regression target */
+    float pad[2];
 } BhMlpTarget;
 int        bhSurrogateParamCount(void);
 bh_error_t bhSurrogateCreate(uint64_t seed, BhSurrogate** s);
@@ -292,6 +293,16 @@ bh_error_t bhSurrogateForward(BhSurrogate* s, const float* devInputs, int n, flo
  * to every parameter for n rows (host outputs; synchronizes). */
 bh_error_t bhSurrogateLossGrad(BhSurrogate* s, const float* devInputs, const void* devTargets, int n,
                                float* lossHost, float* gradHost, bh_stream_t stream);
+/* Attach a trained surrogate to the render path (NULL detaches).  With the
+ * FP64 band pass on, every escaping non-band camera ray inside the training
+ * domain (psi <= 65 deg, camera radius 6..40 rs, q <= 0.6) is evaluated by
+ * the network; rays whose predicted disk-hit probability is below
+ * diskProbMax and predicted exit spread below sigmaMax (rad) take the
+ * predicted exit direction and skip RK4; all others are traced by RK4.  The
+ * surrogate must outlive its attachment. */
+bh_error_t bhSetSurrogate(BhSurrogate* s, float sigmaMax, float diskProbMax);
+/* Candidates and accepted rays of the last frame on `stream` (synchronizes). */
+bh_error_t bhLastSurrogateStats(unsigned* candidates, unsigned* accepted, bh_stream_t stream);
 /* The MFMA GEMM used by the network, exposed for tests: C (+)= op(A) op(B)
  * with bf16 operands (device) and f32 C (device; zero it when splitK > 1). */
 bh_error_t bhMlpGemmTest(const void* A, int lda, int transA, const void* B, int ldb, int transB, float* C, int ldc,

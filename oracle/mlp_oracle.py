@@ -100,41 +100,52 @@ def forward(p, x, bf=False):
     return y, cache
 
 
-def loss_grad(p, x, t, bf=False):
-    """Loss sums [regression, disk BCE, hit radius] over rows and the gradient
-    of the mean loss w.r.t. the parameter vector (library layout)."""
+CLIP = 0.02
+REG_W = 1e3
+
+
+def loss_grad(p, x, t, bf=False, tgt=None):
+    """Loss sums [regression + log-variance, disk BCE, hit radius] over rows
+    and the gradient of the mean loss w.r.t. the parameter vector (library
+    layout).  The log-variance target log(min(|e|, clip)^2 / sigma0^2 + 1e-6)
+    is held fixed in the gradient; pass `tgt` to fix it explicitly (finite
+    differences).  Returns (losses, grad, tgt)."""
     r = bf16 if bf else (lambda a: np.asarray(a, np.float64))
     y, c = forward(p, x, bf)
     n = y.shape[0]
     g = np.zeros_like(y)
-    esc = t["escaped"] > 0.5
+    reg = (t["escaped"] > 0.5) & (t["band"] < 0.5)
     disk = t["disk"].astype(np.float64)
     y2 = y[:, 2]
     l_bce = np.maximum(y2, 0) - y2 * disk + np.log1p(np.exp(-np.abs(y2)))
     g[:, 2] = (1.0 / (1.0 + np.exp(-y2)) - disk) / n
-    inv0 = 1.0 / float(SIGMA0) ** 2
     e0 = y[:, 0] - t["d0"]
     e1 = y[:, 1] - t["d1"]
-    s = np.clip(y[:, 4], -20, 20)
-    w = np.exp(-s) * inv0
-    l_reg = np.where(esc, 0.5 * w * (e0 ** 2 + e1 ** 2) + s, 0.0)
-    g[:, 0] = np.where(esc, w * e0 / n, 0.0)
-    g[:, 1] = np.where(esc, w * e1 / n, 0.0)
-    inside = (y[:, 4] > -20) & (y[:, 4] < 20)
-    g[:, 4] = np.where(esc & inside, (1.0 - 0.5 * w * (e0 ** 2 + e1 ** 2)) / n, 0.0)
-    hit = esc & (disk > 0.5)
+    en2 = e0 ** 2 + e1 ** 2
+    c2 = np.minimum(en2, CLIP ** 2)
+    inside = en2 <= CLIP ** 2
+    if tgt is None:
+        tgt = np.log(c2 / float(SIGMA0) ** 2 + 1e-6)
+    ds = y[:, 4] - tgt
+    l_reg = np.where(reg, REG_W * c2 + 0.1 * ds ** 2, 0.0)
+    g[:, 0] = np.where(reg & inside, 2.0 * REG_W * e0 / n, 0.0)
+    g[:, 1] = np.where(reg & inside, 2.0 * REG_W * e1 / n, 0.0)
+    g[:, 4] = np.where(reg, 0.2 * ds / n, 0.0)
+    hit = reg & (disk > 0.5)
     eh = y[:, 3] - t["hitR"] * np.float32(0.1)
     l_hit = np.where(hit, eh ** 2, 0.0)
     g[:, 3] = np.where(hit, 2.0 * eh / n, 0.0)
     g[:, N_OUT:] = 0.0
     dY = r(g)
+    gT = g.copy()
+    gT[:, 4] = 0.0               # the log-variance head does not shape the hidden layers
+    dYT = r(gT)
     grad = np.zeros(N_PARAMS, np.float64)
-    Ws, _ = unpack(p)
     Wq = c["Wq"]
     L = N_LAYERS - 1
     grad[W_OFF[L]:W_OFF[L] + OUT_PAD * HID] = (dY.T @ c["X"][L]).ravel()
     grad[B_OFF[L]:B_OFF[L] + OUT_PAD] = dY.sum(0)
-    dA = dY @ Wq[L]
+    dA = dYT @ Wq[L]
     for l in range(L - 1, -1, -1):
         Z = c["Z"][l]
         sg = 1.0 / (1.0 + np.exp(-Z))
@@ -145,10 +156,17 @@ def loss_grad(p, x, t, bf=False):
         if l > 0:
             dA = dZ @ Wq[l]
     losses = np.array([l_reg.sum(), l_bce.sum(), l_hit.sum()])
-    return losses, grad
+    return losses, grad, tgt
 
 
-def mean_loss(p, x, t):
-    """Mean total loss of the FP64 network (for finite differences)."""
-    l, _ = loss_grad(p, x, t, bf=False)
-    return l.sum() / x.shape[0]
+def mean_loss(p, x, t, tgt, with_var=True):
+    """Mean total loss of the FP64 network with the log-variance target
+    fixed (for finite differences); with_var=False drops the log-variance
+    term, whose gradient does not reach the hidden layers."""
+    l, _, _ = loss_grad(p, x, t, bf=False, tgt=tgt)
+    total = l.sum()
+    if not with_var:
+        y, _ = forward(p, x, False)
+        reg = (t["escaped"] > 0.5) & (t["band"] < 0.5)
+        total -= np.where(reg, 0.1 * (y[:, 4] - tgt) ** 2, 0.0).sum()
+    return total / x.shape[0]

@@ -97,7 +97,7 @@ def test_gradients_match_cpu(gpu, net):
     p = net.params()
     # move the log-variance head away from 0 so every loss term is active
     loss, g = net.loss_grad(x, t)
-    lr, gr = mo.loss_grad(p, x, t, bf=True)
+    lr, gr, _ = mo.loss_grad(p, x, t, bf=True)
     say("loss gpu", loss, "cpu", lr)
     assert np.allclose(loss, lr, rtol=2e-3, atol=1e-3)
     for l in range(mo.N_LAYERS):
@@ -162,5 +162,5 @@ def test_training_reduces_loss(gpu):
     first, last = loss[:20].mean(0), loss[-20:].mean(0)
     say("loss first", first, "last", last)
     assert np.isfinite(loss).all()
-    assert last[0] < first[0] - 1.0          # regression NLL (includes log-variance) falls
+    assert last[0] < 0.7 * first[0]          # exit regression + log-variance fit falls
     assert last[1] < 0.8 * first[1]          # disk-hit BCE falls

@@ -1,7 +1,10 @@
 """Pins of the CPU surrogate network (oracle/mlp_oracle.py), CPU only.
 
 * the analytic backward pass equals central finite differences of the FP64
-  loss for parameters of every layer (weights and biases, all output heads);
+  loss (log-variance target held fixed, as in training) for parameters of
+  every layer (weights and biases, all output heads), except that the
+  log-variance head's error signal is kept out of the hidden layers: those
+  gradients are compared with finite differences of the loss without it;
 * the parameter layout matches the library's (bhSurrogateParamCount);
 * bfloat16 rounding is round-to-nearest-even on the float32 bit pattern.
 """
@@ -23,7 +26,7 @@ def random_case(seed, n=24):
         p[mo.B_OFF[l]:mo.B_OFF[l] + mo.layer_out(l)] = rng.normal(0, 0.1, mo.layer_out(l))
     x = rng.uniform(-1, 1, (n, 6)).astype(np.float32)
     t = np.zeros(n, dtype=[("d0", "<f4"), ("d1", "<f4"), ("disk", "<f4"), ("hitR", "<f4"), ("escaped", "<f4"),
-                           ("pad", "<f4", (3,))])
+                           ("band", "<f4"), ("pad", "<f4", (2,))])
     t["d0"] = rng.normal(0, 2e-3, n)
     t["d1"] = rng.normal(0, 2e-3, n)
     t["disk"] = rng.random(n) < 0.4
@@ -55,7 +58,7 @@ def test_bf16_rounding():
 @pytest.mark.parametrize("seed", [0, 1])
 def test_backward_matches_finite_differences(seed):
     p, x, t = random_case(seed)
-    _, g = mo.loss_grad(p, x, t, bf=False)
+    _, g, tgt = mo.loss_grad(p, x, t, bf=False)
     rng = np.random.default_rng(100 + seed)
     idx = []
     for l in range(mo.N_LAYERS):
@@ -68,6 +71,7 @@ def test_backward_matches_finite_differences(seed):
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        fd = (mo.mean_loss(pp, x, t) - mo.mean_loss(pm, x, t)) / (2 * h)
+        head = i >= mo.W_OFF[mo.N_LAYERS - 1]
+        fd = (mo.mean_loss(pp, x, t, tgt, head) - mo.mean_loss(pm, x, t, tgt, head)) / (2 * h)
         scale = max(abs(fd), abs(g[i]), 1e-3 * np.abs(g).max())
         assert abs(fd - g[i]) <= 2e-5 * scale, (i, fd, g[i])
