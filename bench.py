@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-row-step", type=int, default=4)
     ap.add_argument("--math", type=int, default=None, help="0 exact, 1 fast (default: library default)")
     ap.add_argument("--band", type=int, default=None, help="FP64 critical-band pass 0/1 (default: library default)")
+    ap.add_argument("--surrogate-steps", type=int, default=3000,
+                    help="train the learned surrogate this many Adam steps and time MLP+fallback (0: skip)")
+    ap.add_argument("--surrogate-sigma", type=float, default=1e-3, help="router: max predicted exit spread (rad)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -63,6 +66,59 @@ def copy_calibration(mib):
         bt.check(bt.load().bhCopyDevice(dst.ptr, src.ptr, n), "bhCopyDevice")
     bt.synchronize()
     print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
+
+
+def surrogate_leg(bt, lib, frames, steps, sigma, plain_ms):
+    """Train the surrogate (library HIP kernels, 2^22 plain-kernel rays), attach
+    it and time the same frames with MLP + RK4 fallback; exit errors of the
+    accepted rays against the plain FP32 trace of the same frame."""
+    import ctypes as C
+
+    import numpy as np
+
+    t0 = time.perf_counter()
+    s = bt.Surrogate(seed=2)
+    s.dataset(1024, 4096, seed=2)
+    loss = s.train(steps, batch=65536, lr=2e-3)
+    bt.synchronize()
+    train_s = time.perf_counter() - t0
+    P = frames[0]
+    plain = bt.trace_records(P)
+    bt.set_surrogate(s, sigma, 1e-3)
+    try:
+        rec = bt.trace_records(P)
+        cand, acc = bt.last_surrogate_stats()
+        fb = bt.Framebuffer(P.width, P.height)
+        fb.render(P)
+        bt.synchronize()
+        lib.bhEnableKernelTimers(1)
+        tr, ms = C.c_float(), []
+        for f in frames:
+            fb.render(f)
+            lib.bhLastKernelTimes(C.byref(tr), None, None)
+            ms.append(tr.value)
+        lib.bhEnableKernelTimers(0)
+        fb.close()
+    finally:
+        bt.set_surrogate(None)
+        s.close()
+    m = ((rec["flags"] >> 8) == 0) & ((rec["flags"] & 1) == 1)
+    err = None
+    if m.any():
+        a = rec["escDir"][m].astype(np.float64)
+        b = plain["escDir"][m].astype(np.float64)
+        a /= np.linalg.norm(a, axis=1, keepdims=True)
+        b /= np.linalg.norm(b, axis=1, keepdims=True)
+        ang = 2 * np.arcsin(np.clip(np.linalg.norm(a - b, axis=1) / 2, 0, 1))
+        err = {"max": float(ang.max()), "p999": float(np.quantile(ang, 0.999))}
+    trace_ms = sum(ms) / len(ms)
+    return {"train_steps": steps, "train_s": round(train_s, 2), "final_loss": [round(float(x), 5) for x in
+                                                                               loss[-50:].mean(0)],
+            "router_sigma": sigma, "candidates": int(cand), "accepted": int(acc),
+            "fallback_frac": round(1.0 - acc / (P.width * P.height), 5),
+            "trace_ms": round(trace_ms, 3), "speedup_vs_plain": round(plain_ms / trace_ms, 3),
+            "mrays_trace": round(P.width * P.height / trace_ms / 1e3, 3),
+            "exit_err_vs_plain_fp32": err}
 
 
 def cpu_baseline(P, row_step):
@@ -153,6 +209,8 @@ def main():
                       "bloom_tonemap": round(sum(post_ms) / len(post_ms), 3)},
         "roofline": roof,
     }
+    if a.surrogate_steps > 0:
+        res["surrogate"] = surrogate_leg(bt, lib, frames, a.surrogate_steps, a.surrogate_sigma, avg_trace)
     if not a.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(frames[0], a.cpu_row_step)
     print(json.dumps(res), flush=True)

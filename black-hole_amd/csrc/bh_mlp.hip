@@ -167,8 +167,8 @@ hipError_t launchSiluBack(const float* dA, const float* Z, int M, int N, __bf16*
 // disk-hit logit, 3 hit radius / 10, 4 s = log of the expected squared exit
 // error over sigma0^2.  Per row (mean over the M rows of the batch):
 //   all rows:               BCE(disk logit, disk)
-//   escaped, not band:      kRegW |e|^2 for |e| <= kClip (constant beyond:
-//                           outliers do not pull the prediction), and
+//   escaped, not band:      kRegW x Huber(|e|; kClip) (quadratic inside the
+//                           clip radius, linear beyond), and
 //                           0.1 (s - log(min(|e|, kClip)^2 / sigma0^2 + 1e-6))^2
 //                           with the error term held fixed (s learns the
 //                           error it will make)
@@ -201,11 +201,19 @@ __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const floa
             const float e0 = y[0] - t.d0, e1 = y[1] - t.d1;
             const float en2 = e0 * e0 + e1 * e1;
             const float c2 = fminf(en2, kMlpClip * kMlpClip);
-            lReg = kMlpRegW * c2;
             if (en2 <= kMlpClip * kMlpClip)
             {
+                lReg = kMlpRegW * en2;
                 g[0] = 2.f * kMlpRegW * e0 * invM;
                 g[1] = 2.f * kMlpRegW * e1 * invM;
+            }
+            else
+            {
+                // Huber: linear beyond the clip radius
+                const float en = sqrtf(en2);
+                lReg = kMlpRegW * (2.f * kMlpClip * en - kMlpClip * kMlpClip);
+                g[0] = 2.f * kMlpRegW * kMlpClip * e0 / en * invM;
+                g[1] = 2.f * kMlpRegW * kMlpClip * e1 / en * invM;
             }
             const float tgt = __logf(c2 / (kMlpSigma0 * kMlpSigma0) + 1e-6f);
             const float ds = y[4] - tgt;

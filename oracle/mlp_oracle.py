@@ -127,9 +127,12 @@ def loss_grad(p, x, t, bf=False, tgt=None):
     if tgt is None:
         tgt = np.log(c2 / float(SIGMA0) ** 2 + 1e-6)
     ds = y[:, 4] - tgt
-    l_reg = np.where(reg, REG_W * c2 + 0.1 * ds ** 2, 0.0)
-    g[:, 0] = np.where(reg & inside, 2.0 * REG_W * e0 / n, 0.0)
-    g[:, 1] = np.where(reg & inside, 2.0 * REG_W * e1 / n, 0.0)
+    en = np.sqrt(en2)
+    huber = np.where(inside, en2, 2.0 * CLIP * en - CLIP ** 2)
+    l_reg = np.where(reg, REG_W * huber + 0.1 * ds ** 2, 0.0)
+    scale_g = np.where(inside, 1.0, CLIP / np.maximum(en, 1e-300))
+    g[:, 0] = np.where(reg, 2.0 * REG_W * e0 * scale_g / n, 0.0)
+    g[:, 1] = np.where(reg, 2.0 * REG_W * e1 * scale_g / n, 0.0)
     g[:, 4] = np.where(reg, 0.2 * ds / n, 0.0)
     hit = reg & (disk > 0.5)
     eh = y[:, 3] - t["hitR"] * np.float32(0.1)

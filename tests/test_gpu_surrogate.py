@@ -58,7 +58,15 @@ def test_gemm_layouts(gpu, M, N, K, ta, tb, split):
 
 @pytest.fixture(scope="module")
 def net(gpu):
+    """A network with every head at a scale where the loss terms are active
+    (the library starts the output layer near zero)."""
     s = gpu.Surrogate(seed=7)
+    p = s.params()
+    L = mo.N_LAYERS - 1
+    rng = np.random.default_rng(11)
+    W = rng.uniform(-1, 1, (mo.N_OUT, mo.HID)) * np.sqrt(3.0 / mo.HID) * 0.05
+    p[mo.W_OFF[L]:mo.W_OFF[L] + mo.N_OUT * mo.HID] = W.ravel()
+    s.set_params(p)
     yield s
     s.close()
 
