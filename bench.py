@@ -33,6 +33,11 @@ HBM_PEAK_GBS = 8000.0
 # (counted-float instantiation of the oracle; 1 flop per +,-,*,/,sqrt,
 # floor, transcendental).  Keyed by frame index (disk time 0, 0.5, 1).
 FLOPS = {0: 506225668732 + 80870400, 1: 506225457204 + 80870400, 2: 506228100901 + 80870400}
+# HBM bytes per launch of the FP32 tile kernel on WORKLOAD (FP64 band on):
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by a
+# plain-copy calibration (FETCH x2.0, WRITE x1.0); profiles/pmc_trace_kerr09_1080p_high.md
+TRAFFIC_BYTES = (6773 * 2 + 65100) * 1024
+FP32_NONPACKED_TFLOPS = 78.6
 
 
 def parse():
@@ -192,8 +197,10 @@ def main():
         fl = sum(FLOPS[f] for f in per_frame) / len(per_frame)
         achieved = fl / (avg_trace * 1e-3) / 1e12
         roof = {"bound": "valu_fp32", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "traceTileKernel", "flops_per_frame": int(fl), "trace_ms": round(avg_trace, 3)}
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                "traffic": TRAFFIC_BYTES if prec.fp64Band else None,
+                "kernel": "traceTileKernel", "flops_per_frame": int(fl), "trace_ms": round(avg_trace, 3),
+                "frac_of_nonpacked_fp32": round(achieved / FP32_NONPACKED_TFLOPS, 4)}
     res = {
         "metric": METRIC, "value": round(mrays, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",

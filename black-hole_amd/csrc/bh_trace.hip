@@ -227,6 +227,10 @@ template <class Ray>
 __global__ __launch_bounds__(64) void traceBandKernel(BandArgs A)
 {
     const unsigned n = *A.count;
+    // band rays are the longest of the frame and run beside the FP32 tile
+    // kernel: give their wavefronts issue priority so the frame is not
+    // waiting on them
+    __builtin_amdgcn_s_setprio(3);
     const d3 o = dk3(A.P.camPos.x, A.P.camPos.y, A.P.camPos.z);
     const float2 jit = launchJitter(A.P, A.jitter, A.dynIndex);
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
