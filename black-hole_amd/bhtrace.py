@@ -77,6 +77,34 @@ class BhPrecision(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BhViewState(C.Structure):
+    _fields_ = [("azimuth", C.c_float), ("elevation", C.c_float), ("distance", C.c_float),
+                ("oriX", C.c_float), ("oriY", C.c_float), ("oriZ", C.c_float),
+                ("model", C.c_int), ("aStar", C.c_float), ("q", C.c_float), ("allowNaked", C.c_int),
+                ("exposure", C.c_float), ("diskOn", C.c_int), ("bloomOn", C.c_int), ("hotSpotsOn", C.c_int),
+                ("animPaused", C.c_int), ("dPhi", C.c_float), ("maxSteps", C.c_int), ("fovY", C.c_float),
+                ("diskTime", C.c_float), ("sampleIndex", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+KEY_PRESS, KEY_REPEAT = 1, 2
+KEY_F1, KEY_F2, KEY_F3, KEY_F4, KEY_F5 = 0x101, 0x102, 0x103, 0x104, 0x105
+HELD = {"W": 1 << 0, "S": 1 << 1, "A": 1 << 2, "D": 1 << 3, "Q": 1 << 4, "E": 1 << 5, "-": 1 << 6, "=": 1 << 7,
+        "X": 1 << 8, "Y": 1 << 9, "Z": 1 << 10, "SHIFT": 1 << 11}
+
+
+class BhDiffStats(C.Structure):
+    _fields_ = [("pixels", C.c_longlong), ("maxDiff", C.c_int * 3), ("pad", C.c_int),
+                ("hist", C.c_longlong * 8), ("fracWithin1", C.c_double)]
+
+
+class BhAngleStats(C.Structure):
+    _fields_ = [("bothEscaped", C.c_longlong), ("outcomeMismatch", C.c_longlong), ("over1mrad", C.c_longlong),
+                ("median", C.c_double), ("p999", C.c_double), ("max", C.c_double)]
+
+
 MLP_TARGET_DTYPE = np.dtype([("d0", "<f4"), ("d1", "<f4"), ("disk", "<f4"), ("hitR", "<f4"), ("escaped", "<f4"),
                              ("band", "<f4"), ("pad", "<f4", (2,))])
 
@@ -100,6 +128,19 @@ SIGNATURES = [
     ("bhHorizonOuterFP64", C.c_double, [C.c_double, C.c_double, C.c_double]),
     ("bhIscoFP64", C.c_double, [C.c_double, C.c_double, C.c_double]),
     ("bhFillCamera", None, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
+    ("bhFillCameraOriented", None, [_P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                     C.c_float]),
+    ("bhViewStateDefaults", None, [C.POINTER(BhViewState)]),
+    ("bhViewerCreate", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("bhViewerDestroy", None, [_vp]),
+    ("bhViewerGetState", C.c_int, [_vp, C.POINTER(BhViewState)]),
+    ("bhViewerSetState", C.c_int, [_vp, C.POINTER(BhViewState)]),
+    ("bhViewerResize", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    ("bhViewerReset", None, [_vp]),
+    ("bhViewerOrbit", None, [_vp, C.c_float, C.c_float]),
+    ("bhViewerZoom", None, [_vp, C.c_float]),
+    ("bhViewerKey", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("bhViewerFrame", C.c_int, [_vp, C.c_float, C.c_uint, _P]),
     ("bhWorkloadParams", C.c_int, [C.c_char_p, C.c_int, _P]),
     ("bhWorkloadViews", C.c_int, [C.c_char_p, C.c_uint64, C.c_int, _P]),
     ("bhRenderBatch", C.c_int, [C.c_int, _P, _vp, _vp, _vp, _vp, _vp]),
@@ -147,6 +188,11 @@ SIGNATURES = [
     ("bhLastSurrogateStats", C.c_int, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), _vp]),
     ("bhMlpGemmTest", C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
                                 C.c_int, C.c_int, _vp]),
+    ("bhWritePPM", C.c_int, [C.c_char_p, _vp, C.c_int, C.c_int]),
+    ("bhWritePNG", C.c_int, [C.c_char_p, _vp, C.c_int, C.c_int]),
+    ("bhWritePFM", C.c_int, [C.c_char_p, _vp, C.c_int, C.c_int]),
+    ("bhImageDiff", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(BhDiffStats)]),
+    ("bhExitAngleDiff", C.c_int, [_vp, _vp, C.c_int, _vp, C.POINTER(BhAngleStats)]),
     ("bhDeviceAlloc", C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     ("bhDeviceFree", C.c_int, [_vp]),
     ("bhCopyToDevice", C.c_int, [_vp, _vp, C.c_size_t]),
@@ -322,6 +368,25 @@ def render_pipeline(P: RenderParams, accum_init: np.ndarray | None = None, frame
     return rgba, accum
 
 
+def render_sequence(params) -> tuple[np.ndarray, np.ndarray]:
+    """launchRenderPipeline over a sequence of frames of one size sharing one
+    accumulation buffer (initially zero); returns the last frame's
+    (rgba uint8 [H,W,4], accum float32 [H,W,4])."""
+    lib = load()
+    params = list(params)
+    W, H = params[0].width, params[0].height
+    if any((P.width, P.height) != (W, H) for P in params):
+        raise ValueError("all frames of a sequence must have one size")
+    bw, bh = (W + 1) // 2, (H + 1) // 2
+    out, acc = DeviceBuffer(W * H * 4), DeviceBuffer(W * H * 16)
+    bA, bB = DeviceBuffer(bw * bh * 16), DeviceBuffer(bw * bh * 16)
+    acc.fill(0)
+    for P in params:
+        check(lib.launchRenderPipeline(out.ptr, acc.ptr, bA.ptr, bB.ptr, C.byref(P), None), "launchRenderPipeline")
+    synchronize()
+    return out.download(np.uint8).reshape(H, W, 4), acc.download(np.float32).reshape(H, W, 4)
+
+
 def render_kernel(P: RenderParams) -> np.ndarray:
     lib = load()
     W, H = max(P.width, 1), max(P.height, 1)
@@ -339,6 +404,124 @@ def trace_records(P: RenderParams) -> np.ndarray:
     check(lib.bhTraceRecords(buf.ptr, C.byref(P), None), "bhTraceRecords")
     synchronize()
     return buf.download(RECORD_DTYPE).reshape(Q.height, Q.width)
+
+
+# ------------------------------------------------------------------ viewer
+class Viewer:
+    """Caller-side producer of RenderParams (orbit camera, body orientation,
+    model knobs, disk clock, accumulation restarts); wraps bhViewer*."""
+
+    def __init__(self, width: int = 1280, height: int = 720):
+        self.lib = load()
+        h = C.c_void_p()
+        if self.lib.bhViewerCreate(width, height, C.byref(h)) != 0:
+            raise BhError("bhViewerCreate failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.bhViewerDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def state(self) -> BhViewState:
+        s = BhViewState()
+        self.lib.bhViewerGetState(self.h, C.byref(s))
+        return s
+
+    def set_state(self, s: BhViewState) -> None:
+        if self.lib.bhViewerSetState(self.h, C.byref(s)) != 0:
+            raise BhError("bhViewerSetState failed")
+
+    def update(self, **kw) -> None:
+        s = self.state
+        for k, v in kw.items():
+            setattr(s, k, v)
+        self.set_state(s)
+
+    def resize(self, width: int, height: int, swap_rb: bool = False) -> None:
+        if self.lib.bhViewerResize(self.h, width, height, int(swap_rb)) != 0:
+            raise BhError("bhViewerResize failed")
+
+    def reset(self) -> None:
+        self.lib.bhViewerReset(self.h)
+
+    def orbit(self, dx: float, dy: float) -> None:
+        self.lib.bhViewerOrbit(self.h, dx, dy)
+
+    def zoom(self, steps: float) -> None:
+        self.lib.bhViewerZoom(self.h, steps)
+
+    def key(self, key, action: int = KEY_PRESS) -> int:
+        k = ord(key) if isinstance(key, str) else int(key)
+        return self.lib.bhViewerKey(self.h, k, action)
+
+    def frame(self, dt: float, held=()):
+        """Advance by dt with the named held keys; returns (RenderParams, restarted)."""
+        mask = 0
+        for k in held:
+            mask |= HELD[k]
+        P = RenderParams()
+        r = self.lib.bhViewerFrame(self.h, dt, mask, C.byref(P))
+        if r < 0:
+            raise BhError("bhViewerFrame failed")
+        return P, bool(r)
+
+
+# ------------------------------------------------------------ headless output
+def _u8_image(rgba: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(rgba, np.uint8)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError("expected an RGBA8 image of shape (H, W, 4)")
+    return a
+
+
+def write_image(path: str, rgba: np.ndarray) -> None:
+    """Write an (H, W, 4) uint8 frame as PPM (.ppm) or PNG (.png)."""
+    a = _u8_image(rgba)
+    fn = load().bhWritePNG if path.lower().endswith(".png") else load().bhWritePPM
+    if fn(path.encode(), a.ctypes.data, a.shape[1], a.shape[0]) != 0:
+        raise BhError(f"writing {path} failed")
+
+
+def write_pfm(path: str, hdr: np.ndarray) -> None:
+    """Write an (H, W, 4) float32 HDR buffer as PFM (RGB, little-endian)."""
+    a = np.ascontiguousarray(hdr, np.float32)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError("expected a float4 image of shape (H, W, 4)")
+    if load().bhWritePFM(path.encode(), a.ctypes.data, a.shape[1], a.shape[0]) != 0:
+        raise BhError(f"writing {path} failed")
+
+
+def image_diff(a: np.ndarray, b: np.ndarray) -> BhDiffStats:
+    """Per-channel LSB statistics of two RGBA8 frames of one size."""
+    a, b = _u8_image(a), _u8_image(b)
+    if a.shape != b.shape:
+        raise ValueError(f"image sizes differ: {a.shape} vs {b.shape}")
+    st = BhDiffStats()
+    if load().bhImageDiff(a.ctypes.data, b.ctypes.data, a.shape[1], a.shape[0], C.byref(st)) != 0:
+        raise BhError("bhImageDiff failed")
+    return st
+
+
+def exit_angle_diff(a: np.ndarray, b: np.ndarray):
+    """Exit-direction angles (rad, -1 where a ray did not escape in both) of
+    two ray-record arrays, and their BhAngleStats."""
+    a = np.ascontiguousarray(a, RECORD_DTYPE).ravel()
+    b = np.ascontiguousarray(b, RECORD_DTYPE).ravel()
+    if a.shape != b.shape:
+        raise ValueError("record counts differ")
+    ang = np.empty(a.size, np.float32)
+    st = BhAngleStats()
+    if load().bhExitAngleDiff(a.ctypes.data, b.ctypes.data, a.size, ang.ctypes.data, C.byref(st)) != 0:
+        raise BhError("bhExitAngleDiff failed")
+    return ang, st
 
 
 def probe_grid(P: RenderParams, nx: int, ny: int) -> np.ndarray:

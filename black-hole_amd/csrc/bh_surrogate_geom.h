@@ -33,7 +33,7 @@ struct RayFrame
 };
 
 // er, eth, eph: BL basis at the camera (trace.cuh:673-682 form).
-BHK_HD inline void camBasis(v3 o, v3& er, v3& eth, v3& eph, float& r, float& th)
+BHK_HD void camBasis(v3 o, v3& er, v3& eth, v3& eph, float& r, float& th)
 {
     r = sqrtf(o.x * o.x + o.y * o.y + o.z * o.z);
     const float c = fminf(fmaxf(o.y / r, -1.f), 1.f);
@@ -45,14 +45,14 @@ BHK_HD inline void camBasis(v3 o, v3& er, v3& eth, v3& eph, float& r, float& th)
     eph = v3{-sp, 0.f, -cp};
 }
 
-BHK_HD inline v3 cross3(v3 a, v3 b) { return v3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-BHK_HD inline float dot3(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-BHK_HD inline v3 scale3(v3 a, float s) { return v3{a.x * s, a.y * s, a.z * s}; }
-BHK_HD inline v3 add3(v3 a, v3 b) { return v3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-BHK_HD inline v3 unit3(v3 a) { return scale3(a, 1.f / sqrtf(fmaxf(dot3(a, a), 1e-30f))); }
+BHK_HD v3 cross3(v3 a, v3 b) { return v3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+BHK_HD float dot3(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+BHK_HD v3 scale3(v3 a, float s) { return v3{a.x * s, a.y * s, a.z * s}; }
+BHK_HD v3 add3(v3 a, v3 b) { return v3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+BHK_HD v3 unit3(v3 a) { return scale3(a, 1.f / sqrtf(fmaxf(dot3(a, a), 1e-30f))); }
 
 // Frame and screen coordinates of the unit direction d from a camera with basis er, eth, eph.
-BHK_HD inline RayFrame rayFrame(v3 d, v3 er, v3 eth, v3 eph, float rc)
+BHK_HD RayFrame rayFrame(v3 d, v3 er, v3 eth, v3 eph, float rc)
 {
     RayFrame F;
     F.d0 = d;
@@ -67,7 +67,7 @@ BHK_HD inline RayFrame rayFrame(v3 d, v3 er, v3 eth, v3 eph, float rc)
     return F;
 }
 
-BHK_HD inline float weakDeflection(float M, float b, float psi) { return 2.f * M / fmaxf(b, 1e-3f) * (1.f + cosf(psi)); }
+BHK_HD float weakDeflection(float M, float b, float psi) { return 2.f * M / fmaxf(b, 1e-3f) * (1.f + cosf(psi)); }
 
 // Deflection of an ingoing Schwarzschild / Reissner-Nordstrom null ray from
 // a static camera at radius rc (angle psi to the direction of the hole) up
@@ -81,9 +81,9 @@ BHK_HD inline float weakDeflection(float M, float b, float psi) { return 2.f * M
 // 16-point Gauss-Legendre integrates each leg; the exit tangent adds
 // atan(u_e / sqrt(G(u_e))).  FP64.  Returns the first-order weak-field value
 // when no turning point is found.
-BHK_HD inline double sphG(double M, double Q2, double B, double u) { return B - u * u * (1.0 - 2.0 * M * u + Q2 * u * u); }
+BHK_HD double sphG(double M, double Q2, double B, double u) { return B - u * u * (1.0 - 2.0 * M * u + Q2 * u * u); }
 
-BHK_HD inline float sphDeflection(float Mf, float Qf, float rcf, float psif, float escRf)
+BHK_HD float sphDeflection(float Mf, float Qf, float rcf, float psif, float escRf)
 {
     const double t16[16] = {0.0052995325041750307, 0.0277124884633837,   0.067184398806084122, 0.1222977958224985,
                             0.19106187779867811,   0.27099161117138632,  0.35919822461037054,  0.45249374508118129,
@@ -129,7 +129,7 @@ BHK_HD inline float sphDeflection(float Mf, float Qf, float rcf, float psif, flo
 }
 
 // Normalised inputs: b / 40, alpha / pi, a*, q / 0.6, log(r_c / 6) / log(40 / 6), th_c / pi.
-BHK_HD inline void surrogateInputs(const RayFrame& F, float aStar, float q, float rc, float thc, float* x)
+BHK_HD void surrogateInputs(const RayFrame& F, float aStar, float q, float rc, float thc, float* x)
 {
     x[0] = F.b / kSurBMax;
     x[1] = F.alpha * 0.318309886183791f;
@@ -140,7 +140,7 @@ BHK_HD inline void surrogateInputs(const RayFrame& F, float aStar, float q, floa
 }
 
 // Deflection components of an exit direction e in the ray frame.
-BHK_HD inline void exitDeflection(const RayFrame& F, v3 e, float& dA, float& dB)
+BHK_HD void exitDeflection(const RayFrame& F, v3 e, float& dA, float& dB)
 {
     e = unit3(e);
     const float c = dot3(e, F.d0);
@@ -152,7 +152,7 @@ BHK_HD inline void exitDeflection(const RayFrame& F, v3 e, float& dA, float& dB)
 }
 
 // Exit direction from deflection components.
-BHK_HD inline v3 exitFromDeflection(const RayFrame& F, float dA, float dB)
+BHK_HD v3 exitFromDeflection(const RayFrame& F, float dA, float dB)
 {
     const float D = sqrtf(dA * dA + dB * dB);
     if (D <= 0.f) return F.d0;

@@ -86,6 +86,71 @@ double bhIscoFP64(double M, double a, double Q);   /* prograde equatorial ISCO *
 void bhFillCamera(RenderParams* p, float azimuthRad, float elevationRad,
                   float distance, float fovYDegrees);
 
+/* Orbit camera with the black hole's body orientation R = Rz(oriZ) Ry(oriY)
+ * Rx(oriX) about fixed world axes: follows fillCamera and
+ * applyBodyOrientation (main.cpp:147-167, 186-228).  With zero orientation
+ * it equals bhFillCamera. */
+void bhFillCameraOriented(RenderParams* p, float azimuthRad, float elevationRad, float distance,
+                          float fovYDegrees, float oriX, float oriY, float oriZ);
+
+/* ------------------------------------------------------------------------ */
+/* Viewer layer: the caller-side producer of RenderParams                   */
+/* Follows the reference application loop (main.cpp:48-228, 230-337,        */
+/* 418-478): orbit camera and its clamps, body orientation, model knobs,     */
+/* quality presets, disk clock, and the progressive-accumulation rules       */
+/* (restart on any geometry change; accumMode 1 when the disk is still,      */
+/* 2 when it animates; sample index capped at 4096).  Host only.            */
+/* ------------------------------------------------------------------------ */
+typedef struct BhViewState
+{
+    float azimuth, elevation, distance;   /* orbit camera: rad, rad, code units (rs = 1) */
+    float oriX, oriY, oriZ;               /* body orientation (rad) */
+    int   model;                          /* BH_SCHWARZSCHILD .. BH_KERR_NEWMAN */
+    float aStar, q;                       /* knobs; each model uses the ones that apply */
+    int   allowNaked;
+    float exposure;
+    int   diskOn, bloomOn, hotSpotsOn, animPaused;
+    float dPhi;
+    int   maxSteps;
+    float fovY;                           /* degrees (60 in the reference) */
+    float diskTime;                       /* disk clock (advanced by dt unless paused) */
+    int   sampleIndex;                    /* sample index of the next frame */
+} BhViewState;
+
+typedef struct BhViewer BhViewer;
+enum { BH_KEY_PRESS = 1, BH_KEY_REPEAT = 2 };
+/* Keys: printable keys by their ASCII code ('[', ']', ',', '.', 'B', 'N',
+ * 'H', 'R', ' ', '1', '2', '3'); function keys below. */
+enum { BH_KEY_F1 = 0x101, BH_KEY_F2, BH_KEY_F3, BH_KEY_F4, BH_KEY_F5 };
+/* Held-key mask for bhViewerFrame (W/S zoom, A/D azimuth, Q/E elevation,
+ * -/= exposure, X/Y/Z body rotation, SHIFT reverses the rotation). */
+enum
+{
+    BH_HELD_W = 1u << 0, BH_HELD_S = 1u << 1, BH_HELD_A = 1u << 2, BH_HELD_D = 1u << 3,
+    BH_HELD_Q = 1u << 4, BH_HELD_E = 1u << 5, BH_HELD_EXPOSURE_DOWN = 1u << 6, BH_HELD_EXPOSURE_UP = 1u << 7,
+    BH_HELD_X = 1u << 8, BH_HELD_Y = 1u << 9, BH_HELD_Z = 1u << 10, BH_HELD_SHIFT = 1u << 11
+};
+/* Reference defaults (main.cpp:48-88). */
+void bhViewStateDefaults(BhViewState* s);
+int  bhViewerCreate(int width, int height, BhViewer** v);
+void bhViewerDestroy(BhViewer* v);
+int  bhViewerGetState(const BhViewer* v, BhViewState* s);
+/* Replaces the state (clamped); model knob changes are re-derived at the
+ * next frame. */
+int  bhViewerSetState(BhViewer* v, const BhViewState* s);
+int  bhViewerResize(BhViewer* v, int width, int height, int swapRB);
+/* Camera and orientation back to the defaults (key R). */
+void bhViewerReset(BhViewer* v);
+/* Mouse drag (0.005 rad per pixel) and wheel (distance x 0.92^steps). */
+void bhViewerOrbit(BhViewer* v, float dxPixels, float dyPixels);
+void bhViewerZoom(BhViewer* v, float wheelSteps);
+/* One key event; returns 1 if it changed the state. */
+int  bhViewerKey(BhViewer* v, int key, int action);
+/* Advance by dt seconds (clamped to 0.1) with the held keys, and produce the
+ * frame's parameters.  Returns 1 when the accumulation restarts (first
+ * frame or changed geometry), 0 when it continues, -1 on bad arguments. */
+int  bhViewerFrame(BhViewer* v, float dt, unsigned held, RenderParams* out);
+
 /* Named workload presets (camera, model, disk and post settings).  `frame`
  * selects the disk time where a workload has several frames.  Returns 0 on
  * success, -1 for an unknown name.  Names:
@@ -307,6 +372,44 @@ bh_error_t bhLastSurrogateStats(unsigned* candidates, unsigned* accepted, bh_str
  * with bf16 operands (device) and f32 C (device; zero it when splitK > 1). */
 bh_error_t bhMlpGemmTest(const void* A, int lda, int transA, const void* B, int ldb, int transB, float* C, int ldc,
                          int M, int N, int K, int splitK, bh_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Headless output formats and image comparison (host memory only)          */
+/* Replace the reference's presentation path: the Vulkan interop buffer and */
+/* per-swapchain-image copy (vulkan_context.cpp:446-484, 529-588;           */
+/* cuda_interop.cpp:61-74).  Return 0 on success, -1 on a bad argument or   */
+/* an I/O failure.                                                          */
+/* ------------------------------------------------------------------------ */
+/* Binary PPM (P6): 8-bit RGB, alpha dropped, rows top to bottom. */
+int bhWritePPM(const char* path, const bh_uchar4* rgba, int width, int height);
+/* PNG: 8-bit RGBA, filter 0, zlib stream of stored (uncompressed) blocks. */
+int bhWritePNG(const char* path, const bh_uchar4* rgba, int width, int height);
+/* PFM ("PF"): float RGB of the HDR buffer, little-endian (scale -1), rows
+ * bottom to top as the format defines; w dropped. */
+int bhWritePFM(const char* path, const bh_float4* hdr, int width, int height);
+
+typedef struct BhDiffStats
+{
+    long long pixels;
+    int       maxDiff[3];   /* per channel, in LSB */
+    int       pad;
+    long long hist[8];      /* per-pixel max channel diff: 0,1,2,3,4-7,8-15,16-63,64+ */
+    double    fracWithin1;  /* share of pixels whose channels all differ by <= 1 LSB */
+} BhDiffStats;
+/* LSB statistics of two RGBA8 images of the same size (alpha ignored). */
+int bhImageDiff(const bh_uchar4* a, const bh_uchar4* b, int width, int height, BhDiffStats* out);
+
+typedef struct BhAngleStats
+{
+    long long bothEscaped;      /* rays escaped in both records */
+    long long outcomeMismatch;  /* escaped in one record only */
+    long long over1mrad;        /* escaped in both, angle > 1e-3 rad */
+    double    median, p999, max;   /* exit-direction angle (rad) over bothEscaped */
+} BhAngleStats;
+/* Exit-direction angle between two ray-record buffers of n rays (host
+ * copies of bhTraceRecords output).  anglesOut (may be NULL): float[n], the
+ * angle per ray, -1 where the ray did not escape in both. */
+int bhExitAngleDiff(const BhRayRecord* a, const BhRayRecord* b, int n, float* anglesOut, BhAngleStats* out);
 
 /* ------------------------------------------------------------------------ */
 /* Device memory and stream utilities (plumbing for callers without HIP)    */
