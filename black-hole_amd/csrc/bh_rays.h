@@ -177,35 +177,44 @@ BHK_INL KDeriv kerrRHS(const FrameK& K, float E, float Lz, float r, float pr, fl
                        float& Kout)
 {
     if (s < 1e-4f) s = 1e-4f;
+    const float a = K.a, a2 = K.a2;
+    KDeriv d;
+    if constexpr (M::fast)
+    {
+        // The same Hamiltonian right-hand side, reduced algebraically: three
+        // reciprocals (1/sin th, 1/Delta, 1/Sigma), P/Delta formed once, and
+        // W_r = -(2 P P' Delta - P^2 Delta') / Delta^2 = -4 r E (P/Delta) + (P/Delta)^2 Delta'.
+        const float aE = a * E;
+        const float r2 = r * r;
+        const float is = M::rcp(s), is2 = is * is;
+        const float Pp = fmaf(E, r2, E * a2 - a * Lz);
+        float Del = fmaf(r, r - K.twoM, a2 + K.Q2);
+        if (Del < 1e-6f) Del = 1e-6f;
+        const float Sig = fmaf(a2 * cth, cth, r2);
+        const float iDel = M::rcp(Del), invS = M::rcp(Sig);
+        const float PD = Pp * iDel;
+        const float B = fmaf(Lz, is, -aE * s);
+        const float W = fmaf(B, B, -Pp * PD);
+        const float Dpr = Del * pr;
+        const float Kc = fmaf(Dpr, pr, fmaf(pth, pth, W));
+        if (kWant) Kout = Kc;
+        const float dDel = 2.f * r - K.twoM;
+        const float Wr = fmaf(PD * PD, dDel, -4.f * E * r * PD);
+        const float dB = -cth * fmaf(Lz, is2, aE);
+        const float iSK = invS * Kc;
+        d.r = Dpr * invS;
+        d.th = pth * invS;
+        d.ph = fmaf(a, PD, fmaf(Lz, is2, -aE)) * invS;
+        d.pr = invS * fmaf(-0.5f, fmaf(dDel * pr, pr, Wr), r * iSK);
+        d.pth = invS * fmaf(-B, dB, -(a2 * s * cth) * iSK);
+        return d;
+    }
     float s2 = s * s;
-    float r2 = r * r, a = K.a, a2 = K.a2;
+    float r2 = r * r;
     float Sig = r2 + a2 * cth * cth;
     float Del = r2 - K.twoM * r + a2 + K.Q2;
     if (Del < 1e-6f) Del = 1e-6f;
     float Pp = E * (r2 + a2) - a * Lz;
-    KDeriv d;
-    if constexpr (M::fast)
-    {
-        // three reciprocals (1/s, 1/Delta, 1/Sigma) instead of five divisions
-        const float is = M::rcp(s), is2 = is * is;
-        const float iDel = M::rcp(Del), iDel2 = iDel * iDel;
-        const float invS = M::rcp(Sig), invS2 = invS * invS;
-        float B = Lz * is - a * E * s;
-        float W = B * B - Pp * Pp * iDel;
-        float Kc = Del * pr * pr + pth * pth + W;
-        if (kWant) Kout = Kc;
-        float dDel = 2.f * r - K.twoM;
-        float dP = 2.f * r * E;
-        float Wr = -(2.f * Pp * dP * Del - Pp * Pp * dDel) * iDel2;
-        float dB = -Lz * cth * is2 - a * E * cth;
-        float Wth = 2.f * B * dB;
-        d.r = Del * pr * invS;
-        d.th = pth * invS;
-        d.ph = ((Lz * is2 - a * E) + a * Pp * iDel) * invS;
-        d.pr = -0.5f * invS * (dDel * pr * pr + Wr) + r * invS2 * Kc;
-        d.pth = -0.5f * invS * Wth - a2 * s * cth * invS2 * Kc;
-        return d;
-    }
     float B = M::div(Lz, s) - a * E * s;
     float W = B * B - M::div(Pp * Pp, Del);
     float Kc = Del * pr * pr + pth * pth + W;

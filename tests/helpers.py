@@ -84,3 +84,38 @@ def angle_between(a, b):
     b = b / np.linalg.norm(b, axis=1, keepdims=True)
     c = np.sqrt(((a - b) ** 2).sum(-1))
     return 2.0 * np.arcsin(np.clip(c / 2.0, 0.0, 1.0))
+
+
+def pixel_dirs(P, xs, ys, jx=0.5, jy=0.5):
+    """Pinhole directions of pixels (xs, ys) in float64 (cameraRayDir,
+    trace.cuh:431-438)."""
+    F = np.array([P.camForward.x, P.camForward.y, P.camForward.z])
+    R = np.array([P.camRight.x, P.camRight.y, P.camRight.z])
+    U = np.array([P.camUp.x, P.camUp.y, P.camUp.z])
+    fx = (np.asarray(xs, np.float64) + jx) / P.width
+    fy = (np.asarray(ys, np.float64) + jy) / P.height
+    d = F + R * ((fx * 2 - 1) * P.tanHalfFov * P.aspect)[:, None] + U * ((1 - fy * 2) * P.tanHalfFov)[:, None]
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def ill_conditioned(oracle, P, pix, eps=1e-6, amp=1e-3):
+    """Rays (flat pixel indices) whose FP64 trace is ill-conditioned: moving
+    the initial direction by eps rad along the camera's right or up axis
+    changes the outcome or turns the exit direction by more than amp rad (an
+    amplification of amp / eps).  For such rays two float traces that differ
+    in the last bit of any intermediate cannot be expected to agree."""
+    pix = np.asarray(pix, np.int64)
+    if pix.size == 0:
+        return np.zeros(0, bool)
+    ys, xs = np.divmod(pix, P.width)
+    d = pixel_dirs(P, xs, ys)
+    base = oracle.probe_dirs(P, d.astype(np.float32), mode=1)
+    ill = np.zeros(pix.size, bool)
+    for ax in ((P.camRight.x, P.camRight.y, P.camRight.z), (P.camUp.x, P.camUp.y, P.camUp.z)):
+        dp = d + eps * np.asarray(ax, np.float64)
+        dp /= np.linalg.norm(dp, axis=1, keepdims=True)
+        o = oracle.probe_dirs(P, dp.astype(np.float32), mode=1)
+        ill |= o[:, 0] != base[:, 0]
+        esc = (o[:, 0] == 1) & (base[:, 0] == 1)
+        ill[esc] |= angle_between(o[esc, 1:4], base[esc, 1:4]) > amp
+    return ill
