@@ -38,6 +38,7 @@ struct BandArgs
     unsigned char* mask;    // [W*H]
     unsigned* list;         // [W*H] band pixel indices
     unsigned* count;        // list length (zero before classification)
+    unsigned* queue;        // band-trace queue head (zero before the band trace)
     float2 jitter;
     const int* dynIndex;    // device sample index overriding P.sampleIndex (or null)
     // learned surrogate: escaping non-band rays inside the training domain

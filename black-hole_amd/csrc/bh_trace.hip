@@ -220,9 +220,16 @@ __global__ __launch_bounds__(256) void traceBatchKernel(const ViewDesc* __restri
 }
 
 // ---------------------------------------------------------------- FP64 band rays
-// One band ray per thread over the classification list (grid-stride); the
-// geometry is double precision, the disk chords are sampled in float with
-// the object's policy, exactly as for the FP32 rays.
+// Band rays come from the classification list through a queue, one ray per
+// lane.  Band rays differ widely in length, so a lane that finishes takes
+// the next ray instead of idling until the wavefront's longest ray ends:
+// every kBandChunk steps the wavefront refills once at least kBandRefill of
+// its lanes are idle (one queue atomic for all of them).  The geometry is
+// double precision; the disk chords are sampled in float with the object's
+// policy, exactly as for the FP32 rays.
+constexpr int kBandChunk = 32;
+constexpr int kBandRefill = 16;
+
 template <class Ray>
 __global__ __launch_bounds__(64) void traceBandKernel(BandArgs A)
 {
@@ -231,25 +238,65 @@ __global__ __launch_bounds__(64) void traceBandKernel(BandArgs A)
     // kernel: give their wavefronts issue priority so the frame is not
     // waiting on them
     __builtin_amdgcn_s_setprio(3);
+    const unsigned lane = __lane_id();
     const d3 o = dk3(A.P.camPos.x, A.P.camPos.y, A.P.camPos.z);
     const float2 jit = launchJitter(A.P, A.jitter, A.dynIndex);
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    Ray R;
+    Chord ch;
+    ch.n = 0;
+    bool active = false, drained = false;
+    unsigned pix = 0;
+    for (;;)
     {
-        const unsigned pix = A.list[i];
-        const int px = (int)(pix % (unsigned)A.P.width), py = (int)(pix / (unsigned)A.P.width);
-        Ray R;
-        R.init(A.B, o, cameraDir64(A.P, px, py, jit));
-        Chord ch;
-        bool done = (R.status != RAY_RUNNING);
-        while (!done)
+        const bool need = !active && !drained;
+        const unsigned long long needMask = __ballot(need);
+        if (needMask)
         {
-            done = R.advance(A.B, A.D, ch);
-            if (ch.n > 0) diskChordSerial<Pol>(A.D, R.diskE(), R.diskLz(), ch, R.acc, R.trans);
+            const int leader = __ffsll((long long)needMask) - 1;
+            unsigned base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.queue, (unsigned)__popcll(needMask));
+            base = __shfl(base, leader);
+            if (need)
+            {
+                const unsigned id = base + (unsigned)__popcll(needMask & ((1ull << lane) - 1ull));
+                if (id >= n) drained = true;
+                else
+                {
+                    pix = A.list[id];
+                    const int px = (int)(pix % (unsigned)A.P.width), py = (int)(pix / (unsigned)A.P.width);
+                    R.init(A.B, o, cameraDir64(A.P, px, py, jit));
+                    active = true;
+                }
+            }
         }
-        float4* out = reinterpret_cast<float4*>(A.rec + pix);
-        const int flags = (R.status == RAY_ESCAPED ? 1 : 2) | (R.step << 8);
-        out[0] = make_float4((float)R.escDir.x, (float)R.escDir.y, (float)R.escDir.z, __int_as_float(flags));
-        out[1] = make_float4(R.acc.x, R.acc.y, R.acc.z, R.trans);
+        if (__ballot(active) == 0ull)
+        {
+            if (__ballot(!drained) == 0ull) break;
+            continue;
+        }
+        for (int k = 0; k < kBandChunk; ++k)
+        {
+            if (active)
+            {
+                bool done = (R.status != RAY_RUNNING);
+                if (!done)
+                {
+                    done = R.advance(A.B, A.D, ch);
+                    if (ch.n > 0) diskChordSerial<Pol>(A.D, R.diskE(), R.diskLz(), ch, R.acc, R.trans);
+                }
+                if (done)
+                {
+                    float4* out = reinterpret_cast<float4*>(A.rec + pix);
+                    const int flags = (R.status == RAY_ESCAPED ? 1 : 2) | (R.step << 8);
+                    out[0] = make_float4((float)R.escDir.x, (float)R.escDir.y, (float)R.escDir.z,
+                                         __int_as_float(flags));
+                    out[1] = make_float4(R.acc.x, R.acc.y, R.acc.z, R.trans);
+                    active = false;
+                }
+            }
+            const unsigned long long idle = __ballot(!active && !drained);
+            if (__popcll(idle) >= kBandRefill || __ballot(active) == 0ull) break;
+        }
     }
 }
 

@@ -121,6 +121,8 @@ BhPrecision g_prec = {1, 3e-2f, 1e-3f};
 constexpr int kBandCountSlot = 16;            // counter[16]: band list length (own 64-byte line)
 constexpr int kCandCountSlot = 32;            // counter[32]: surrogate candidates
 constexpr int kAcceptSlot = 48;               // counter[48]: rays answered by the surrogate
+constexpr int kBandQueueSlot = 64;            // counter[64]: band-trace queue head
+constexpr size_t kCounterBytes = 512;
 // learned surrogate attached to the render path (null: off)
 struct SurrogateRoute
 {
@@ -154,9 +156,9 @@ hipError_t getScratch(hipStream_t st, size_t nRec, Scratch** out)
     Scratch& S = g_scratch[{dev, st}];
     if (!S.counter)
     {
-        e = hipMalloc(&S.counter, 256);
+        e = hipMalloc(&S.counter, kCounterBytes);
         if (e != hipSuccess) { S.counter = nullptr; return e; }
-        e = hipMemset(S.counter, 0, 256);
+        e = hipMemset(S.counter, 0, kCounterBytes);
         if (e != hipSuccess) return e;
     }
     if (nRec > S.recCap)
@@ -296,6 +298,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.mask = Sc->bandMask;
         BA.list = Sc->bandList;
         BA.count = counter + kBandCountSlot;
+        BA.queue = counter + kBandQueueSlot;
         BA.jitter = jitterOf(P.accumMode, P.sampleIndex);
         BA.dynIndex = dynIndex;
         BA.candInputs = nullptr;
@@ -311,6 +314,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             BA.candList = Sc->candList;
         }
         e = bhk::launchResetCounter(BA.count, st);
+        if (e == hipSuccess) e = bhk::launchResetCounter(BA.queue, st);
         if (e == hipSuccess) e = bhk::launchResetCounter(counter + kCandCountSlot, st);
         if (e == hipSuccess) e = bhk::launchResetCounter(counter + kAcceptSlot, st);
         if (e == hipSuccess) e = bhk::launchBandClassify(BA, st);
