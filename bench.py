@@ -10,8 +10,12 @@ t = 0, 0.5, 1.  Inputs are device resident; timing brackets exactly
 Prints one JSON line.  `roofline` is the trace kernel's achieved fraction of
 the FP32 vector peak: exact algorithmic flops of each frame (counted-float
 oracle, see FLOPS below) over the trace kernel's HIP-event time on the
-launch stream.  `cpu_baseline` is the CPU oracle (same algorithm, g++ -O3,
-OpenMP) on a bounded sample of the same frame.
+launch stream.  `post_roofline` is bloom + tonemap against the HBM peak.
+`fp64_error` is the exit-direction error of every pixel of each frame
+against a full-frame FP64 trace.  `surrogate` trains the MLP on 2^24 rays
+with the library's HIP kernels and times MLP + RK4 fallback on the same
+frames.  `cpu_baseline` is the CPU oracle (same algorithm, g++ -O3, OpenMP)
+on a bounded sample of the same frame.
 """
 import argparse
 import json
@@ -46,6 +50,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=9)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-error", action="store_true",
+                    help="skip the per-frame exit error against a full-frame FP64 trace")
     ap.add_argument("--cpu-row-step", type=int, default=4)
     ap.add_argument("--math", type=int, default=None, help="0 exact, 1 fast (default: library default)")
     ap.add_argument("--band", type=int, default=None, help="FP64 critical-band pass 0/1 (default: library default)")
@@ -74,7 +80,7 @@ def copy_calibration(mib):
 
 
 def surrogate_leg(bt, lib, frames, steps, sigma, plain_ms):
-    """Train the surrogate (library HIP kernels, 2^22 plain-kernel rays), attach
+    """Train the surrogate (library HIP kernels, 2^24 plain-kernel rays), attach
     it and time the same frames with MLP + RK4 fallback; exit errors of the
     accepted rays against the plain FP32 trace of the same frame."""
     import ctypes as C
@@ -83,7 +89,7 @@ def surrogate_leg(bt, lib, frames, steps, sigma, plain_ms):
 
     t0 = time.perf_counter()
     s = bt.Surrogate(seed=2)
-    s.dataset(1024, 4096, seed=2)
+    s.dataset(4096, 4096, seed=2)
     loss = s.train(steps, batch=65536, lr=2e-3)
     bt.synchronize()
     train_s = time.perf_counter() - t0
@@ -124,6 +130,45 @@ def surrogate_leg(bt, lib, frames, steps, sigma, plain_ms):
             "trace_ms": round(trace_ms, 3), "speedup_vs_plain": round(plain_ms / trace_ms, 3),
             "mrays_trace": round(P.width * P.height / trace_ms / 1e3, 3),
             "exit_err_vs_plain_fp32": err}
+
+
+def fp64_error_leg(bt, frames):
+    """Exit-direction error of the product path (the current precision and
+    schedule) on every pixel of each frame, against the library's own FP64
+    trace of the whole frame (every pixel in the FP64 band, exact math; that
+    trace is checked against the oracle's FP64 trace in
+    tests/test_gpu_fp64_band.py)."""
+    import numpy as np
+
+    p0, s0 = bt.get_precision(), bt.get_schedule()
+    out = []
+    for k, P in enumerate(frames):
+        prod = bt.trace_records(P)
+        try:
+            bt.set_precision(band=1, crit=1e30, polar=1e30)
+            bt.set_schedule(math=0)
+            ref = bt.trace_records(P)
+        finally:
+            bt.restore_precision(p0)
+            bt.restore_schedule(s0)
+        _, st = bt.exit_angle_diff(prod, ref)
+        out.append({"disk_time": P.diskTime, "rays": int(P.width * P.height), "escaped_both": st.bothEscaped,
+                    "outcome_mismatch": st.outcomeMismatch, "max_rad": float("%.3e" % st.max),
+                    "p999_rad": float("%.3e" % st.p999), "over_1e-3": st.over1mrad})
+    return out
+
+
+def post_roofline(W, H, post_ms):
+    """Bloom + tonemap: algorithmic HBM bytes per frame over their event time.
+    Down-sample/horizontal blur reads the float4 accumulation and writes the
+    half-resolution image; the vertical blur/composite reads both, writes the
+    uchar4 frame and the blurred half-resolution image."""
+    half = ((W + 1) // 2) * ((H + 1) // 2)
+    nbytes = 16 * W * H + 16 * half + 16 * W * H + 16 * half + 4 * W * H + 16 * half
+    gbs = nbytes / (post_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_frame": nbytes, "ms": round(post_ms, 4),
+            "kernels": ["bloomDownBlurHKernel", "blurVCompositeKernel"]}
 
 
 def cpu_baseline(P, row_step):
@@ -215,7 +260,10 @@ def main():
         "kernel_ms": {"trace": round(avg_trace, 3), "shade": round(sum(shade_ms) / len(shade_ms), 3),
                       "bloom_tonemap": round(sum(post_ms) / len(post_ms), 3)},
         "roofline": roof,
+        "post_roofline": post_roofline(W, H, sum(post_ms) / len(post_ms)),
     }
+    if not a.no_fp64_error:
+        res["fp64_error"] = fp64_error_leg(bt, frames)
     if a.surrogate_steps > 0:
         res["surrogate"] = surrogate_leg(bt, lib, frames, a.surrogate_steps, a.surrogate_sigma, avg_trace)
     if not a.no_cpu_baseline:

@@ -171,6 +171,42 @@ template <class M> struct SphRay
 // ========================================================================= Kerr-Newman
 struct KDeriv { float r, th, ph, pr, pth; };
 
+// Fast-policy right-hand side that also returns P, P/Delta, 1/Sigma and the
+// unclamped Delta of the state (reused by the slow-light integral).
+template <class M, bool kWant>
+BHK_INL KDeriv kerrRHSFastX(const FrameK& K, float E, float Lz, float r, float pr, float pth, float s, float cth,
+                            float& Kout, float& PpO, float& PDO, float& invSO, float& DelO)
+{
+    if (s < 1e-4f) s = 1e-4f;
+    const float a = K.a, a2 = K.a2;
+    KDeriv d;
+    const float aE = a * E;
+    const float r2 = r * r;
+    const float is = M::rcp(s), is2 = is * is;
+    const float Pp = fmaf(E, r2, E * a2 - a * Lz);
+    const float DelRaw = fmaf(r, r - K.twoM, a2 + K.Q2);
+    const float Del = DelRaw < 1e-6f ? 1e-6f : DelRaw;
+    const float Sig = fmaf(a2 * cth, cth, r2);
+    const float iDel = M::rcp(Del), invS = M::rcp(Sig);
+    const float PD = Pp * iDel;
+    const float B = fmaf(Lz, is, -aE * s);
+    const float W = fmaf(B, B, -Pp * PD);
+    const float Dpr = Del * pr;
+    const float Kc = fmaf(Dpr, pr, fmaf(pth, pth, W));
+    if (kWant) Kout = Kc;
+    const float dDel = 2.f * r - K.twoM;
+    const float Wr = fmaf(PD * PD, dDel, -4.f * E * r * PD);
+    const float dB = -cth * fmaf(Lz, is2, aE);
+    const float iSK = invS * Kc;
+    d.r = Dpr * invS;
+    d.th = pth * invS;
+    d.ph = fmaf(a, PD, fmaf(Lz, is2, -aE)) * invS;
+    d.pr = invS * fmaf(-0.5f, fmaf(dDel * pr, pr, Wr), r * iSK);
+    d.pth = invS * fmaf(-B, dB, -(a2 * s * cth) * iSK);
+    PpO = Pp; PDO = PD; invSO = invS; DelO = DelRaw;
+    return d;
+}
+
 // Hamiltonian right-hand side (trace.cuh:620-659); s, c = sin/cos of th.
 template <class M, bool kWant>
 BHK_INL KDeriv kerrRHS(const FrameK& K, float E, float Lz, float r, float pr, float pth, float s, float cth,
@@ -242,12 +278,16 @@ template <class M, bool kWant> struct KerrRay
     float dLr, dLth, dLph;        // last RK4 stage derivative (budget exit)
     v3 prevPos, dir3, acc, escDir;
     int step, status, haveDeriv;
+    KDeriv nx;                    // fast policy: right-hand side at the current state
+    float nxK;
+    int haveNext;
 
     BHK_INL void init(const FrameK& K, v3 origin, v3 dir)
     {
         status = RAY_RUNNING;
         step = 0;
         haveDeriv = 0;
+        haveNext = 0;
         escDir = dir;
         acc = mk3(0.f, 0.f, 0.f);
         trans = 1.0f;
@@ -313,7 +353,9 @@ template <class M, bool kWant> struct KerrRay
             return true;
         }
         float Kv = 0.f;
-        KDeriv d1 = kerrRHS<M, kWant>(K, E, Lz, r, pr, pth, sth, cth, Kv);
+        KDeriv d1;
+        if (M::fast && haveNext) { d1 = nx; Kv = nxK; }
+        else d1 = kerrRHS<M, kWant>(K, E, Lz, r, pr, pth, sth, cth, Kv);
         if (!isfinite(r) || !isfinite(pr) || !isfinite(pth) || !isfinite(d1.r) || !isfinite(d1.pr))
         { status = RAY_CAPTURED; return true; }
         if (r <= K.capR) { status = RAY_CAPTURED; return true; }
@@ -367,6 +409,21 @@ template <class M, bool kWant> struct KerrRay
         haveDeriv = 1;
         sth = M::sin_(th);
         cth = M::cos_(th);
+        if constexpr (M::fast)
+        {
+            // right-hand side at the new state (the next step's first stage);
+            // slow light from its P/Delta and 1/Sigma (trace.cuh:840-851;
+            // Delta is clamped at 1e-5 there)
+            float Pp, PD, iS, DelRaw;
+            nx = kerrRHSFastX<M, kWant>(K, E, Lz, r, pr, pth, sth, cth, nxK, Pp, PD, iS, DelRaw);
+            haveNext = 1;
+            const float s = sth < 1e-4f ? 1e-4f : sth;
+            const float a = K.a;
+            const float PDs = DelRaw < 1e-5f ? Pp * 1e5f : PD;
+            const float dtdl = fmaf(a, Lz - a * E * s * s, (r * r + K.a2) * PDs) * iS;
+            tFlight += h * fabsf(dtdl);
+        }
+        else
         {
             // slow light (trace.cuh:840-851)
             float s = sth < 1e-4f ? 1e-4f : sth;

@@ -99,3 +99,26 @@ def test_band_pipeline_pixels_vs_oracle(gpu, oracle, band_on, name, frame, w, h)
     st = lsb_stats(rgba, o_rgba)
     say(name, frame, "band pipeline vs oracle", st)
     assert st["le1"] >= 0.999
+
+
+@pytest.mark.parametrize("name,frame,w,h", [("kerr09_1080p_high", 0, 320, 180), ("kn_m07_q04_2160p_high", 0, 256, 144)])
+def test_full_frame_fp64_trace_matches_fp64_oracle(gpu, oracle, name, frame, w, h):
+    """With every pixel in the band and exact math the library traces the
+    whole frame in FP64 (the per-frame reference bench.py reports the
+    product's exit error against); it agrees with the oracle's FP64 trace."""
+    old_p, old_s = gpu.get_precision(), gpu.get_schedule()
+    try:
+        gpu.set_precision(band=1, crit=1e30, polar=1e30)
+        gpu.set_schedule(math=0)
+        P = scaled_workload(gpu, name, frame, w, h)
+        ang, mism, _ = exit_error(gpu, oracle, P)
+        n, _ = gpu.last_band_stats()
+    finally:
+        gpu.restore_precision(old_p)
+        gpu.restore_schedule(old_s)
+    q = np.quantile(ang, [0.5, 0.999])
+    say(name, frame, "full-frame FP64: band rays", n, "outcome mismatch", mism,
+        "p50 %.2e p99.9 %.2e max %.2e" % (q[0], q[1], ang.max()))
+    assert n == w * h
+    assert mism <= 2
+    assert q[1] <= 1e-6 and ang.max() <= 1e-4
