@@ -434,6 +434,16 @@ template <class M> BHK_INL int chordPrepare(const DiskParams& D, v3 a, v3 b, flo
     float rcMin = fminf(rcA, rcB), rcMax = fmaxf(rcA, rcB);
     if (rcMin - segLen > D.rout) return 0;
     if (rcMax + segLen < D.rin) return 0;
+    // Scale-height cull (not in the reference; it only skips chords whose
+    // sub-samples all fail the |dz| <= 2.6 gate of diskSubSample, so the
+    // result is unchanged).  With a and b on one side of the midplane every
+    // point of the chord has |y| >= min(|a.y|, |b.y|); its cylindrical radius
+    // is at most rcMax (a norm of an affine function is convex), and inside
+    // [rin, rout] the scale height is H = (0.040 + 0.015 rc / rout) rc <=
+    // 0.055 rc, floored at 1e-4.  The 2% margin covers float rounding.
+    if ((a.y > 0.f) == (b.y > 0.f) &&
+        fminf(fabsf(a.y), fabsf(b.y)) > 1.02f * 2.6f * fmaxf(0.055f * rcMax, 1e-4f))
+        return 0;
     int nSub = 1 + (int)(segLen * 6.5f);
     if (nSub > 12) nSub = 12;
     c.a = a;
