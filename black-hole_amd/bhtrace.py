@@ -179,6 +179,7 @@ SIGNATURES = [
     ("bhSurrogateGetParams", C.c_int, [_vp, _vp, C.c_int]),
     ("bhSurrogateConfigs", C.c_int, [C.c_uint64, C.c_int, _P]),
     ("bhSurrogateDataset", C.c_int, [_vp, C.c_int, C.c_int, C.c_uint64, _vp]),
+    ("bhSurrogateDatasetFor", C.c_int, [_vp, _P, C.c_int, C.c_int, C.c_uint64, _vp]),
     ("bhSurrogateDatasetView", C.c_int, [_vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                          C.POINTER(C.c_int)]),
     ("bhSurrogateTrain", C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _vp]),
@@ -645,6 +646,11 @@ class Surrogate:
 
     def dataset(self, n_cfg: int, rays_per_cfg: int, seed: int = 2) -> None:
         check(load().bhSurrogateDataset(self.ptr, n_cfg, rays_per_cfg, seed, None), "bhSurrogateDataset")
+
+    def dataset_for(self, views, rays_per_view: int, seed: int = 2) -> None:
+        """Training set from the cameras of the given RenderParams views."""
+        arr = (RenderParams * len(views))(*views)
+        check(load().bhSurrogateDatasetFor(self.ptr, arr, len(views), rays_per_view, seed, None), "dataset_for")
 
     def dataset_view(self):
         """(inputs [n,6], targets [n] MLP_TARGET_DTYPE, escDir [n,3]) copied to host."""

@@ -285,13 +285,14 @@ bh_error_t bhSurrogateGetParams(BhSurrogate* s, float* host, int count)
     return e;
 }
 
-bh_error_t bhSurrogateDataset(BhSurrogate* s, int nCfg, int raysPerCfg, uint64_t seed, bh_stream_t stream)
+namespace {
+
+// Training set of raysPerCfg rays from the camera of each configuration.
+hipError_t buildDataset(BhSurrogate* s, const std::vector<RenderParams>& P, int raysPerCfg, uint64_t seed,
+                        hipStream_t st)
 {
-    if (!s || nCfg < 1 || raysPerCfg < 64 || raysPerCfg % 64) return hipErrorInvalidValue;
-    hipStream_t st = (hipStream_t)stream;
-    std::vector<RenderParams> P((size_t)nCfg);
+    const int nCfg = (int)P.size();
     std::vector<bhk::FrameK> K((size_t)nCfg);
-    bhSurrogateConfigs(seed, nCfg, P.data());
     std::vector<bhk::BandK> Bk((size_t)nCfg);
     BhPrecision prec;
     bhGetPrecision(&prec);
@@ -339,6 +340,25 @@ bh_error_t bhSurrogateDataset(BhSurrogate* s, int nCfg, int raysPerCfg, uint64_t
     (void)hipFree(dB);
     if (e == hipSuccess) s->setN = (int)n;
     return e;
+}
+
+}  // namespace
+
+bh_error_t bhSurrogateDataset(BhSurrogate* s, int nCfg, int raysPerCfg, uint64_t seed, bh_stream_t stream)
+{
+    if (!s || nCfg < 1 || raysPerCfg < 64 || raysPerCfg % 64) return hipErrorInvalidValue;
+    std::vector<RenderParams> P((size_t)nCfg);
+    bhSurrogateConfigs(seed, nCfg, P.data());
+    return buildDataset(s, P, raysPerCfg, seed, (hipStream_t)stream);
+}
+
+bh_error_t bhSurrogateDatasetFor(BhSurrogate* s, const RenderParams* views, int nViews, int raysPerView,
+                                 uint64_t seed, bh_stream_t stream)
+{
+    if (!s || !views || nViews < 1 || raysPerView < 64 || raysPerView % 64) return hipErrorInvalidValue;
+    std::vector<RenderParams> P(views, views + nViews);
+    for (auto& p : P) bhSanitizeRenderParams(&p);
+    return buildDataset(s, P, raysPerView, seed, (hipStream_t)stream);
 }
 
 bh_error_t bhSurrogateDatasetView(BhSurrogate* s, float** inputs, void** targets, float** escDir, int* n)

@@ -344,6 +344,11 @@ int        bhSurrogateConfigs(uint64_t seed, int n, RenderParams* out);
 /* Traces nCfg x raysPerCfg rays (raysPerCfg a multiple of 64) with the plain
  * FP32 kernel into the training set (replacing any previous one). */
 bh_error_t bhSurrogateDataset(BhSurrogate* s, int nCfg, int raysPerCfg, uint64_t seed, bh_stream_t stream);
+/* Training set from the cameras of given views (for adapting the network to
+ * a steady view): raysPerView rays (a multiple of 64) from each view's
+ * camera position, psi up to 65 degrees, traced like bhSurrogateDataset. */
+bh_error_t bhSurrogateDatasetFor(BhSurrogate* s, const RenderParams* views, int nViews, int raysPerView,
+                                 uint64_t seed, bh_stream_t stream);
 /* Device pointers of the training set: inputs float[n*6], targets
  * BhMlpTarget[n], plain-trace exit directions float[n*3]. */
 bh_error_t bhSurrogateDatasetView(BhSurrogate* s, float** inputs, void** targets, float** escDir, int* n);
