@@ -27,6 +27,14 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
         const bool ok = bandStats(A.B, o, cameraDir64(A.P, px, py, jit), delta, polar);
         band = !ok || (!(A.B.kerr == 0 && delta == 1.0) && (fabs(delta) < A.B.critBand || polar < A.B.polarBand));
         A.mask[i] = band ? 1 : 0;
+        if (A.tileKey && !band)
+        {
+            // cost key of the pixel's tile: rays near the critical curve or a
+            // polar turning point wind the longest; non-negative floats order
+            // like their bit patterns
+            const float k = (float)fmin(fabs(delta), 10.0 * polar);
+            atomicMin(A.tileKey + (py >> 3) * A.tilesX + (px >> 3), __float_as_uint(k));
+        }
         if (A.candList && !band && delta < -A.B.critBand)
         {
             // candidate of the surrogate: escaping ingoing ray inside the training domain
@@ -65,6 +73,78 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
             for (int k = 0; k < kMlpInputs; ++k) A.candInputs[(size_t)slot * kMlpInputs + k] = x[k];
         }
     }
+}
+
+// ---------------------------------------------------------------- tile order
+// The FP32 tile tracer takes tiles from a queue; a tile costs as many RK4
+// steps as its longest ray.  Taking the costliest tiles first keeps the last
+// tiles of the frame short, so the persistent grid drains evenly.  Tiles are
+// counting-sorted into 256 buckets of their key (1/16-octave steps of
+// log2 key, smallest key first); order within a bucket is arbitrary, which
+// changes no result (tiles are independent).
+constexpr int kTileBuckets = 256;
+
+BHK_INL int tileBucket(unsigned keyBits)
+{
+    const float k = __uint_as_float(keyBits);
+    if (!(k < 3.0e38f)) return kTileBuckets - 1;        // no FP32 ray in the tile (or no key)
+    const float b = (log2f(fmaxf(k, 1e-30f)) + 8.f) * 16.f;
+    return b <= 0.f ? 0 : (b >= (float)(kTileBuckets - 2) ? kTileBuckets - 2 : (int)b);
+}
+
+__global__ void tileKeyFillKernel(unsigned* tileKey, int n, unsigned* hist)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tileKey[i] = 0xFFFFFFFFu;
+    if (i < 2 * kTileBuckets) hist[i] = 0u;
+}
+
+__global__ void tileHistKernel(const unsigned* tileKey, int n, unsigned* hist)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(hist + tileBucket(tileKey[i]), 1u);
+}
+
+// one workgroup of 256: exclusive scan of the bucket counts into the
+// scatter cursors hist[256..511]
+__global__ __launch_bounds__(kTileBuckets) void tileScanKernel(unsigned* hist)
+{
+    __shared__ unsigned s[kTileBuckets];
+    const int t = threadIdx.x;
+    const unsigned v = hist[t];
+    s[t] = v;
+    __syncthreads();
+    for (int d = 1; d < kTileBuckets; d <<= 1)
+    {
+        const unsigned add = t >= d ? s[t - d] : 0u;
+        __syncthreads();
+        s[t] += add;
+        __syncthreads();
+    }
+    hist[kTileBuckets + t] = s[t] - v;
+}
+
+__global__ void tileScatterKernel(const unsigned* tileKey, int n, unsigned* hist, unsigned* order)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) order[atomicAdd(hist + kTileBuckets + tileBucket(tileKey[i]), 1u)] = (unsigned)i;
+}
+
+hipError_t launchTileKeyFill(unsigned* tileKey, int nTiles, unsigned* hist, hipStream_t st)
+{
+    const int n = nTiles > 2 * kTileBuckets ? nTiles : 2 * kTileBuckets;
+    hipLaunchKernelGGL(tileKeyFillKernel, dim3((n + 255) / 256), dim3(256), 0, st, tileKey, nTiles, hist);
+    return hipGetLastError();
+}
+
+hipError_t launchTileOrder(const unsigned* tileKey, int nTiles, unsigned* hist, unsigned* order, hipStream_t st)
+{
+    if (nTiles <= 0) return hipSuccess;
+    const dim3 g((nTiles + 255) / 256), b(256);
+    hipLaunchKernelGGL(tileHistKernel, g, b, 0, st, tileKey, nTiles, hist);
+    hipLaunchKernelGGL(tileScanKernel, dim3(1), dim3(kTileBuckets), 0, st, hist);
+    hipLaunchKernelGGL(tileScatterKernel, g, b, 0, st, tileKey, nTiles, hist, order);
+    return hipGetLastError();
 }
 
 hipError_t launchBandClassify(const BandArgs& A, hipStream_t st)

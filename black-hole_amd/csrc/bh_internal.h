@@ -25,6 +25,7 @@ struct TraceArgs
     float2 jitter;          // sub-pixel sample position
     const unsigned char* bandMask;  // [W*H] 1: traced by the FP64 band kernel (or null)
     const int* dynIndex;    // device sample index overriding P.sampleIndex (or null)
+    const unsigned* tileOrder;      // [tiles] queue position -> tile, longest first (or null: row-major)
 };
 
 // FP64 critical-band pass: classification of every pixel, then the listed
@@ -39,6 +40,8 @@ struct BandArgs
     unsigned* list;         // [W*H] band pixel indices
     unsigned* count;        // list length (zero before classification)
     unsigned* queue;        // band-trace queue head (zero before the band trace)
+    unsigned* tileKey;      // [tiles] cost key of each 8x8 tile (all ones before classification, or null)
+    int tilesX;             // ceil(W/8)
     float2 jitter;
     const int* dynIndex;    // device sample index overriding P.sampleIndex (or null)
     // learned surrogate: escaping non-band rays inside the training domain
@@ -174,6 +177,11 @@ hipError_t launchBlurVComposite(const RenderParams& P, const float4* accum, cons
 hipError_t launchComposite(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st);
 // batched post chain over a device ViewDesc table (max sizes bound the grids)
 hipError_t launchResetBatch(const BatchArgs& A, hipStream_t st);
+// Tile order for the FP32 tile tracer from the classification's per-tile
+// keys: fill (before classification), then a counting sort into `order`
+// (hist: 512 counters of scratch).
+hipError_t launchTileKeyFill(unsigned* tileKey, int nTiles, unsigned* hist, hipStream_t st);
+hipError_t launchTileOrder(const unsigned* tileKey, int nTiles, unsigned* hist, unsigned* order, hipStream_t st);
 hipError_t launchShadeBlendBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);
 hipError_t launchBloomBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);
 

@@ -180,7 +180,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Pol::fast ?
     {
         const unsigned t = nextTile(A.counter, lane);
         if (t >= nTiles) break;
-        traceTile<Ray, kCoop>(A.P, A.K, A.rec, A.tilesX, jit, t, lane, A.bandMask);
+        const unsigned tile = A.tileOrder ? A.tileOrder[t] : t;
+        traceTile<Ray, kCoop>(A.P, A.K, A.rec, A.tilesX, jit, tile, lane, A.bandMask);
     }
 }
 

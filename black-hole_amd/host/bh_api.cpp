@@ -101,6 +101,12 @@ struct Scratch
     unsigned char* bandMask = nullptr;
     unsigned* bandList = nullptr;
     size_t bandCap = 0;
+    // tile order of the FP32 tile tracer (longest tiles first): per-tile
+    // keys, the order, and 512 counters of the counting sort
+    unsigned* tileKey = nullptr;
+    unsigned* tileOrder = nullptr;
+    unsigned* tileHist = nullptr;
+    size_t tileCap = 0;
     // surrogate candidates (inputs, pixel list); counts in counter[32] and
     // counter[48] (accepted)
     float* candInputs = nullptr;
@@ -122,6 +128,10 @@ constexpr int kBandCountSlot = 16;            // counter[16]: band list length (
 constexpr int kCandCountSlot = 32;            // counter[32]: surrogate candidates
 constexpr int kAcceptSlot = 48;               // counter[48]: rays answered by the surrogate
 constexpr int kBandQueueSlot = 64;            // counter[64]: band-trace queue head
+// Tiles are ordered longest first when a frame has at most this many tiles
+// per wavefront of the persistent grid (1080p: 8; 2160p: 32, where the tail
+// is already short and the ordering gains nothing).
+constexpr long long kOrderTilesPerWave = 16;
 constexpr size_t kCounterBytes = 512;
 // learned surrogate attached to the render path (null: off)
 struct SurrogateRoute
@@ -240,6 +250,28 @@ SurrogateRoute currentSurrogate()
     return g_sur;
 }
 
+hipError_t getTileScratch(Scratch* S, size_t nTiles, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (nTiles <= S->tileCap) return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (S->tileKey)
+    {
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        (void)hipFree(S->tileKey);
+        (void)hipFree(S->tileOrder);
+        (void)hipFree(S->tileHist);
+        S->tileKey = nullptr; S->tileOrder = nullptr; S->tileHist = nullptr; S->tileCap = 0;
+    }
+    e = hipMalloc(&S->tileKey, nTiles * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&S->tileOrder, nTiles * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&S->tileHist, 512 * sizeof(unsigned));
+    if (e != hipSuccess) return e;
+    S->tileCap = nTiles;
+    return hipSuccess;
+}
+
 hipError_t getBandScratch(Scratch* S, size_t n, hipStream_t st)
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -285,7 +317,15 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     unsigned* counter = Sc->counter;
     const BhPrecision pr = currentPrecision();
     const bool band = pr.fp64Band != 0 && P.width > 0 && P.height > 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    const int kerr = isKerr(P.model) ? 1 : 0;
+    const int tiles = sc.raysPerRefill >= 64 ? 1 : 0;
+    const int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, sc.diskCoop, tiles)
+                                : bhk::traceOccupancyExact(kerr, sc.wgSize, sc.diskCoop, tiles);
+    const long long want = (long long)occ * numCUs(dev);
     bhk::BandArgs BA;
+    BA.tileKey = nullptr;
     if (band)
     {
         const size_t n = (size_t)P.width * (size_t)P.height;
@@ -299,6 +339,18 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.list = Sc->bandList;
         BA.count = counter + kBandCountSlot;
         BA.queue = counter + kBandQueueSlot;
+        BA.tilesX = (P.width + 7) / 8;
+        BA.tileKey = nullptr;
+        const long long nT = (long long)BA.tilesX * ((P.height + 7) / 8);
+        if (tiles && nT <= kOrderTilesPerWave * want * (sc.wgSize / 64))
+        {
+            // whole-tile schedule with few tiles per wavefront, where the last
+            // tiles set the frame time: order the tiles by the
+            // classification's cost keys, longest first
+            e = getTileScratch(Sc, (size_t)nT, st);
+            if (e != hipSuccess) return e;
+            BA.tileKey = Sc->tileKey;
+        }
         BA.jitter = jitterOf(P.accumMode, P.sampleIndex);
         BA.dynIndex = dynIndex;
         BA.candInputs = nullptr;
@@ -313,11 +365,14 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             BA.candInputs = Sc->candInputs;
             BA.candList = Sc->candList;
         }
+        const int nTiles = BA.tilesX * ((P.height + 7) / 8);
         e = bhk::launchResetCounter(BA.count, st);
+        if (e == hipSuccess && BA.tileKey) e = bhk::launchTileKeyFill(BA.tileKey, nTiles, Sc->tileHist, st);
         if (e == hipSuccess) e = bhk::launchResetCounter(BA.queue, st);
         if (e == hipSuccess) e = bhk::launchResetCounter(counter + kCandCountSlot, st);
         if (e == hipSuccess) e = bhk::launchResetCounter(counter + kAcceptSlot, st);
         if (e == hipSuccess) e = bhk::launchBandClassify(BA, st);
+        if (e == hipSuccess && BA.tileKey) e = bhk::launchTileOrder(BA.tileKey, nTiles, Sc->tileHist, Sc->tileOrder, st);
         if (e != hipSuccess) return e;
         if (sur.s)
         {
@@ -347,6 +402,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     }
     bhk::TraceArgs A;
     A.bandMask = band ? Sc->bandMask : nullptr;
+    A.tileOrder = (band && BA.tileKey) ? Sc->tileOrder : nullptr;
     A.P = P;
     A.rec = rec;
     A.counter = counter;
@@ -358,13 +414,6 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     A.coop = sc.diskCoop;
     A.jitter = jitterOf(P.accumMode, P.sampleIndex);
     A.dynIndex = dynIndex;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-    const int kerr = isKerr(P.model) ? 1 : 0;
-    const int tiles = sc.raysPerRefill >= 64 ? 1 : 0;
-    int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, sc.diskCoop, tiles)
-                          : bhk::traceOccupancyExact(kerr, sc.wgSize, sc.diskCoop, tiles);
-    long long want = (long long)occ * numCUs(dev);
     long long maxUseful = ((long long)A.totalIds + sc.wgSize - 1) / sc.wgSize;
     bhk::LaunchShape S;
     S.wgSize = sc.wgSize;
@@ -840,7 +889,12 @@ bh_error_t bhAccumCapture(BhAccumulator* acc, const RenderParams& p, bh_stream_t
     const size_t n = (size_t)P.width * (size_t)P.height;
     Scratch* S = nullptr;
     e = getScratch(cs, n, &S);
-    if (e == hipSuccess && currentPrecision().fp64Band) e = getBandScratch(S, n, cs);
+    if (e == hipSuccess && currentPrecision().fp64Band)
+    {
+        e = getBandScratch(S, n, cs);
+        if (e == hipSuccess) e = getTileScratch(S, (size_t)((P.width + 7) / 8) * (size_t)((P.height + 7) / 8), cs);
+        if (e == hipSuccess && currentSurrogate().s) e = getCandScratch(S, n, cs);
+    }
     if (e != hipSuccess) return e;
     if (acc->exec) { (void)hipGraphExecDestroy(acc->exec); acc->exec = nullptr; }
     if (acc->graph) { (void)hipGraphDestroy(acc->graph); acc->graph = nullptr; }

@@ -172,3 +172,41 @@ def test_training_reduces_loss(gpu):
     assert np.isfinite(loss).all()
     assert last[0] < 0.7 * first[0]          # exit regression + log-variance fit falls
     assert last[1] < 0.8 * first[1]          # disk-hit BCE falls
+
+
+def test_view_dataset_uses_the_view_camera(gpu, oracle):
+    """bhSurrogateDatasetFor: every row carries the given view's camera inputs
+    and black hole, and the exit labels agree with the FP64 oracle."""
+    P = gpu.workload("kerr09_1080p_high", 0)
+    P.diskEnabled = 0
+    s = gpu.Surrogate(seed=1)
+    try:
+        s.dataset_for([P], 256, seed=4)
+        X, T, E = s.dataset_view()
+    finally:
+        s.close()
+    assert X.shape == (256, 6)
+    o = np.array([P.camPos.x, P.camPos.y, P.camPos.z], np.float64)
+    r = np.linalg.norm(o)
+    assert np.allclose(X[:, 2], P.aSpin / P.M) and np.allclose(X[:, 3], 0.0)
+    assert np.allclose(X[:, 4], np.log(r / 6.0) / np.log(40.0 / 6.0), atol=1e-5)
+    assert np.allclose(X[:, 5], np.arccos(o[1] / r) / np.pi, atol=1e-5)
+    assert (T["disk"] == 0).all()                 # disk off: no disk labels
+    th = np.arccos(np.clip(o[1] / r, -1, 1))
+    ph = np.arctan2(-o[2], o[0])
+    er = np.array([np.sin(th) * np.cos(ph), np.cos(th), -np.sin(th) * np.sin(ph)])
+    eth = np.array([np.cos(th) * np.cos(ph), -np.sin(th), -np.cos(th) * np.sin(ph)])
+    eph = np.array([-np.sin(ph), 0.0, -np.cos(ph)])
+    b = X[:, 0].astype(np.float64) * 40.0
+    al = X[:, 1].astype(np.float64) * np.pi
+    psi = np.arcsin(np.clip(b / r, 0, 1))
+    d = (-np.cos(psi))[:, None] * er + np.sin(psi)[:, None] * (np.cos(al)[:, None] * (-eth) + np.sin(al)[:, None] * eph)
+    rec = oracle.probe_dirs(P, d.astype(np.float32), mode=1)
+    esc = (rec[:, 0] == 1) & (T["escaped"] > 0.5)
+    assert ((rec[:, 0] == 1) == (T["escaped"] > 0.5)).mean() >= 0.99
+    a = E[esc].astype(np.float64)
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    bd = rec[esc, 1:4] / np.linalg.norm(rec[esc, 1:4], axis=1, keepdims=True)
+    ang = 2 * np.arcsin(np.clip(np.linalg.norm(a - bd, axis=1) / 2, 0, 1))
+    say("view dataset: %d escaped, labels vs FP64 max %.2e" % (int(esc.sum()), ang.max()))
+    assert (ang <= 1e-3).mean() >= 0.99
