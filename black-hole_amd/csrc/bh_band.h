@@ -503,9 +503,8 @@ template <class Pol, class D> struct KerrRay64
             double sp, cp;
             D::sincos_(ph, sp, cp);
             const d3 pos = dk3(r * sth * cp, r * cth, -r * sth * sp);
-            const d3 stepv = pos - prevPos;
-            const double sl = dlen(stepv);
-            if (sl > F2D(1e-6)) dir3 = stepv * (1.0 / sl);
+            // the unit step direction is not formed (its only reader in the
+            // reference, the disk segment sampler, ignores it)
             if (trans > 0.01f) chordPrepare<Pol>(Dk, tof3(prevPos), tof3(pos), (float)(B.diskTime - tFlight), ch);
             prevPos = pos;
         }

@@ -436,11 +436,12 @@ template <class M, bool kWant> struct KerrRay
         }
         if (K.disk)
         {
+            // the reference also keeps the unit step direction here
+            // (trace.cuh:859-862), but its only reader, the disk segment
+            // sampler, ignores it (trace.cuh:270), and the exit direction
+            // comes from the coordinate velocity: it is not formed
             float sp = M::sin_(ph), cp = M::cos_(ph);
             v3 pos = mk3(r * sth * cp, r * cth, -r * sth * sp);
-            v3 stepv = pos - prevPos;
-            float sl = len3<M>(stepv);
-            if (sl > 1e-6f) dir3 = stepv * M::rcp(sl);
             if (trans > 0.01f) chordPrepare<M>(K.D, prevPos, pos, K.diskTime - tFlight, ch);
             prevPos = pos;
         }

@@ -100,12 +100,19 @@ def test_render_kernel_single_pass(gpu, oracle, math):
 
 def test_post_chain_bit_exact(gpu, oracle):
     """Bloom + composite on the GPU's own accumulation buffer reproduce the
-    oracle's post chain bit for bit (the post chain is exact arithmetic)."""
+    oracle's post chain: IEEE arithmetic without contraction on both sides,
+    so every pixel matches except where the gamma's powf (OCML on the GPU,
+    glibc on the host; each within 1 ulp, neither always correctly rounded)
+    lands on the other side of a quantisation boundary: at most 1 LSB, on at
+    most 1 in 10^4 pixels."""
     P = scaled_workload(gpu, "kerr09_1080p_high", 0, 333, 187)   # odd sizes: edge clamps
     say("post chain 333x187")
     rgba, acc = gpu.render_pipeline(P)
     ref = oracle.post(gpu.sanitized(P)[0], acc)
-    assert np.array_equal(rgba, ref)
+    d = np.abs(rgba.astype(np.int32) - ref.astype(np.int32)).max(-1)
+    say("post chain: %d of %d pixels differ, max %d LSB" % (int((d > 0).sum()), d.size, int(d.max())))
+    assert d.max() <= 1
+    assert (d > 0).sum() <= max(1, d.size // 10000)
 
 
 @pytest.mark.parametrize("math", MODES)
