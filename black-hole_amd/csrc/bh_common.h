@@ -104,7 +104,18 @@ template <class M> BHK_INL v3 normalize(v3 a) { return a * M::rsq(fmaxf(dot(a, a
 template <class M> BHK_INL v3 vdiv(v3 a, float s) { return a * M::rcp(s); }
 BHK_INL v3 lerp3(v3 a, v3 b, float t) { return a + (b - a) * t; }
 BHK_INL float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
-BHK_INL float fract(float x) { return x - floorf(x); }
+// x - floor(x).  The hardware fract equals it wherever x - floor(x) does not
+// round up to 1, i.e. for every x outside (-2^-25, 0); the hash arguments
+// are lattice coordinates times 0.1031/0.1030/0.0973, lattice sums scaled
+// likewise, or non-negative, so none falls there.
+BHK_INL float fract(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fractf(x);
+#else
+    return x - floorf(x);
+#endif
+}
 BHK_INL float lerpf(float a, float b, float t) { return a + (b - a) * t; }
 template <class M> BHK_INL float smoothstepf(float e0, float e1, float x)
 {

@@ -371,14 +371,30 @@ template <class M, bool kWant> struct KerrRay
             return true;
         }
         // adaptive affine step (trace.cuh:793-809)
-        float h = M::rcp(M::div(fabsf(d1.r), K.kRad * fmaxf(r - K.rFloor, 0.02f)) +
-                         M::divc(fabsf(d1.th) + fabsf(d1.ph), K.kAng, K.invKAng) + 1e-5f);
-        if (r < K.rPhotonRefine) h *= 0.55f;
-        float spd = fabsf(d1.r) + r * (fabsf(d1.th) + fabsf(d1.ph)) + 1e-6f;
+        const float spd = fabsf(d1.r) + r * (fabsf(d1.th) + fabsf(d1.ph)) + 1e-6f;
         float capLen;
         if (K.disk && r < K.diskOuterCap) capLen = (r * fabsf(cth) < 1.8f) ? 1.2f : 2.2f;
         else capLen = fmaxf(1.2f, 0.22f * r);
-        h = fminf(h, M::div(capLen, spd));
+        float h;
+        if constexpr (M::fast)
+        {
+            // h = min(f Bq / (|dr| + C Bq), capLen / spd) with one reciprocal:
+            // 1 / (|dr| / Bq + C) = Bq / (|dr| + C Bq), and the smaller of two
+            // positive ratios is picked by cross-multiplying
+            const float Bq = K.kRad * fmaxf(r - K.rFloor, 0.02f);
+            const float C = fmaf(fabsf(d1.th) + fabsf(d1.ph), K.invKAng, 1e-5f);
+            const float D = fmaf(C, Bq, fabsf(d1.r));
+            const float n1 = (r < K.rPhotonRefine) ? 0.55f * Bq : Bq;
+            const bool first = n1 * spd <= capLen * D;
+            h = (first ? n1 : capLen) * M::rcp(first ? D : spd);
+        }
+        else
+        {
+            h = M::rcp(M::div(fabsf(d1.r), K.kRad * fmaxf(r - K.rFloor, 0.02f)) +
+                       M::divc(fabsf(d1.th) + fabsf(d1.ph), K.kAng, K.invKAng) + 1e-5f);
+            if (r < K.rPhotonRefine) h *= 0.55f;
+            h = fminf(h, M::div(capLen, spd));
+        }
 
         // classic RK4; the weighted stage sum (d1 + 2 d2) + 2 d3 + d4 is
         // accumulated as each stage completes (same operations, same order)
