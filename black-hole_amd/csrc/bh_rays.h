@@ -280,14 +280,12 @@ template <class M, bool kWant> struct KerrRay
     int step, status, haveDeriv;
     KDeriv nx;                    // fast policy: right-hand side at the current state
     float nxK;
-    int haveNext;
 
     BHK_INL void init(const FrameK& K, v3 origin, v3 dir)
     {
         status = RAY_RUNNING;
         step = 0;
         haveDeriv = 0;
-        haveNext = 0;
         escDir = dir;
         acc = mk3(0.f, 0.f, 0.f);
         trans = 1.0f;
@@ -326,6 +324,9 @@ template <class M, bool kWant> struct KerrRay
         tFlight = 0.f;
         prevPos = origin;
         dir3 = dir;
+        // fast policy: the first step's first stage (each step then leaves
+        // the next step's first stage in nx)
+        if constexpr (M::fast) nx = kerrRHS<M, kWant>(K, E, Lz, r, pr, pth, sth, cth, nxK);
     }
 
     BHK_INL v3 bodyDir(float dr_, float dth_, float dph_, float sp, float cp) const
@@ -354,7 +355,7 @@ template <class M, bool kWant> struct KerrRay
         }
         float Kv = 0.f;
         KDeriv d1;
-        if (M::fast && haveNext) { d1 = nx; Kv = nxK; }
+        if constexpr (M::fast) { d1 = nx; Kv = nxK; }
         else d1 = kerrRHS<M, kWant>(K, E, Lz, r, pr, pth, sth, cth, Kv);
         if (!isfinite(r) || !isfinite(pr) || !isfinite(pth) || !isfinite(d1.r) || !isfinite(d1.pr))
         { status = RAY_CAPTURED; return true; }
@@ -432,7 +433,6 @@ template <class M, bool kWant> struct KerrRay
             // Delta is clamped at 1e-5 there)
             float Pp, PD, iS, DelRaw;
             nx = kerrRHSFastX<M, kWant>(K, E, Lz, r, pr, pth, sth, cth, nxK, Pp, PD, iS, DelRaw);
-            haveNext = 1;
             const float s = sth < 1e-4f ? 1e-4f : sth;
             const float a = K.a;
             const float PDs = DelRaw < 1e-5f ? Pp * 1e5f : PD;
