@@ -123,7 +123,7 @@ struct Scratch
 std::mutex g_mu;
 std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 BhSchedule g_sched = {256, 4096, 64, 1, 0};   // wgSize, stepsPerChunk, raysPerRefill, mathMode, diskCoop
-BhPrecision g_prec = {1, 3e-2f, 1e-3f};
+BhPrecision g_prec = {1, 4e-2f, 1e-3f};
 constexpr int kBandCountSlot = 16;            // counter[16]: band list length (own 64-byte line)
 constexpr int kCandCountSlot = 32;            // counter[32]: surrogate candidates
 constexpr int kAcceptSlot = 48;               // counter[48]: rays answered by the surrogate

@@ -292,7 +292,7 @@ void bhAutotuneClear(void);
 typedef struct BhPrecision
 {
     int   fp64Band;     /* 0 off (pure FP32, the reference's arithmetic), 1 on */
-    float critBand;     /* default 3e-2 */
+    float critBand;     /* default 4e-2 */
     float polarBand;    /* default 1e-3 */
 } BhPrecision;
 void bhGetPrecision(BhPrecision* p);
