@@ -92,6 +92,29 @@ struct D64Fast
         s = (q & 2) ? -s0 : s0;
         c = ((q + 1) & 2) ? -c0 : c0;
     }
+    // sin and cos of x0 + dl from s0, c0 = sin, cos of x0: for |dl| < 1/16
+    // by angle addition with Taylor series of sin dl (to dl^9) and cos dl (to
+    // dl^10), whose truncation stays below 1e-20; otherwise directly
+    BHK_INL static void sincosNear(double x0, double s0, double c0, double dl, double& s, double& c)
+    {
+        if (!(fabs(dl) < 0.0625))
+        {
+            sincos_(x0 + dl, s, c);
+            return;
+        }
+        const double d2 = dl * dl;
+        double ps = __builtin_fma(d2, 2.7557319223985893e-06, -1.984126984126984e-04);
+        ps = __builtin_fma(ps, d2, 8.333333333333333e-03);
+        ps = __builtin_fma(ps, d2, -1.6666666666666666e-01);
+        const double sd = __builtin_fma(dl * d2, ps, dl);
+        double pc = __builtin_fma(d2, -2.755731922398589e-07, 2.48015873015873e-05);
+        pc = __builtin_fma(pc, d2, -1.388888888888889e-03);
+        pc = __builtin_fma(pc, d2, 4.1666666666666664e-02);
+        pc = __builtin_fma(pc, d2, -0.5);
+        const double cd = __builtin_fma(d2, pc, 1.0);
+        s = __builtin_fma(s0, cd, c0 * sd);
+        c = __builtin_fma(c0, cd, -(s0 * sd));
+    }
 };
 
 // Per-frame FP64 constants: the RenderParams floats widened to double and
@@ -404,10 +427,47 @@ BHK_INL KD64 kerrRHS64(const BandK& B, double E, double Lz, double r, double pr,
     return d;
 }
 
+// Fast-policy FP64 right-hand side: the same Hamiltonian reduced as in the
+// FP32 fast path (three reciprocals, P/Delta formed once,
+// W_r = -4 r E (P/Delta) + (P/Delta)^2 Delta'), with explicit FMAs; also
+// returns P, P/Delta, 1/Sigma and the unclamped Delta for the slow-light
+// integral.
+template <class D>
+BHK_INL KD64 kerrRHS64Fx(const BandK& B, double E, double Lz, double r, double pr, double pth, double s, double cth,
+                         double& PpO, double& PDO, double& invSO, double& DelO)
+{
+    if (s < F2D(1e-4)) s = F2D(1e-4);
+    const double a = B.a, a2 = a * a, aE = a * E, r2 = r * r;
+    const double is = D::rcp(s), is2 = is * is;
+    const double Pp = __builtin_fma(E, r2, E * a2 - a * Lz);
+    const double DelRaw = __builtin_fma(r, r - 2.0 * B.M, a2 + B.Q * B.Q);
+    const double Del = DelRaw < F2D(1e-6) ? F2D(1e-6) : DelRaw;
+    const double Sig = __builtin_fma(a2 * cth, cth, r2);
+    const double iDel = D::rcp(Del), invS = D::rcp(Sig);
+    const double PD = Pp * iDel;
+    const double Bq = __builtin_fma(Lz, is, -(aE * s));
+    const double W = __builtin_fma(Bq, Bq, -(Pp * PD));
+    const double Dpr = Del * pr;
+    const double K = __builtin_fma(Dpr, pr, __builtin_fma(pth, pth, W));
+    const double dDel = 2.0 * r - 2.0 * B.M;
+    const double Wr = __builtin_fma(PD * PD, dDel, -(4.0 * E * r * PD));
+    const double dB = -cth * __builtin_fma(Lz, is2, aE);
+    const double iSK = invS * K;
+    KD64 d;
+    d.r = Dpr * invS;
+    d.th = pth * invS;
+    d.ph = __builtin_fma(a, PD, __builtin_fma(Lz, is2, -aE)) * invS;
+    d.pr = invS * __builtin_fma(-0.5, __builtin_fma(dDel * pr, pr, Wr), r * iSK);
+    d.pth = invS * __builtin_fma(-Bq, dB, -((a2 * s * cth) * iSK));
+    PpO = Pp; PDO = PD; invSO = invS; DelO = DelRaw;
+    return d;
+}
+
 template <class Pol, class D> struct KerrRay64
 {
     double r, th, ph, pr, pth, sth, cth, E, Lz, tFlight;
     KD64 dLast;
+    KD64 nx;                      // fast policy: right-hand side at the current state
     d3 prevPos, dir3, escDir;
     v3 acc;
     float trans, Ef, Lzf;
@@ -432,6 +492,11 @@ template <class Pol, class D> struct KerrRay64
         tFlight = 0.0;
         prevPos = origin;
         dir3 = dir;
+        if constexpr (D::fast)
+        {
+            double t0, t1, t2, t3;
+            nx = kerrRHS64Fx<D>(B, E, Lz, r, pr, pth, sth, cth, t0, t1, t2, t3);
+        }
     }
 
     BHK_INL d3 bodyDir(const KD64& d) const
@@ -453,30 +518,61 @@ template <class Pol, class D> struct KerrRay64
             else { status = RAY_ESCAPED; escDir = haveDeriv ? bodyDir(dLast) : dir3; }
             return true;
         }
-        const KD64 d1 = kerrRHS64<D>(B, E, Lz, r, pr, pth, sth, cth);
+        KD64 d1;
+        if constexpr (D::fast) d1 = nx;
+        else d1 = kerrRHS64<D>(B, E, Lz, r, pr, pth, sth, cth);
         if (!isfinite(r) || !isfinite(pr) || !isfinite(pth) || !isfinite(d1.r) || !isfinite(d1.pr))
         { status = RAY_CAPTURED; return true; }
         if (r <= B.capR) { status = RAY_CAPTURED; return true; }
         if (r >= B.escR && d1.r > 0.0) { escDir = bodyDir(d1); status = RAY_ESCAPED; return true; }
-        double h = D::rcp(D::div(fabs(d1.r), B.kRad * fmax(r - B.rFloor, F2D(0.02))) +
-                          D::div(fabs(d1.th) + fabs(d1.ph), B.kAng) + F2D(1e-5));
-        if (r < B.rPhotonRefine) h = h * F2D(0.55);
         const double spd = fabs(d1.r) + r * (fabs(d1.th) + fabs(d1.ph)) + F2D(1e-6);
         double capLen;
         if (B.disk && r < B.diskOuterCap) capLen = (r * fabs(cth) < F2D(1.8)) ? F2D(1.2) : F2D(2.2);
         else capLen = fmax(F2D(1.2), F2D(0.22) * r);
-        h = fmin(h, D::div(capLen, spd));
+        double h;
+        if constexpr (D::fast)
+        {
+            // one reciprocal: f Bq / (|dr| + C Bq) against capLen / spd
+            const double Bq = B.kRad * fmax(r - B.rFloor, F2D(0.02));
+            const double C = D::div(fabs(d1.th) + fabs(d1.ph), B.kAng) + F2D(1e-5);
+            const double Dn = __builtin_fma(C, Bq, fabs(d1.r));
+            const double n1 = (r < B.rPhotonRefine) ? F2D(0.55) * Bq : Bq;
+            const bool first = n1 * spd <= capLen * Dn;
+            h = (first ? n1 : capLen) * D::rcp(first ? Dn : spd);
+        }
+        else
+        {
+            h = D::rcp(D::div(fabs(d1.r), B.kRad * fmax(r - B.rFloor, F2D(0.02))) +
+                       D::div(fabs(d1.th) + fabs(d1.ph), B.kAng) + F2D(1e-5));
+            if (r < B.rPhotonRefine) h = h * F2D(0.55);
+            h = fmin(h, D::div(capLen, spd));
+        }
         const double hh = F2D(0.5) * h;
         double ss, cs;
-        double yth = th + hh * d1.th;
-        D::sincos_(yth, ss, cs);
-        const KD64 d2 = kerrRHS64<D>(B, E, Lz, r + hh * d1.r, pr + hh * d1.pr, pth + hh * d1.pth, ss, cs);
-        yth = th + hh * d2.th;
-        D::sincos_(yth, ss, cs);
-        const KD64 d3_ = kerrRHS64<D>(B, E, Lz, r + hh * d2.r, pr + hh * d2.pr, pth + hh * d2.pth, ss, cs);
-        yth = th + h * d3_.th;
-        D::sincos_(yth, ss, cs);
-        const KD64 d4 = kerrRHS64<D>(B, E, Lz, r + h * d3_.r, pr + h * d3_.pr, pth + h * d3_.pth, ss, cs);
+        KD64 d2, d3_, d4;
+        if constexpr (D::fast)
+        {
+            double t0, t1, t2, t3;
+            D::sincosNear(th, sth, cth, hh * d1.th, ss, cs);
+            d2 = kerrRHS64Fx<D>(B, E, Lz, r + hh * d1.r, pr + hh * d1.pr, pth + hh * d1.pth, ss, cs, t0, t1, t2, t3);
+            D::sincosNear(th, sth, cth, hh * d2.th, ss, cs);
+            d3_ = kerrRHS64Fx<D>(B, E, Lz, r + hh * d2.r, pr + hh * d2.pr, pth + hh * d2.pth, ss, cs, t0, t1, t2,
+                                 t3);
+            D::sincosNear(th, sth, cth, h * d3_.th, ss, cs);
+            d4 = kerrRHS64Fx<D>(B, E, Lz, r + h * d3_.r, pr + h * d3_.pr, pth + h * d3_.pth, ss, cs, t0, t1, t2, t3);
+        }
+        else
+        {
+            double yth = th + hh * d1.th;
+            D::sincos_(yth, ss, cs);
+            d2 = kerrRHS64<D>(B, E, Lz, r + hh * d1.r, pr + hh * d1.pr, pth + hh * d1.pth, ss, cs);
+            yth = th + hh * d2.th;
+            D::sincos_(yth, ss, cs);
+            d3_ = kerrRHS64<D>(B, E, Lz, r + hh * d2.r, pr + hh * d2.pr, pth + hh * d2.pth, ss, cs);
+            yth = th + h * d3_.th;
+            D::sincos_(yth, ss, cs);
+            d4 = kerrRHS64<D>(B, E, Lz, r + h * d3_.r, pr + h * d3_.pr, pth + h * d3_.pth, ss, cs);
+        }
         const double h6 = D::fast ? h * (1.0 / 6.0) : h / 6.0;
         r = r + h6 * (d1.r + 2.0 * d2.r + 2.0 * d3_.r + d4.r);
         th = th + h6 * (d1.th + 2.0 * d2.th + 2.0 * d3_.th + d4.th);
@@ -489,6 +585,18 @@ template <class Pol, class D> struct KerrRay64
         dLast = d4;
         haveDeriv = 1;
         D::sincos_(th, sth, cth);
+        if constexpr (D::fast)
+        {
+            // next step's first stage; slow light from its P/Delta and 1/Sigma
+            double Pp, PD, iS, DelRaw;
+            nx = kerrRHS64Fx<D>(B, E, Lz, r, pr, pth, sth, cth, Pp, PD, iS, DelRaw);
+            const double s = sth < F2D(1e-4) ? F2D(1e-4) : sth;
+            const double a = B.a;
+            const double PDs = DelRaw < F2D(1e-5) ? Pp * (1.0 / F2D(1e-5)) : PD;
+            const double dtdl = __builtin_fma(a, Lz - a * E * s * s, (r * r + a * a) * PDs) * iS;
+            tFlight = tFlight + h * fabs(dtdl);
+        }
+        else
         {
             const double s = sth < F2D(1e-4) ? F2D(1e-4) : sth;
             const double a = B.a;
