@@ -40,7 +40,7 @@ FLOPS = {0: 506225668732 + 80870400, 1: 506225457204 + 80870400, 2: 506228100901
 # HBM bytes per launch of the FP32 tile kernel on WORKLOAD (FP64 band on):
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by a
 # plain-copy calibration (FETCH x2.0, WRITE x1.0); profiles/pmc_trace_kerr09_1080p_high.md
-TRAFFIC_BYTES = (6773 * 2 + 65100) * 1024
+TRAFFIC_BYTES = (7772 * 2 + 65008) * 1024
 FP32_NONPACKED_TFLOPS = 78.6
 
 

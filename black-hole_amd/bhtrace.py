@@ -170,6 +170,7 @@ SIGNATURES = [
     ("bhGetPrecision", None, [C.POINTER(BhPrecision)]),
     ("bhSetPrecision", C.c_int, [C.POINTER(BhPrecision)]),
     ("bhLastBandStats", C.c_int, [C.POINTER(C.c_uint), C.POINTER(C.c_float), _vp]),
+    ("bhLastTileOrder", C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_int), _vp]),
     ("bhEnableKernelTimers", None, [C.c_int]),
     ("bhLastKernelTimes", C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("bhSurrogateParamCount", C.c_int, []),
@@ -745,6 +746,18 @@ def set_precision(band: int | None = None, crit: float | None = None, polar: flo
         p.polarBand = polar
     check(load().bhSetPrecision(C.byref(p)), "bhSetPrecision")
     return p
+
+
+def last_tile_order(n_max: int, stream=None):
+    """(keys, order) of the last frame's tile queue, or None when it was not ordered."""
+    keys = np.zeros(n_max, np.uint32)
+    order = np.zeros(n_max, np.uint32)
+    n = C.c_int()
+    e = load().bhLastTileOrder(keys.ctypes.data, order.ctypes.data, n_max, C.byref(n), stream)
+    if n.value == 0:
+        return None
+    check(e, "bhLastTileOrder")
+    return keys[:n.value], order[:n.value]
 
 
 def restore_precision(p: BhPrecision) -> None:

@@ -18,6 +18,8 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool band = false, cand = false;
     float x[kMlpInputs];
+    int tile = -1;               // tile of this pixel (-1: none)
+    unsigned key = 0xFFFFFFFFu;  // its cost key (all ones: no FP32 ray)
     if (i < W * H)
     {
         const int px = i % W, py = i / W;
@@ -27,14 +29,11 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
         const bool ok = bandStats(A.B, o, cameraDir64(A.P, px, py, jit), delta, polar);
         band = !ok || (!(A.B.kerr == 0 && delta == 1.0) && (fabs(delta) < A.B.critBand || polar < A.B.polarBand));
         A.mask[i] = band ? 1 : 0;
-        if (A.tileKey && !band)
-        {
-            // cost key of the pixel's tile: rays near the critical curve or a
-            // polar turning point wind the longest; non-negative floats order
-            // like their bit patterns
-            const float k = (float)fmin(fabs(delta), 10.0 * polar);
-            atomicMin(A.tileKey + (py >> 3) * A.tilesX + (px >> 3), __float_as_uint(k));
-        }
+        tile = (py >> 3) * A.tilesX + (px >> 3);
+        // cost key of the pixel's tile: rays near the critical curve or a
+        // polar turning point wind the longest; non-negative floats order
+        // like their bit patterns
+        if (!band) key = __float_as_uint((float)fmin(fabs(delta), 10.0 * polar));
         if (A.candList && !band && delta < -A.B.critBand)
         {
             // candidate of the surrogate: escaping ingoing ray inside the training domain
@@ -52,6 +51,21 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
         }
     }
     const unsigned lane = __lane_id();
+    if (A.tileKey)
+    {
+        // Eight consecutive lanes usually hold eight pixels of one tile row:
+        // such groups reduce their keys first and issue one atomic.
+        const int lead = __shfl(tile, (int)(lane & ~7u));
+        const unsigned long long same = __ballot(tile == lead);
+        if (((same >> (lane & ~7u)) & 0xFFull) == 0xFFull)
+        {
+            key = min(key, (unsigned)__shfl_xor((int)key, 1));
+            key = min(key, (unsigned)__shfl_xor((int)key, 2));
+            key = min(key, (unsigned)__shfl_xor((int)key, 4));
+            if ((lane & 7u) == 0u && tile >= 0 && key != 0xFFFFFFFFu) atomicMin(A.tileKey + tile, key);
+        }
+        else if (tile >= 0 && key != 0xFFFFFFFFu) atomicMin(A.tileKey + tile, key);
+    }
     const unsigned long long m = __ballot(band);
     if (m != 0ull)
     {
@@ -99,10 +113,17 @@ __global__ void tileKeyFillKernel(unsigned* tileKey, int n, unsigned* hist)
     if (i < 2 * kTileBuckets) hist[i] = 0u;
 }
 
-__global__ void tileHistKernel(const unsigned* tileKey, int n, unsigned* hist)
+// counts per bucket: a workgroup histogram in LDS, then one global atomic
+// per bucket and workgroup (many tiles share a bucket)
+__global__ __launch_bounds__(kTileBuckets) void tileHistKernel(const unsigned* tileKey, int n, unsigned* hist)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(hist + tileBucket(tileKey[i]), 1u);
+    __shared__ unsigned h[kTileBuckets];
+    const int t = threadIdx.x, i = blockIdx.x * kTileBuckets + t;
+    h[t] = 0u;
+    __syncthreads();
+    if (i < n) atomicAdd(h + tileBucket(tileKey[i]), 1u);
+    __syncthreads();
+    if (h[t]) atomicAdd(hist + t, h[t]);
 }
 
 // one workgroup of 256: exclusive scan of the bucket counts into the
@@ -124,10 +145,26 @@ __global__ __launch_bounds__(kTileBuckets) void tileScanKernel(unsigned* hist)
     hist[kTileBuckets + t] = s[t] - v;
 }
 
-__global__ void tileScatterKernel(const unsigned* tileKey, int n, unsigned* hist, unsigned* order)
+// each workgroup reserves one range per bucket from the bucket cursors and
+// places its tiles by their rank inside the workgroup
+__global__ __launch_bounds__(kTileBuckets) void tileScatterKernel(const unsigned* tileKey, int n, unsigned* hist,
+                                                                  unsigned* order)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) order[atomicAdd(hist + kTileBuckets + tileBucket(tileKey[i]), 1u)] = (unsigned)i;
+    __shared__ unsigned h[kTileBuckets], base[kTileBuckets];
+    const int t = threadIdx.x, i = blockIdx.x * kTileBuckets + t;
+    h[t] = 0u;
+    __syncthreads();
+    int b = 0;
+    unsigned rank = 0u;
+    if (i < n)
+    {
+        b = tileBucket(tileKey[i]);
+        rank = atomicAdd(h + b, 1u);
+    }
+    __syncthreads();
+    if (h[t]) base[t] = atomicAdd(hist + kTileBuckets + t, h[t]);
+    __syncthreads();
+    if (i < n) order[base[b] + rank] = (unsigned)i;
 }
 
 hipError_t launchTileKeyFill(unsigned* tileKey, int nTiles, unsigned* hist, hipStream_t st)
@@ -140,7 +177,7 @@ hipError_t launchTileKeyFill(unsigned* tileKey, int nTiles, unsigned* hist, hipS
 hipError_t launchTileOrder(const unsigned* tileKey, int nTiles, unsigned* hist, unsigned* order, hipStream_t st)
 {
     if (nTiles <= 0) return hipSuccess;
-    const dim3 g((nTiles + 255) / 256), b(256);
+    const dim3 g((nTiles + kTileBuckets - 1) / kTileBuckets), b(kTileBuckets);
     hipLaunchKernelGGL(tileHistKernel, g, b, 0, st, tileKey, nTiles, hist);
     hipLaunchKernelGGL(tileScanKernel, dim3(1), dim3(kTileBuckets), 0, st, hist);
     hipLaunchKernelGGL(tileScatterKernel, g, b, 0, st, tileKey, nTiles, hist, order);

@@ -107,6 +107,7 @@ struct Scratch
     unsigned* tileOrder = nullptr;
     unsigned* tileHist = nullptr;
     size_t tileCap = 0;
+    int lastOrderTiles = 0;      // tiles ordered in the last frame (0: row-major)
     // surrogate candidates (inputs, pixel list); counts in counter[32] and
     // counter[48] (accepted)
     float* candInputs = nullptr;
@@ -341,6 +342,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.queue = counter + kBandQueueSlot;
         BA.tilesX = (P.width + 7) / 8;
         BA.tileKey = nullptr;
+        Sc->lastOrderTiles = 0;
         const long long nT = (long long)BA.tilesX * ((P.height + 7) / 8);
         if (tiles && nT <= kOrderTilesPerWave * want * (sc.wgSize / 64))
         {
@@ -350,6 +352,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             e = getTileScratch(Sc, (size_t)nT, st);
             if (e != hipSuccess) return e;
             BA.tileKey = Sc->tileKey;
+            Sc->lastOrderTiles = (int)nT;
         }
         BA.jitter = jitterOf(P.accumMode, P.sampleIndex);
         BA.dynIndex = dynIndex;
@@ -1105,6 +1108,30 @@ bh_error_t bhLastBandStats(unsigned* count, float* bandMs, bh_stream_t stream)
     }
     return e;
 }
+bh_error_t bhLastTileOrder(unsigned* keys, unsigned* order, int capacity, int* nTiles, bh_stream_t stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    Scratch* S = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_scratch.find({dev, st});
+        if (it == g_scratch.end()) return hipErrorNotReady;
+        S = &it->second;
+    }
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    const int n = S->lastOrderTiles;
+    if (nTiles) *nTiles = n;
+    if (n == 0) return hipErrorNotReady;
+    if (capacity < n) return hipErrorInvalidValue;
+    if (keys) e = hipMemcpy(keys, S->tileKey, (size_t)n * sizeof(unsigned), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && order) e = hipMemcpy(order, S->tileOrder, (size_t)n * sizeof(unsigned), hipMemcpyDeviceToHost);
+    return e;
+}
+
 bh_error_t bhLastKernelTimes(float* traceMs, float* shadeMs, float* postMs)
 {
     Scratch* S = nullptr;

@@ -300,6 +300,13 @@ bh_error_t bhSetPrecision(const BhPrecision* p);   /* rejects negative or non-fi
 /* Band rays of the last frame traced on `stream` and, when kernel timers
  * are on, the FP64 band kernel's time in ms (synchronizes the stream). */
 bh_error_t bhLastBandStats(unsigned* count, float* bandMs, bh_stream_t stream);
+/* Tile order of the last frame on `stream` (whole-tile schedule with the
+ * band pass on, when the frame was ordered longest first): per-tile cost keys
+ * (float bits of min(|delta|, 10 polar) over the tile's FP32 rays, all ones
+ * when none) and the queue order, `capacity` entries each (either may be
+ * NULL).  *nTiles receives the tile count; hipErrorNotReady when the frame
+ * was not ordered.  Synchronizes the stream. */
+bh_error_t bhLastTileOrder(unsigned* keys, unsigned* order, int capacity, int* nTiles, bh_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Timing (HIP events recorded on the launch stream)                        */

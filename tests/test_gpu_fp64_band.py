@@ -122,3 +122,39 @@ def test_full_frame_fp64_trace_matches_fp64_oracle(gpu, oracle, name, frame, w, 
     assert n == w * h
     assert mism <= 2
     assert q[1] <= 1e-6 and ang.max() <= 1e-4
+
+
+@pytest.mark.parametrize("name,frame,w,h", [("kerr09_1080p_high", 0, 480, 270), ("kn_m07_q04_2160p_high", 0, 333, 187)])
+def test_tile_order_keys_and_permutation(gpu, oracle, band_on, name, frame, w, h):
+    """The tile queue of a band-on frame: each tile's key is the smallest
+    min(|delta|, 10 polar) over its FP32 rays (the oracle's FP64 band
+    statistics), the order is a permutation of the tiles, and it visits the
+    256 log2-key buckets in ascending order."""
+    P = scaled_workload(gpu, name, frame, w, h)
+    gpu.trace_records(P)
+    tx, ty = (w + 7) // 8, (h + 7) // 8
+    got = gpu.last_tile_order(tx * ty)
+    assert got is not None, "frame was not ordered"
+    keys, order = got
+    assert keys.size == tx * ty
+    assert np.array_equal(np.sort(order), np.arange(tx * ty))
+    m, d, pol = oracle.band_mask(P, band_on.critBand, band_on.polarBand)
+    k = np.where(m > 0, np.inf, np.minimum(np.abs(d), 10.0 * pol)).astype(np.float32)
+    pad = np.full((ty * 8, tx * 8), np.inf, np.float32)
+    pad[:h, :w] = k
+    want = pad.reshape(ty, 8, tx, 8).min(axis=(1, 3)).ravel()
+    gk = keys.view(np.float32).copy()
+    gk[keys == 0xFFFFFFFF] = np.inf
+    fin = np.isfinite(want)
+    assert np.array_equal(np.isfinite(gk), fin)
+    assert np.allclose(gk[fin], want[fin], rtol=1e-6, atol=0)
+
+    def bucket(v):
+        b = (np.log2(np.maximum(v, 1e-30)) + 8.0) * 16.0
+        out = np.clip(b, 0, 254).astype(int)
+        out[~np.isfinite(v)] = 255
+        return out
+    bo = bucket(gk[order])
+    # ascending buckets along the queue (1-bucket slack for float log2)
+    assert (np.diff(bo) >= -1).all()
+    say(name, "tiles", tx * ty, "ordered; first buckets", bo[:4], "last", bo[-4:])
