@@ -7,10 +7,10 @@ Kerr a*=0.9 1920x1080 'high' workload, cycling the disk time over
 t = 0, 0.5, 1.  Inputs are device resident; timing brackets exactly
 `--steps` frames with device synchronizations, after `--warmup` frames.
 
-Prints one JSON line.  `roofline` is the trace kernel's achieved fraction of
-the FP32 vector peak: exact algorithmic flops of each frame (counted-float
-oracle, see FLOPS below) over the trace kernel's HIP-event time on the
-launch stream.  `post_roofline` is bloom + tonemap against the HBM peak.
+Prints one JSON line.  `roofline` is the FP32 tile kernel's achieved
+fraction of the FP32 vector peak: the exact algorithmic flops of the pixels
+it traces (counted-float oracle, see FLOPS below) over its HIP-event time on
+the stream it runs on.  `post_roofline` is bloom + tonemap against the HBM peak.
 `fp64_error` is the exit-direction error of every pixel of each frame
 against a full-frame FP64 trace.  `surrogate` trains the MLP on 2^24 rays
 with the library's HIP kernels and times MLP + RK4 fallback on the same
@@ -31,12 +31,15 @@ WORKLOAD = "kerr09_1080p_high"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
-# Exact algorithmic flops per frame of WORKLOAD (trace kernel: camera ray,
-# integration, disk radiative transfer), from
+# Exact algorithmic flops per frame of WORKLOAD (ray integration and disk
+# radiative transfer of every pixel), and the share of the FP64 band pixels
+# (|delta| < 0.04 or sin^2 theta_min < 1e-3: 29,830 pixels, traced by the
+# FP64 band kernel, not the FP32 tile kernel), from
 #   python oracle/count_flops.py kerr09_1080p_high 0 1 2
 # (counted-float instantiation of the oracle; 1 flop per +,-,*,/,sqrt,
 # floor, transcendental).  Keyed by frame index (disk time 0, 0.5, 1).
-FLOPS = {0: 506225668732 + 80870400, 1: 506225457204 + 80870400, 2: 506228100901 + 80870400}
+FLOPS = {0: 506225668732, 1: 506225457204, 2: 506228100901}
+BAND_FLOPS = {0: 14268228423, 1: 14270951097, 2: 14272393917}
 # HBM bytes per launch of the FP32 tile kernel on WORKLOAD (FP64 band on):
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by a
 # plain-copy calibration (FETCH x2.0, WRITE x1.0); profiles/pmc_trace_kerr09_1080p_high.md
@@ -218,16 +221,19 @@ def main():
 
     lib.bhEnableKernelTimers(1)
     tr, sh, po = C.c_float(), C.c_float(), C.c_float()
-    trace_ms, shade_ms, post_ms, per_frame = [], [], [], []
+    trace_ms, shade_ms, post_ms, tile_ms, band_ms, per_frame = [], [], [], [], [], []
     bt.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
         f = i % 3
         fb.render(frames[f])
         bt.check(lib.bhLastKernelTimes(C.byref(tr), C.byref(sh), C.byref(po)), "bhLastKernelTimes")
+        tk, bk = bt.last_trace_kernel_ms()
         trace_ms.append(tr.value)
         shade_ms.append(sh.value)
         post_ms.append(po.value)
+        tile_ms.append(tk)
+        band_ms.append(bk)
         per_frame.append(f)
     bt.synchronize()
     total = time.perf_counter() - t0
@@ -237,14 +243,18 @@ def main():
     ms_per_step = total * 1e3 / a.steps
     mrays = W * H * a.steps / total / 1e6
     avg_trace = sum(trace_ms) / len(trace_ms)
+    avg_tile = sum(tile_ms) / len(tile_ms)
     roof = None
-    if FLOPS:
-        fl = sum(FLOPS[f] for f in per_frame) / len(per_frame)
-        achieved = fl / (avg_trace * 1e-3) / 1e12
+    default_band = abs(prec.critBand - 0.04) < 1e-6 and abs(prec.polarBand - 1e-3) < 1e-9
+    if FLOPS and (default_band or not prec.fp64Band):
+        # the tile kernel's algorithmic flops: every pixel's, less the band
+        # pixels' when the band kernel traces those
+        fl = sum(FLOPS[f] - (BAND_FLOPS[f] if prec.fp64Band else 0) for f in per_frame) / len(per_frame)
+        achieved = fl / (avg_tile * 1e-3) / 1e12
         roof = {"bound": "valu_fp32", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                 "traffic": TRAFFIC_BYTES if prec.fp64Band else None,
-                "kernel": "traceTileKernel", "flops_per_frame": int(fl), "trace_ms": round(avg_trace, 3),
+                "kernel": "traceTileKernel", "flops_per_launch": int(fl), "kernel_ms": round(avg_tile, 3),
                 "frac_of_nonpacked_fp32": round(achieved / FP32_NONPACKED_TFLOPS, 4)}
     res = {
         "metric": METRIC, "value": round(mrays, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
@@ -257,7 +267,9 @@ def main():
                                  "band_rays": band_rays},
                    "math_mode": "fast" if sched.mathMode else "exact",
                    "wg_size": sched.wgSize, "steps_per_chunk": sched.stepsPerChunk},
-        "kernel_ms": {"trace": round(avg_trace, 3), "shade": round(sum(shade_ms) / len(shade_ms), 3),
+        "kernel_ms": {"trace": round(avg_trace, 3), "tile_kernel": round(avg_tile, 3),
+                      "band_kernel": round(sum(band_ms) / len(band_ms), 3),
+                      "shade": round(sum(shade_ms) / len(shade_ms), 3),
                       "bloom_tonemap": round(sum(post_ms) / len(post_ms), 3)},
         "roofline": roof,
         "post_roofline": post_roofline(W, H, sum(post_ms) / len(post_ms)),

@@ -171,6 +171,7 @@ SIGNATURES = [
     ("bhSetPrecision", C.c_int, [C.POINTER(BhPrecision)]),
     ("bhLastBandStats", C.c_int, [C.POINTER(C.c_uint), C.POINTER(C.c_float), _vp]),
     ("bhLastTileOrder", C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_int), _vp]),
+    ("bhLastTraceKernelMs", C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), _vp]),
     ("bhEnableKernelTimers", None, [C.c_int]),
     ("bhLastKernelTimes", C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("bhSurrogateParamCount", C.c_int, []),
@@ -746,6 +747,13 @@ def set_precision(band: int | None = None, crit: float | None = None, polar: flo
         p.polarBand = polar
     check(load().bhSetPrecision(C.byref(p)), "bhSetPrecision")
     return p
+
+
+def last_trace_kernel_ms(stream=None):
+    """(tile kernel ms, band kernel ms) of the last timed frame."""
+    t, b = C.c_float(), C.c_float()
+    check(load().bhLastTraceKernelMs(C.byref(t), C.byref(b), stream), "bhLastTraceKernelMs")
+    return t.value, b.value
 
 
 def last_tile_order(n_max: int, stream=None):

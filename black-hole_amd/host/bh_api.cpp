@@ -114,6 +114,8 @@ struct Scratch
     unsigned* candList = nullptr;
     size_t candCap = 0;
     hipEvent_t bandEv[2] = {nullptr, nullptr};   // timing: band pass start / end
+    hipEvent_t tileEv[2] = {nullptr, nullptr};   // timing: FP32 tile kernel start / end (its own stream)
+    bool tileTimed = false;                      // tileEv hold the last timed frame
     // the FP32 tile trace runs on `aux` concurrently with the FP64 band
     // kernel on the caller's stream: fork after classification, join before
     // shading
@@ -424,11 +426,23 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     if (S.blocks < 1) S.blocks = 1;
     hipError_t e = bhk::launchResetCounter(counter, st);
     if (e != hipSuccess) return e;
-    if (!band) return sc.mathMode ? bhk::launchTraceFast(A, S, st) : bhk::launchTraceExact(A, S, st);
+    const bool timed = allowTimers && g_timers.load() != 0;
+    if (timed && !Sc->tileEv[0])
+    {
+        if (hipEventCreate(&Sc->tileEv[0]) != hipSuccess || hipEventCreate(&Sc->tileEv[1]) != hipSuccess)
+            return hipErrorOutOfMemory;
+    }
+    Sc->tileTimed = timed;
+    if (!band)
+    {
+        if (timed) (void)hipEventRecord(Sc->tileEv[0], st);
+        e = sc.mathMode ? bhk::launchTraceFast(A, S, st) : bhk::launchTraceExact(A, S, st);
+        if (timed && e == hipSuccess) (void)hipEventRecord(Sc->tileEv[1], st);
+        return e;
+    }
     // The band kernel goes first on the caller's stream so its few, long
     // FP64 wavefronts are resident before the persistent FP32 tile grid
     // fills the machine; the tile kernel follows on the auxiliary stream.
-    const bool timed = allowTimers && g_timers.load() != 0;
     if (timed && !Sc->bandEv[0])
     {
         if (hipEventCreate(&Sc->bandEv[0]) != hipSuccess || hipEventCreate(&Sc->bandEv[1]) != hipSuccess)
@@ -443,8 +457,10 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     if (timed) (void)hipEventRecord(Sc->bandEv[1], st);
     e = hipStreamWaitEvent(Sc->aux, Sc->fork, 0);
     if (e != hipSuccess) return e;
+    if (timed) (void)hipEventRecord(Sc->tileEv[0], Sc->aux);
     e = sc.mathMode ? bhk::launchTraceFast(A, S, Sc->aux) : bhk::launchTraceExact(A, S, Sc->aux);
     if (e != hipSuccess) return e;
+    if (timed) (void)hipEventRecord(Sc->tileEv[1], Sc->aux);
     e = hipEventRecord(Sc->join, Sc->aux);
     if (e != hipSuccess) return e;
     return hipStreamWaitEvent(st, Sc->join, 0);
@@ -1108,6 +1124,30 @@ bh_error_t bhLastBandStats(unsigned* count, float* bandMs, bh_stream_t stream)
     }
     return e;
 }
+bh_error_t bhLastTraceKernelMs(float* tileMs, float* bandMs, bh_stream_t stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    Scratch* S = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_scratch.find({dev, st});
+        if (it == g_scratch.end() || !it->second.tileTimed || !it->second.tileEv[0]) return hipErrorNotReady;
+        S = &it->second;
+    }
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (tileMs) e = hipEventElapsedTime(tileMs, S->tileEv[0], S->tileEv[1]);
+    if (e == hipSuccess && bandMs)
+    {
+        *bandMs = 0.f;
+        if (S->bandEv[0]) e = hipEventElapsedTime(bandMs, S->bandEv[0], S->bandEv[1]);
+    }
+    return e;
+}
+
 bh_error_t bhLastTileOrder(unsigned* keys, unsigned* order, int capacity, int* nTiles, bh_stream_t stream)
 {
     hipStream_t st = (hipStream_t)stream;

@@ -300,6 +300,11 @@ bh_error_t bhSetPrecision(const BhPrecision* p);   /* rejects negative or non-fi
 /* Band rays of the last frame traced on `stream` and, when kernel timers
  * are on, the FP64 band kernel's time in ms (synchronizes the stream). */
 bh_error_t bhLastBandStats(unsigned* count, float* bandMs, bh_stream_t stream);
+/* HIP-event times (ms) of the FP32 tile kernel, on the stream it runs on,
+ * and of the FP64 band kernel (0 when the band pass is off) of the last frame
+ * traced on `stream` with kernel timers on.  hipErrorNotReady when that
+ * frame was not timed.  Synchronizes the stream. */
+bh_error_t bhLastTraceKernelMs(float* tileMs, float* bandMs, bh_stream_t stream);
 /* Tile order of the last frame on `stream` (whole-tile schedule with the
  * band pass on, when the frame was ordered longest first): per-tile cost keys
  * (float bits of min(|delta|, 10 polar) over the tile's FP32 rays, all ones

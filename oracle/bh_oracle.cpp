@@ -422,7 +422,11 @@ void bho_trace_frame(const RenderParams* Pp, int mode, int rowStart, int rowStep
 // Exact algorithmic flop counts of one frame (rows as above):
 // out[0] trace (integration + disk), out[1] sky shading, out[2] sample setup
 // and blend, out[3] finalize (tonemap), out[4] total RK4 steps.
-void bho_count_flops(const RenderParams* Pp, int rowStart, int rowStep, int nthreads, uint64_t* out5)
+// Counted-float flops of a frame: [0] trace, [1] sky, [2] camera ray and
+// blend, [3] finalize, [4] RK4 steps, and with the FP64 band rule on, the
+// part of [0] and [4] that falls on band pixels: [5] trace flops, [6] steps,
+// [7] band pixels.  (Band pixels are counted as if traced in float.)
+void bho_count_flops(const RenderParams* Pp, int rowStart, int rowStep, int nthreads, uint64_t* out8)
 {
     const RenderParams P = *Pp;
     int nRows = nRowsOf(P.height, rowStart, rowStep);
@@ -454,12 +458,18 @@ void bho_count_flops(const RenderParams* Pp, int rowStart, int rowStep, int nthr
         a[2] += (f1 - f0) + (f4 - f3);
         a[3] += f5 - f4;
         a[4] += (uint64_t)t.steps;
+        if (pixelInBand(P, px, py, jit.x, jit.y))
+        {
+            a[5] += f2 - f1;
+            a[6] += (uint64_t)t.steps;
+            a[7] += 1;
+        }
     });
-    for (int k = 0; k < 5; ++k)
+    for (int k = 0; k < 8; ++k)
     {
         uint64_t s = 0;
         for (int t = 0; t < nthreads; ++t) s += acc[(size_t)t * 40 + k];
-        out5[k] = s;
+        out8[k] = s;
     }
 }
 

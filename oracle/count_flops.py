@@ -23,6 +23,8 @@ def main():
     name = sys.argv[1]
     frames = [int(x) for x in sys.argv[2:]] or [0]
     O = pyoracle.Oracle()
+    prec = bhtrace.get_precision()   # the library's default FP64 band rule (host setting)
+    O.set_band(prec.fp64Band, prec.critBand, prec.polarBand)
     out = {}
     for f in frames:
         P = bhtrace.workload(name, f)

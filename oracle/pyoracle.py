@@ -93,10 +93,11 @@ class Oracle:
         return m, d, pol
 
     def count_flops(self, P, row_start=0, row_step=1, threads=0):
-        out = np.zeros(5, np.uint64)
+        """Counted-float flops; with set_band on, also the band pixels' share."""
+        out = np.zeros(8, np.uint64)
         self.L.bho_count_flops(C.byref(P), row_start, row_step, threads, _ptr(out))
         return dict(trace=int(out[0]), sky=int(out[1]), setup_blend=int(out[2]), finalize=int(out[3]),
-                    steps=int(out[4]))
+                    steps=int(out[4]), band_trace=int(out[5]), band_steps=int(out[6]), band_pixels=int(out[7]))
 
     def render_pipeline(self, P, accum=None, threads=0):
         W, H = max(P.width, 1), max(P.height, 1)
