@@ -682,6 +682,21 @@ BHK_INL v3 bloomBrightPass(v3 c)
     float w = smoothstepf<MathExact>(0.95f, 2.2f, l);
     return c * w;
 }
+// (unsigned char)(clamp(powf(c, 1/2.2) + dd, 0, 1) * 255 + 0.5), with the
+// gamma first taken from the hardware log2/exp2 (relative error near 1e-7,
+// so y below moves by < 3e-4): wherever y lies farther than 1e-3 from an
+// integer its truncation is the same as with powf, and only the other
+// samples pay for powf.
+BHK_INL unsigned char quantizeGamma(float c, float dd)
+{
+#pragma clang fp contract(off)
+    constexpr float g = 1.0f / 2.2f;
+    float y = clampf(__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(c) * g) + dd, 0.f, 1.f) * 255.f + 0.5f;
+    const float fr = y - floorf(y);
+    if (fr < 1e-3f || fr > 1.f - 1e-3f) y = clampf(powf(c, g) + dd, 0.f, 1.f) * 255.f + 0.5f;
+    return (unsigned char)y;
+}
+
 BHK_INL uchar4 finalizePixel(v3 hdr, v3 bloom, const RenderParams& P, int px, int py)
 {
 #pragma clang fp contract(off)
@@ -692,13 +707,13 @@ BHK_INL uchar4 finalizePixel(v3 hdr, v3 bloom, const RenderParams& P, int px, in
     v3 lv = mk3(lum, lum, lum);
     v3 t = lv + (c - lv) * 1.12f;
     c = mk3(clampf(t.x, 0.f, 1.f), clampf(t.y, 0.f, 1.f), clampf(t.z, 0.f, 1.f));
-    c = pow3<MathExact>(c, 1.0f / 2.2f);
     v2 fp = mk2((float)px, (float)py);
     float d = (hash12(fp) + hash12(fp + mk2(41.3f, 17.7f))) - 1.0f;
     float dd = d * (1.0f / 255.0f);
-    unsigned char r8 = (unsigned char)(clampf(c.x + dd, 0.f, 1.f) * 255.f + 0.5f);
-    unsigned char g8 = (unsigned char)(clampf(c.y + dd, 0.f, 1.f) * 255.f + 0.5f);
-    unsigned char b8 = (unsigned char)(clampf(c.z + dd, 0.f, 1.f) * 255.f + 0.5f);
+    // gamma 1/2.2, dither, quantize
+    unsigned char r8 = quantizeGamma(c.x, dd);
+    unsigned char g8 = quantizeGamma(c.y, dd);
+    unsigned char b8 = quantizeGamma(c.z, dd);
     return P.swapRB ? make_uchar4(b8, g8, r8, 255) : make_uchar4(r8, g8, b8, 255);
 }
 
