@@ -346,8 +346,7 @@ static V3<float> sampleBanded(const RenderParams& P, int px, int py, float jx, f
 // eta = Carter/E^2 (tests of the band restatement).
 double bho_band_delta(float M, float a, float Q, double xi, double eta)
 {
-    RenderParams P;
-    std::memset(&P, 0, sizeof(P));
+    RenderParams P;   // defaults; only the model fields below matter
     P.model = BH_KERR_NEWMAN;
     P.M = M; P.aSpin = a; P.Qc = Q;
     return bandCriticalDelta(P, xi, eta);
