@@ -12,7 +12,8 @@ fraction of the FP32 vector peak: the exact algorithmic flops of the pixels
 it traces (counted-float oracle, see FLOPS below) over its HIP-event time on
 the stream it runs on.  `post_roofline` is bloom + tonemap against the HBM peak.
 `fp64_error` is the exit-direction error of every pixel of each frame
-against a full-frame FP64 trace.  `surrogate` trains the MLP on 2^24 rays
+against a full-frame FP64 trace.  `other_configs` times the other workloads
+(Schwarzschild 256^2, Kerr-Newman 4K, the 64-view batch).  `surrogate` trains the MLP on 2^24 rays
 with the library's HIP kernels and times MLP + RK4 fallback on the same
 frames.  `cpu_baseline` is the CPU oracle (same algorithm, g++ -O3, OpenMP)
 on a bounded sample of the same frame.
@@ -53,6 +54,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=9)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the timing of the other workloads and the 64-view batch")
     ap.add_argument("--no-fp64-error", action="store_true",
                     help="skip the per-frame exit error against a full-frame FP64 trace")
     ap.add_argument("--cpu-row-step", type=int, default=4)
@@ -174,6 +177,50 @@ def post_roofline(W, H, post_ms):
             "kernels": ["bloomDownBlurHKernel", "blurVCompositeKernel"]}
 
 
+def other_configs(bt, lib):
+    """The other workloads at their full sizes (not the headline metric):
+    trace time and frame rate of one-frame launches, and the 64-view uneven
+    batch (views rendered as consecutive pipelines on one stream)."""
+    import ctypes as C
+
+    import numpy as np
+
+    out = {}
+    for name in ("schw_256_balanced", "kn_m07_q04_2160p_high"):
+        P = bt.workload(name, 0)
+        fb = bt.Framebuffer(P.width, P.height)
+        fb.render(P)
+        bt.synchronize()
+        lib.bhEnableKernelTimers(1)
+        tr, ms = C.c_float(), []
+        t0 = time.perf_counter()
+        for _ in range(5):
+            fb.render(P)
+            lib.bhLastKernelTimes(C.byref(tr), None, None)
+            ms.append(tr.value)
+        bt.synchronize()
+        dt = (time.perf_counter() - t0) / 5
+        lib.bhEnableKernelTimers(0)
+        fb.close()
+        out[name] = {"width": P.width, "height": P.height, "ms_per_frame": round(dt * 1e3, 3),
+                     "trace_ms": round(float(np.median(ms)), 3), "mrays_s": round(P.width * P.height / dt / 1e6, 2)}
+    views = bt.workload_views("uneven_views", 1, 64)
+    B = bt.BatchBuffers(views)
+    B.render()
+    bt.synchronize()
+    t0 = time.perf_counter()
+    B.render()
+    bt.synchronize()
+    dt = time.perf_counter() - t0
+    lat = B.latency_ms()
+    rays = sum(P.width * P.height for P in views)
+    out["uneven_views_64"] = {"rays": rays, "batch_ms": round(dt * 1e3, 2), "mrays_s": round(rays / dt / 1e6, 2),
+                              "view_latency_ms_p50": round(float(np.median(lat)), 2),
+                              "view_latency_ms_max": round(float(max(lat)), 2),
+                              "launch": "consecutive single-view pipelines (persistent multi-view kernel disabled)"}
+    return out
+
+
 def cpu_baseline(P, row_step):
     """CPU oracle (float, reference semantics) on every `row_step`-th row."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -276,6 +323,8 @@ def main():
     }
     if not a.no_fp64_error:
         res["fp64_error"] = fp64_error_leg(bt, frames)
+    if not a.no_other_configs:
+        res["other_configs"] = other_configs(bt, lib)
     if a.surrogate_steps > 0:
         res["surrogate"] = surrogate_leg(bt, lib, frames, a.surrogate_steps, a.surrogate_sigma, avg_trace)
     if not a.no_cpu_baseline:
