@@ -46,6 +46,10 @@ BAND_FLOPS = {0: 14268228423, 1: 14270951097, 2: 14272393917}
 # plain-copy calibration (FETCH x2.0, WRITE x1.0); profiles/pmc_trace_kerr09_1080p_high.md
 TRAFFIC_BYTES = (7772 * 2 + 65008) * 1024
 FP32_NONPACKED_TFLOPS = 78.6
+# The same counts for frame 0 of the other workloads: (every pixel, FP64 band
+# pixels' share), from `python oracle/count_flops.py <workload> 0`.
+OTHER_FLOPS = {"schw_256_balanced": (2202725945, 22345464),
+               "kn_m07_q04_2160p_high": (1845561825607, 76329881318)}
 
 
 def parse():
@@ -177,10 +181,11 @@ def post_roofline(W, H, post_ms):
             "kernels": ["bloomDownBlurHKernel", "blurVCompositeKernel"]}
 
 
-def other_configs(bt, lib):
+def other_configs(bt, lib, fp64):
     """The other workloads at their full sizes (not the headline metric):
-    trace time and frame rate of one-frame launches, and the 64-view uneven
-    batch (views rendered as consecutive pipelines on one stream)."""
+    trace time and frame rate of one-frame launches, their exit error against
+    a full-frame FP64 trace (when `fp64`), and the 64-view uneven batch
+    (views rendered as consecutive pipelines on one stream)."""
     import ctypes as C
 
     import numpy as np
@@ -192,18 +197,31 @@ def other_configs(bt, lib):
         fb.render(P)
         bt.synchronize()
         lib.bhEnableKernelTimers(1)
-        tr, ms = C.c_float(), []
+        tr, ms, tile = C.c_float(), [], []
         t0 = time.perf_counter()
         for _ in range(5):
             fb.render(P)
             lib.bhLastKernelTimes(C.byref(tr), None, None)
             ms.append(tr.value)
+            tile.append(bt.last_trace_kernel_ms()[0])
         bt.synchronize()
         dt = (time.perf_counter() - t0) / 5
         lib.bhEnableKernelTimers(0)
         fb.close()
         out[name] = {"width": P.width, "height": P.height, "ms_per_frame": round(dt * 1e3, 3),
                      "trace_ms": round(float(np.median(ms)), 3), "mrays_s": round(P.width * P.height / dt / 1e6, 2)}
+        prec = bt.get_precision()
+        default_band = abs(prec.critBand - 0.04) < 1e-6 and abs(prec.polarBand - 1e-3) < 1e-9
+        if prec.fp64Band and default_band:
+            fl = OTHER_FLOPS[name][0] - OTHER_FLOPS[name][1]
+            tms = float(np.mean(tile))
+            ach = fl / (tms * 1e-3) / 1e12
+            out[name]["roofline"] = {"bound": "valu_fp32", "kernel": "traceTileKernel", "flops_per_launch": fl,
+                                     "kernel_ms": round(tms, 3), "achieved": round(ach, 3), "unit": "TFLOP/s",
+                                     "frac": round(ach / FP32_PEAK_TFLOPS, 4),
+                                     "frac_of_nonpacked_fp32": round(ach / FP32_NONPACKED_TFLOPS, 4)}
+        if fp64:
+            out[name]["fp64_error"] = fp64_error_leg(bt, [P])[0]
     views = bt.workload_views("uneven_views", 1, 64)
     B = bt.BatchBuffers(views)
     B.render()
@@ -221,19 +239,34 @@ def other_configs(bt, lib):
     return out
 
 
-def cpu_baseline(P, row_step):
-    """CPU oracle (float, reference semantics) on every `row_step`-th row."""
+def cpu_baseline(bt, P, row_step, others):
+    """CPU oracle (float, reference semantics) on every `row_step`-th row;
+    with `others`, also a row sample of each of the other workloads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     O = pyoracle.Oracle()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    t = time.perf_counter()
-    hdr, _, _, _ = O.trace_frame(P, 0, 0, row_step, threads)
-    dt = time.perf_counter() - t
-    rays = hdr.shape[0] * hdr.shape[1]
-    return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{WORKLOAD} t=0, every {row_step}th row ({rays} rays, {dt:.1f} s wall)"}
+
+    def sample(Q, step, reps=1):
+        t = time.perf_counter()
+        for _ in range(reps):
+            hdr, _, _, _ = O.trace_frame(Q, 0, 0, step, threads)
+        dt = time.perf_counter() - t
+        rays = hdr.shape[0] * hdr.shape[1] * reps
+        return rays / dt / 1e6, rays, dt
+
+    v, rays, dt = sample(P, row_step)
+    res = {"value": round(v, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": f"{WORKLOAD} t=0, every {row_step}th row ({rays} rays, {dt:.1f} s wall)"}
+    if others:
+        res["other_configs"] = {}
+        for name, step, reps in (("schw_256_balanced", 1, 8), ("kn_m07_q04_2160p_high", 16, 1)):
+            v, rays, dt = sample(bt.workload(name, 0), step, reps)
+            what = "every row" if step == 1 else f"every {step}th row"
+            res["other_configs"][name] = {"value": round(v, 4), "unit": "Mrays/s",
+                                          "sample": f"t=0, {what} x{reps} ({rays} rays, {dt:.1f} s wall)"}
+    return res
 
 
 def main():
@@ -324,11 +357,11 @@ def main():
     if not a.no_fp64_error:
         res["fp64_error"] = fp64_error_leg(bt, frames)
     if not a.no_other_configs:
-        res["other_configs"] = other_configs(bt, lib)
+        res["other_configs"] = other_configs(bt, lib, not a.no_fp64_error)
     if a.surrogate_steps > 0:
         res["surrogate"] = surrogate_leg(bt, lib, frames, a.surrogate_steps, a.surrogate_sigma, avg_trace)
     if not a.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(frames[0], a.cpu_row_step)
+        res["cpu_baseline"] = cpu_baseline(bt, frames[0], a.cpu_row_step, not a.no_other_configs)
     print(json.dumps(res), flush=True)
 
 
