@@ -154,6 +154,8 @@ SIGNATURES = [
     ("bhFramebufferHDR", _vp, [_vp]),
     ("bhFramebufferRender", C.c_int, [_vp, _P, _vp]),
     ("bhFramebufferReadback", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("bhPostChain", C.c_int, [_vp, _vp, _vp, _vp, _P, _vp]),
+    ("bhFramebufferPost", C.c_int, [_vp, _P, _vp]),
     ("bhGetSchedule", None, [C.POINTER(BhSchedule)]),
     ("bhSetSchedule", C.c_int, [C.POINTER(BhSchedule)]),
     ("bhAutotune", C.c_int, [_P, C.c_int, C.POINTER(BhSchedule), C.POINTER(C.c_float), _vp]),
@@ -328,6 +330,13 @@ class Framebuffer:
 
     def render(self, P: RenderParams, stream=None) -> None:
         check(load().bhFramebufferRender(self.fb, C.byref(P), stream), "bhFramebufferRender")
+
+    def post(self, P: RenderParams, stream=None) -> None:
+        """Bloom + tonemap of the framebuffer's HDR image only (no trace)."""
+        check(load().bhFramebufferPost(self.fb, C.byref(P), stream), "bhFramebufferPost")
+
+    def hdr_ptr(self) -> C.c_void_p:
+        return C.c_void_p(load().bhFramebufferHDR(self.fb))
 
     def readback(self, stream=None):
         rgba = np.empty((self.h, self.w, 4), np.uint8)

@@ -822,6 +822,48 @@ bh_error_t bhFramebufferRender(BhFramebuffer* fb, const RenderParams& p, bh_stre
     return (bh_error_t)renderPipeline(fb->color, fb->hdr, fb->bloomA, fb->bloomB, p, (hipStream_t)stream, false);
 }
 
+bh_error_t bhPostChain(bh_uchar4* out, const bh_float4* accum, bh_float4* bloomA, bh_float4* bloomB,
+                       const RenderParams& p, bh_stream_t stream)
+{
+    RenderParams P = p;
+    bhSanitizeRenderParams(&P);
+    if (!out || !accum) return hipErrorInvalidValue;
+    if (P.bloomEnabled && (!bloomA || !bloomB)) return hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    Scratch* S = nullptr;
+    hipError_t e = getScratch(st, 0, &S);
+    if (e != hipSuccess) return e;
+    const float4* acc = reinterpret_cast<const float4*>(accum);
+    uchar4* o = reinterpret_cast<uchar4*>(out);
+    const bool timed = g_timers.load() != 0;
+    if (timed) { recordEvent(S, 0, st); recordEvent(S, 1, st); recordEvent(S, 2, st); }
+    if (P.bloomEnabled)
+    {
+        float4* bA = reinterpret_cast<float4*>(bloomA);
+        float4* bB = reinterpret_cast<float4*>(bloomB);
+        e = bhk::launchBloomDownBlurH(P, acc, bB, st);
+        if (e == hipSuccess) e = bhk::launchBlurVComposite(P, acc, bB, bA, o, st);
+    }
+    else e = bhk::launchComposite(P, acc, o, st);
+    if (e != hipSuccess) return e;
+    if (timed)
+    {
+        recordEvent(S, 3, st);
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_lastTimed = {dev, st};
+    }
+    return hipGetLastError();
+}
+
+bh_error_t bhFramebufferPost(BhFramebuffer* fb, const RenderParams& p, bh_stream_t stream)
+{
+    if (!fb || p.width != fb->w || p.height != fb->h) return hipErrorInvalidValue;
+    return bhPostChain(reinterpret_cast<bh_uchar4*>(fb->color), reinterpret_cast<const bh_float4*>(fb->hdr),
+                       reinterpret_cast<bh_float4*>(fb->bloomA), reinterpret_cast<bh_float4*>(fb->bloomB), p, stream);
+}
+
 bh_error_t bhFramebufferReadback(BhFramebuffer* fb, bh_uchar4* hostColor, bh_float4* hostHdr, bh_stream_t stream)
 {
     if (!fb) return hipErrorInvalidValue;

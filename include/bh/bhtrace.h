@@ -232,6 +232,20 @@ bh_error_t bhFramebufferRender(BhFramebuffer* fb, BH_PARAMS_IN p, bh_stream_t st
 bh_error_t bhFramebufferReadback(BhFramebuffer* fb, bh_uchar4* hostColor,
                                  bh_float4* hostHdr, bh_stream_t stream);
 
+/* The post chain of launchRenderPipeline alone -- bloom (when
+ * p.bloomEnabled) and the composite/tonemap -- on an existing accumulation
+ * buffer (black_hole_kernel.cu:120-132): re-tonemaps a stored HDR image,
+ * e.g. after an exposure or bloom-strength change, without tracing.  Reads
+ * accum (not written), writes out; bloomA/bloomB as in launchRenderPipeline
+ * (required only with bloom).  p is sanitized as in launchRenderPipeline.
+ * With kernel timers on, the chain's time is reported as bhLastKernelTimes'
+ * postMs (traceMs and shadeMs read 0). */
+bh_error_t bhPostChain(bh_uchar4* out, const bh_float4* accum, bh_float4* bloomA,
+                       bh_float4* bloomB, BH_PARAMS_IN p, bh_stream_t stream);
+/* bhPostChain on the framebuffer's own buffers: its HDR image re-tonemapped
+ * into its colour buffer (p.width/height must match the framebuffer). */
+bh_error_t bhFramebufferPost(BhFramebuffer* fb, BH_PARAMS_IN p, bh_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Progressive accumulation, graph-replayed                                 */
 /* ------------------------------------------------------------------------ */

@@ -115,6 +115,37 @@ def test_post_chain_bit_exact(gpu, oracle):
     assert (d > 0).sum() <= max(1, d.size // 10000)
 
 
+@pytest.mark.parametrize("bloom", [1, 0])
+def test_post_chain_entry_retonemaps_stored_hdr(gpu, oracle, bloom):
+    """bhPostChain (through bhFramebufferPost) on the HDR image a pipeline
+    frame left behind: with the same parameters it reproduces the frame's
+    uchar4 output bit for bit; with another exposure and bloom strength it
+    matches the oracle's post chain on that image (same bar as above)."""
+    P = scaled_workload(gpu, "kerr09_1080p_high", 0, 333, 187)
+    P.bloomEnabled = bloom
+    say("post-only entry 333x187 bloom", bloom)
+    fb = gpu.Framebuffer(P.width, P.height)
+    try:
+        fb.render(P)
+        rgba0, acc = fb.readback()
+        fb.post(P)
+        rgba1, acc1 = fb.readback()
+        assert np.array_equal(rgba1, rgba0)
+        assert np.array_equal(acc1.view(np.uint32), acc.view(np.uint32))
+        Q = P.copy()
+        Q.exposure, Q.bloomStrength = 1.7, 0.6
+        fb.post(Q)
+        rgba2, _ = fb.readback()
+    finally:
+        fb.close()
+    ref = oracle.post(gpu.sanitized(Q)[0], acc)
+    d = np.abs(rgba2.astype(np.int32) - ref.astype(np.int32)).max(-1)
+    say("re-tonemapped: %d of %d pixels differ, max %d LSB" % (int((d > 0).sum()), d.size, int(d.max())))
+    assert d.max() <= 1
+    assert (d > 0).sum() <= max(1, d.size // 10000)
+    assert not np.array_equal(rgba2, rgba0)
+
+
 @pytest.mark.parametrize("math", MODES)
 def test_exit_directions_vs_float_oracle(gpu, oracle, math):
     P = scaled_workload(gpu, "kerr09_1080p_high", 0, 320, 180)
