@@ -175,6 +175,10 @@ hipError_t launchBlurVComposite(const RenderParams& P, const float4* accum, cons
                                 uchar4* out, hipStream_t st);
 // composite without bloom
 hipError_t launchComposite(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st);
+// Bloom (when P.bloomEnabled) and composite in one launch, bit-identical to
+// launchBloomDownBlurH + launchBlurVComposite (or launchComposite); writes
+// only out.
+hipError_t launchPostFused(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st);
 // batched post chain over a device ViewDesc table (max sizes bound the grids)
 hipError_t launchResetBatch(const BatchArgs& A, hipStream_t st);
 // Tile order for the FP32 tile tracer from the classification's per-tile

@@ -225,6 +225,127 @@ __global__ __launch_bounds__(256) void blurVCompositeKernel(RenderParams P, cons
     compositeTile(P, accum, srcH, dstV, out, blockIdx.x, blockIdx.y, L);
 }
 
+// ---------------------------------------------------------------- fused post chain
+// Bloom and composite in one launch.  Each workgroup owns a kPW x kPH tile of
+// output pixels and rebuilds the part of the half-resolution bloom image its
+// bilinear taps need, with its blur halos, in LDS:
+//   bright  2x2 box average of the exposed accumulation + bright pass at the
+//           (kPHX + 8) x (kPHY + 8) half-res texels under the two blurs,
+//   blurH   horizontal 9-tap of bright on the kPHX columns,
+//   blurV   vertical 9-tap of blurH on the kPHX x kPHY texels,
+// then upsamples and tonemaps its pixels.  Every value is formed with the
+// same operations, in the same order and with the same edge clamps as the
+// two-pass chain above (bloomRowSegment, compositeTile), so the frame is
+// bit-identical to it.  The halo texels are recomputed by neighbouring tiles
+// instead of going through HBM: the accumulation is read once from HBM
+// (neighbours' halo reads hit the caches) and the uchar4 frame written once,
+// the compulsory 20 B per pixel; the half-res images are not written.
+constexpr int kPW = 64, kPH = 32;
+constexpr int kPHX = kPW / 2 + 2;     // half-res columns under the bilinear taps
+constexpr int kPHY = kPH / 2 + 2;     // half-res rows under the bilinear taps
+constexpr int kPBX = kPHX + 8;        // bright-pass columns (horizontal halo)
+constexpr int kPBY = kPHY + 8;        // bright-pass / blurH rows (vertical halo)
+
+struct PostLds
+{
+    union
+    {
+        float3 bright[kPBY][kPBX];
+        float3 blurV[kPHY][kPHX];
+    } u;
+    float3 blurH[kPBY][kPHX];
+};
+
+__global__ __launch_bounds__(256) void postFusedKernel(RenderParams P, const float4* __restrict__ accum,
+                                                       uchar4* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    __shared__ PostLds L;
+    const int X0 = blockIdx.x * kPW, Y0 = blockIdx.y * kPH;
+    const int W = P.width, H = P.height;
+    const int bw = (W + 1) / 2, bh = (H + 1) / 2;
+    const int hx0 = X0 / 2 - 1, hy0 = Y0 / 2 - 1;   // first half-res tap (unclamped)
+    const bool bloom = P.bloomEnabled != 0;
+    if (bloom)
+    {
+        // bright pass at half-res (clamp(hx0 - 4 + cx), clamp(hy0 - 4 + ry))
+        for (int t = threadIdx.x; t < kPBY * kPBX; t += blockDim.x)
+        {
+            const int ry = t / kPBX, cx = t % kPBX;
+            int y = hy0 - 4 + ry, x = hx0 - 4 + cx;
+            y = y < 0 ? 0 : (y >= bh ? bh - 1 : y);
+            x = x < 0 ? 0 : (x >= bw ? bw - 1 : x);
+            const int yA = y * 2, yB = (yA + 1 < H) ? yA + 1 : yA;
+            const int xa = x * 2, xb = (xa + 1 < W) ? xa + 1 : xa;
+            const float4 a = accum[yA * W + xa], b = accum[yA * W + xb];
+            const float4 c = accum[yB * W + xa], d = accum[yB * W + xb];
+            const v3 avg = mk3(a.x + b.x + c.x + d.x, a.y + b.y + c.y + d.y, a.z + b.z + c.z + d.z) * 0.25f;
+            const v3 bp = bloomBrightPass(avg * P.exposure);
+            L.u.bright[ry][cx] = make_float3(bp.x, bp.y, bp.z);
+        }
+        __syncthreads();
+        // horizontal 9-tap at column clamp(hx0 + cx) of every bright row
+        for (int t = threadIdx.x; t < kPBY * kPHX; t += blockDim.x)
+        {
+            const int ry = t / kPHX, cx = t % kPHX;
+            int x = hx0 + cx;
+            x = x < 0 ? 0 : (x >= bw ? bw - 1 : x);
+            v3 s = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = -4; k <= 4; ++k)
+            {
+                int xs = x + k;
+                xs = xs < 0 ? 0 : (xs >= bw ? bw - 1 : xs);
+                const float3 v = L.u.bright[ry][xs - (hx0 - 4)];
+                s += mk3(v.x, v.y, v.z) * gaussW(k);
+            }
+            L.blurH[ry][cx] = make_float3(s.x, s.y, s.z);
+        }
+        __syncthreads();
+        // vertical 9-tap at row clamp(hy0 + ry) (overwrites the bright texels)
+        for (int t = threadIdx.x; t < kPHY * kPHX; t += blockDim.x)
+        {
+            const int ry = t / kPHX, cx = t % kPHX;
+            int hy = hy0 + ry;
+            hy = hy < 0 ? 0 : (hy >= bh ? bh - 1 : hy);
+            v3 s = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = -4; k <= 4; ++k)
+            {
+                int yy = hy + k;
+                yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
+                const float3 v = L.blurH[yy - (hy0 - 4)][cx];
+                s += mk3(v.x, v.y, v.z) * gaussW(k);
+            }
+            L.u.blurV[ry][cx] = make_float3(s.x, s.y, s.z);
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < kPW * kPH; t += blockDim.x)
+    {
+        const int px = X0 + (t % kPW), py = Y0 + (t / kPW);
+        if (px >= W || py >= H) continue;
+        v3 bl = mk3(0.f, 0.f, 0.f);
+        if (bloom)
+        {
+            const float x = ((float)px + 0.5f) * 0.5f - 0.5f;
+            const float y = ((float)py + 0.5f) * 0.5f - 0.5f;
+            const int ix = (int)floorf(x), iy = (int)floorf(y);
+            const float fx = x - (float)ix, fy = y - (float)iy;
+            auto tap = [&](int xx, int yy) {
+                xx = xx < 0 ? 0 : (xx >= bw ? bw - 1 : xx);
+                yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
+                const float3 v = L.u.blurV[yy - hy0][xx - hx0];
+                return mk3(v.x, v.y, v.z);
+            };
+            const v3 c00 = tap(ix, iy), c10 = tap(ix + 1, iy), c01 = tap(ix, iy + 1), c11 = tap(ix + 1, iy + 1);
+            bl = lerp3(lerp3(c00, c10, fx), lerp3(c01, c11, fx), fy);
+        }
+        const float4 av = accum[py * W + px];
+        out[py * W + px] = finalizePixel(mk3(av.x, av.y, av.z), bl, P, px, py);
+    }
+}
+
 __global__ __launch_bounds__(256) void compositeBatchKernel(const ViewDesc* views)
 {
     __shared__ CompositeLds L;
@@ -297,6 +418,13 @@ hipError_t launchBlurVComposite(const RenderParams& P, const float4* accum, cons
 {
     dim3 grid(blocksFor(P.width, kTW), blocksFor(P.height, kTH));
     hipLaunchKernelGGL(blurVCompositeKernel, grid, dim3(256), 0, st, P, accum, srcH, dstV, out);
+    return hipGetLastError();
+}
+
+hipError_t launchPostFused(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st)
+{
+    dim3 grid(blocksFor(P.width, kPW), blocksFor(P.height, kPH));
+    hipLaunchKernelGGL(postFusedKernel, grid, dim3(256), 0, st, P, accum, out);
     return hipGetLastError();
 }
 

@@ -505,13 +505,9 @@ hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bl
         e = bhk::launchShadeBlend(P, S->rec, accum, dynIndex, st);
         if (e != hipSuccess) return e;
         if (timed) recordEvent(S, 2, st);
-        if (P.bloomEnabled)
-        {
-            e = bhk::launchBloomDownBlurH(P, accum, bloomB, st);
-            if (e != hipSuccess) return e;
-            e = bhk::launchBlurVComposite(P, accum, bloomB, bloomA, out, st);
-        }
-        else e = bhk::launchComposite(P, accum, out, st);
+        // bloom + composite in one launch; bloomA/bloomB stay the caller's
+        // scratch (validated above, as the reference requires them)
+        e = bhk::launchPostFused(P, accum, out, st);
         if (timed) recordEvent(S, 3, st);
     }
     if (timed)
@@ -837,14 +833,7 @@ bh_error_t bhPostChain(bh_uchar4* out, const bh_float4* accum, bh_float4* bloomA
     uchar4* o = reinterpret_cast<uchar4*>(out);
     const bool timed = g_timers.load() != 0;
     if (timed) { recordEvent(S, 0, st); recordEvent(S, 1, st); recordEvent(S, 2, st); }
-    if (P.bloomEnabled)
-    {
-        float4* bA = reinterpret_cast<float4*>(bloomA);
-        float4* bB = reinterpret_cast<float4*>(bloomB);
-        e = bhk::launchBloomDownBlurH(P, acc, bB, st);
-        if (e == hipSuccess) e = bhk::launchBlurVComposite(P, acc, bB, bA, o, st);
-    }
-    else e = bhk::launchComposite(P, acc, o, st);
+    e = bhk::launchPostFused(P, acc, o, st);
     if (e != hipSuccess) return e;
     if (timed)
     {

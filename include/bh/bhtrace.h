@@ -35,10 +35,12 @@ typedef void* bh_stream_t;   /* == hipStream_t (opaque; 0 = null stream)      */
 /* ------------------------------------------------------------------------ */
 
 /* Full-quality frame: one sample per pixel blended into `accum`
- * (float4[W*H], read-modify-write), optional bloom through the half-resolution
- * scratch images bloomA/bloomB (float4[ceil(W/2)*ceil(H/2)] each), then
- * exposure/ACES/gamma/dither into `out` (uchar4[W*H]).  `p` is copied and
- * sanitized; the caller's copy is untouched.  Asynchronous on `stream`.
+ * (float4[W*H], read-modify-write), optional bloom, then
+ * exposure/ACES/gamma/dither into `out` (uchar4[W*H]).  bloomA/bloomB
+ * (float4[ceil(W/2)*ceil(H/2)] each) are required with bloom, as in the
+ * reference, but serve as scratch only: the bloom image is rebuilt per
+ * output tile on chip and their contents are left undefined.  `p` is copied
+ * and sanitized; the caller's copy is untouched.  Asynchronous on `stream`.
  * Replaces: extern "C" cudaError_t launchRenderPipeline(uchar4*, float4*,
  * float4*, float4*, const RenderParams&, cudaStream_t)
  * (black_hole_kernel.cu:108-134). */
@@ -237,7 +239,7 @@ bh_error_t bhFramebufferReadback(BhFramebuffer* fb, bh_uchar4* hostColor,
  * buffer (black_hole_kernel.cu:120-132): re-tonemaps a stored HDR image,
  * e.g. after an exposure or bloom-strength change, without tracing.  Reads
  * accum (not written), writes out; bloomA/bloomB as in launchRenderPipeline
- * (required only with bloom).  p is sanitized as in launchRenderPipeline.
+ * (required with bloom, contents left undefined).  p is sanitized as in launchRenderPipeline.
  * With kernel timers on, the chain's time is reported as bhLastKernelTimes'
  * postMs (traceMs and shadeMs read 0). */
 bh_error_t bhPostChain(bh_uchar4* out, const bh_float4* accum, bh_float4* bloomA,
