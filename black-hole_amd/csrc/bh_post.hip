@@ -284,37 +284,35 @@ __global__ __launch_bounds__(256) void postFusedKernel(RenderParams P, const flo
             L.u.bright[ry][cx] = make_float3(bp.x, bp.y, bp.z);
         }
         __syncthreads();
-        // horizontal 9-tap at column clamp(hx0 + cx) of every bright row
+        // horizontal 9-tap at column hx0 + cx of every bright row.  Entry c
+        // of a bright row holds the texel at clamp(hx0 - 4 + c), so for an
+        // in-image column the taps clamp(hx0 + cx + k) are entries cx + k + 4
+        // without further clamping.  Columns outside the image are computed
+        // the same way but never read: the bilinear taps below read only
+        // in-image texels.
         for (int t = threadIdx.x; t < kPBY * kPHX; t += blockDim.x)
         {
             const int ry = t / kPHX, cx = t % kPHX;
-            int x = hx0 + cx;
-            x = x < 0 ? 0 : (x >= bw ? bw - 1 : x);
             v3 s = mk3(0.f, 0.f, 0.f);
 #pragma unroll
             for (int k = -4; k <= 4; ++k)
             {
-                int xs = x + k;
-                xs = xs < 0 ? 0 : (xs >= bw ? bw - 1 : xs);
-                const float3 v = L.u.bright[ry][xs - (hx0 - 4)];
+                const float3 v = L.u.bright[ry][cx + k + 4];
                 s += mk3(v.x, v.y, v.z) * gaussW(k);
             }
             L.blurH[ry][cx] = make_float3(s.x, s.y, s.z);
         }
         __syncthreads();
-        // vertical 9-tap at row clamp(hy0 + ry) (overwrites the bright texels)
+        // vertical 9-tap at row hy0 + ry (overwrites the bright texels); row
+        // r of blurH holds clamp(hy0 - 4 + r), so the same argument applies
         for (int t = threadIdx.x; t < kPHY * kPHX; t += blockDim.x)
         {
             const int ry = t / kPHX, cx = t % kPHX;
-            int hy = hy0 + ry;
-            hy = hy < 0 ? 0 : (hy >= bh ? bh - 1 : hy);
             v3 s = mk3(0.f, 0.f, 0.f);
 #pragma unroll
             for (int k = -4; k <= 4; ++k)
             {
-                int yy = hy + k;
-                yy = yy < 0 ? 0 : (yy >= bh ? bh - 1 : yy);
-                const float3 v = L.blurH[yy - (hy0 - 4)][cx];
+                const float3 v = L.blurH[ry + k + 4][cx];
                 s += mk3(v.x, v.y, v.z) * gaussW(k);
             }
             L.u.blurV[ry][cx] = make_float3(s.x, s.y, s.z);
