@@ -106,13 +106,6 @@ BHK_INL int tileBucket(unsigned keyBits)
     return b <= 0.f ? 0 : (b >= (float)(kTileBuckets - 2) ? kTileBuckets - 2 : (int)b);
 }
 
-__global__ void tileKeyFillKernel(unsigned* tileKey, int n, unsigned* hist)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) tileKey[i] = 0xFFFFFFFFu;
-    if (i < 2 * kTileBuckets) hist[i] = 0u;
-}
-
 // counts per bucket: a workgroup histogram in LDS, then one global atomic
 // per bucket and workgroup (many tiles share a bucket)
 __global__ __launch_bounds__(kTileBuckets) void tileHistKernel(const unsigned* tileKey, int n, unsigned* hist)
@@ -167,10 +160,32 @@ __global__ __launch_bounds__(kTileBuckets) void tileScatterKernel(const unsigned
     if (i < n) order[base[b] + rank] = (unsigned)i;
 }
 
-hipError_t launchTileKeyFill(unsigned* tileKey, int nTiles, unsigned* hist, hipStream_t st)
+// Start of a band-pass frame in one launch: zero the frame's counter words
+// (tile queue head, band list length and queue head, surrogate counters)
+// and, with tile ordering, set every tile key to "no FP32 ray" and clear
+// the tile histogram.
+__global__ void frameResetKernel(unsigned* counter, int nWords, unsigned* tileKey, int nTiles, unsigned* hist)
 {
-    const int n = nTiles > 2 * kTileBuckets ? nTiles : 2 * kTileBuckets;
-    hipLaunchKernelGGL(tileKeyFillKernel, dim3((n + 255) / 256), dim3(256), 0, st, tileKey, nTiles, hist);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nWords) counter[i] = 0u;
+    if (tileKey)
+    {
+        if (i < nTiles) tileKey[i] = 0xFFFFFFFFu;
+        if (i < 2 * kTileBuckets) hist[i] = 0u;
+    }
+}
+
+hipError_t launchFrameReset(unsigned* counter, int nWords, unsigned* tileKey, int nTiles, unsigned* hist,
+                            hipStream_t st)
+{
+    int n = nWords;
+    if (tileKey)
+    {
+        n = nTiles > n ? nTiles : n;
+        n = 2 * kTileBuckets > n ? 2 * kTileBuckets : n;
+    }
+    hipLaunchKernelGGL(frameResetKernel, dim3((n + 255) / 256), dim3(256), 0, st, counter, nWords, tileKey, nTiles,
+                       hist);
     return hipGetLastError();
 }
 

@@ -184,7 +184,8 @@ hipError_t launchResetBatch(const BatchArgs& A, hipStream_t st);
 // Tile order for the FP32 tile tracer from the classification's per-tile
 // keys: fill (before classification), then a counting sort into `order`
 // (hist: 512 counters of scratch).
-hipError_t launchTileKeyFill(unsigned* tileKey, int nTiles, unsigned* hist, hipStream_t st);
+hipError_t launchFrameReset(unsigned* counter, int nWords, unsigned* tileKey, int nTiles, unsigned* hist,
+                            hipStream_t st);
 hipError_t launchTileOrder(const unsigned* tileKey, int nTiles, unsigned* hist, unsigned* order, hipStream_t st);
 hipError_t launchShadeBlendBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);
 hipError_t launchBloomBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);

@@ -371,11 +371,10 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             BA.candList = Sc->candList;
         }
         const int nTiles = BA.tilesX * ((P.height + 7) / 8);
-        e = bhk::launchResetCounter(BA.count, st);
-        if (e == hipSuccess && BA.tileKey) e = bhk::launchTileKeyFill(BA.tileKey, nTiles, Sc->tileHist, st);
-        if (e == hipSuccess) e = bhk::launchResetCounter(BA.queue, st);
-        if (e == hipSuccess) e = bhk::launchResetCounter(counter + kCandCountSlot, st);
-        if (e == hipSuccess) e = bhk::launchResetCounter(counter + kAcceptSlot, st);
+        // every counter word of the frame (the tile queue head included) and
+        // the tile keys in one launch
+        e = bhk::launchFrameReset(counter, (int)(kCounterBytes / sizeof(unsigned)), BA.tileKey, nTiles,
+                                  Sc->tileHist, st);
         if (e == hipSuccess) e = bhk::launchBandClassify(BA, st);
         if (e == hipSuccess && BA.tileKey) e = bhk::launchTileOrder(BA.tileKey, nTiles, Sc->tileHist, Sc->tileOrder, st);
         if (e != hipSuccess) return e;
@@ -424,7 +423,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     S.wgSize = sc.wgSize;
     S.blocks = (int)(want < maxUseful ? want : maxUseful);
     if (S.blocks < 1) S.blocks = 1;
-    hipError_t e = bhk::launchResetCounter(counter, st);
+    hipError_t e = band ? hipSuccess : bhk::launchResetCounter(counter, st);   // with the band: reset above
     if (e != hipSuccess) return e;
     const bool timed = allowTimers && g_timers.load() != 0;
     if (timed && !Sc->tileEv[0])
