@@ -20,6 +20,7 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
     float x[kMlpInputs];
     int tile = -1;               // tile of this pixel (-1: none)
     unsigned key = 0xFFFFFFFFu;  // its cost key (all ones: no FP32 ray)
+    unsigned bandKey = 0u;       // cost key of a band ray (0: invariants unusable)
     if (i < W * H)
     {
         const int px = i % W, py = i / W;
@@ -34,6 +35,7 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
         // polar turning point wind the longest; non-negative floats order
         // like their bit patterns
         if (!band) key = __float_as_uint((float)fmin(fabs(delta), 10.0 * polar));
+        else bandKey = ok ? __float_as_uint((float)fmin(fabs(delta), 10.0 * polar)) : 0u;
         if (A.candList && !band && delta < -A.B.critBand)
         {
             // candidate of the surrogate: escaping ingoing ray inside the training domain
@@ -72,7 +74,12 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
         unsigned base = 0;
         if (lane == 0) base = atomicAdd(A.count, (unsigned)__popcll(m));
         base = (unsigned)__shfl((int)base, 0);
-        if (band) A.list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = (unsigned)i;
+        if (band)
+        {
+            const unsigned slot = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+            A.list[slot] = (unsigned)i;
+            if (A.listKey) A.listKey[slot] = bandKey;
+        }
     }
     const unsigned long long mc = __ballot(cand);
     if (mc != 0ull)
@@ -158,6 +165,59 @@ __global__ __launch_bounds__(kTileBuckets) void tileScatterKernel(const unsigned
     if (h[t]) base[t] = atomicAdd(hist + kTileBuckets + t, h[t]);
     __syncthreads();
     if (i < n) order[base[b] + rank] = (unsigned)i;
+}
+
+// ---------------------------------------------------------------- band ray order
+// The FP64 band kernel hands its rays out from a queue, 64 to a wavefront.
+// Sorted longest first (smallest key: closest to the critical curve or a
+// polar turning point), each wavefront holds rays of similar length: only
+// the first few run as long as the longest ray, the rest end early and give
+// their SIMD slots (each resident band wavefront displaces one wavefront of
+// the FP32 tile grid) back to the tile kernel.  One workgroup
+// counting-sorts the list into 256 buckets of 1/8 octave of the key
+// (2^-24 .. 2^8); order within a bucket is arbitrary, which changes no
+// result (rays are independent).
+constexpr int kBandBuckets = 256;
+
+BHK_INL int bandBucket(unsigned keyBits)
+{
+    const float k = __uint_as_float(keyBits);
+    if (!(k < 3.0e38f)) return kBandBuckets - 1;
+    const float b = (log2f(fmaxf(k, 1e-30f)) + 24.f) * 8.f;
+    return b <= 0.f ? 0 : (b >= (float)(kBandBuckets - 1) ? kBandBuckets - 1 : (int)b);
+}
+
+__global__ __launch_bounds__(1024) void bandSortKernel(const unsigned* count, const unsigned* list,
+                                                       const unsigned* key, unsigned* sorted)
+{
+    __shared__ unsigned h[kBandBuckets], s[kBandBuckets];
+    const int t = threadIdx.x;
+    const unsigned n = *count;
+    if (t < kBandBuckets) h[t] = 0u;
+    __syncthreads();
+    for (unsigned i = t; i < n; i += blockDim.x) atomicAdd(h + bandBucket(key[i]), 1u);
+    __syncthreads();
+    // exclusive scan of the bucket counts (inclusive Hillis-Steele, then
+    // shifted), every thread at every barrier
+    unsigned v = t < kBandBuckets ? h[t] : 0u;
+    if (t < kBandBuckets) s[t] = v;
+    __syncthreads();
+    for (int d = 1; d < kBandBuckets; d <<= 1)
+    {
+        const unsigned add = (t < kBandBuckets && t >= d) ? s[t - d] : 0u;
+        __syncthreads();
+        if (t < kBandBuckets) s[t] += add;
+        __syncthreads();
+    }
+    if (t < kBandBuckets) h[t] = s[t] - v;   // cursor: first slot of bucket t
+    __syncthreads();
+    for (unsigned i = t; i < n; i += blockDim.x) sorted[atomicAdd(h + bandBucket(key[i]), 1u)] = list[i];
+}
+
+hipError_t launchBandSort(const BandArgs& A, hipStream_t st)
+{
+    hipLaunchKernelGGL(bandSortKernel, dim3(1), dim3(1024), 0, st, A.count, A.list, A.listKey, A.sorted);
+    return hipGetLastError();
 }
 
 // Start of a band-pass frame in one launch: zero the frame's counter words

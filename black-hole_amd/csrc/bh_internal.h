@@ -38,6 +38,8 @@ struct BandArgs
     BhRayRecord* rec;       // [W*H]
     unsigned char* mask;    // [W*H]
     unsigned* list;         // [W*H] band pixel indices
+    unsigned* listKey;      // [W*H] cost key of each listed ray (or null: no ordering)
+    unsigned* sorted;       // [W*H] the list, costliest rays first (or null: trace in list order)
     unsigned* count;        // list length (zero before classification)
     unsigned* queue;        // band-trace queue head (zero before the band trace)
     unsigned* tileKey;      // [tiles] cost key of each 8x8 tile (all ones before classification, or null)
@@ -186,6 +188,7 @@ hipError_t launchResetBatch(const BatchArgs& A, hipStream_t st);
 // (hist: 512 counters of scratch).
 hipError_t launchFrameReset(unsigned* counter, int nWords, unsigned* tileKey, int nTiles, unsigned* hist,
                             hipStream_t st);
+hipError_t launchBandSort(const BandArgs& A, hipStream_t st);
 hipError_t launchTileOrder(const unsigned* tileKey, int nTiles, unsigned* hist, unsigned* order, hipStream_t st);
 hipError_t launchShadeBlendBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);
 hipError_t launchBloomBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);

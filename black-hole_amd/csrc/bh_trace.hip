@@ -235,6 +235,10 @@ template <class Ray>
 __global__ __launch_bounds__(64) void traceBandKernel(BandArgs A)
 {
     const unsigned n = *A.count;
+    // rays sorted longest first: the wavefronts that take the first chunks
+    // run longest, the later ones carry short rays and soon give their
+    // SIMD slots back to the FP32 tile grid
+    const unsigned* list = A.sorted ? A.sorted : A.list;
     // band rays are the longest of the frame and run beside the FP32 tile
     // kernel: give their wavefronts issue priority so the frame is not
     // waiting on them
@@ -263,7 +267,7 @@ __global__ __launch_bounds__(64) void traceBandKernel(BandArgs A)
                 if (id >= n) drained = true;
                 else
                 {
-                    pix = A.list[id];
+                    pix = list[id];
                     const int px = (int)(pix % (unsigned)A.P.width), py = (int)(pix / (unsigned)A.P.width);
                     R.init(A.B, o, cameraDir64(A.P, px, py, jit));
                     active = true;

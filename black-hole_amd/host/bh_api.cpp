@@ -100,6 +100,8 @@ struct Scratch
     // FP64 band pass: per-pixel mask, band pixel list (count in counter[16])
     unsigned char* bandMask = nullptr;
     unsigned* bandList = nullptr;
+    unsigned* bandKey = nullptr;          // [W*H] cost key per listed band ray
+    unsigned* bandSorted = nullptr;       // [W*H] band list, longest rays first
     size_t bandCap = 0;
     // tile order of the FP32 tile tracer (longest tiles first): per-tile
     // keys, the order, and 512 counters of the counting sort
@@ -286,10 +288,15 @@ hipError_t getBandScratch(Scratch* S, size_t n, hipStream_t st)
         if (e != hipSuccess) return e;
         (void)hipFree(S->bandMask);
         (void)hipFree(S->bandList);
-        S->bandMask = nullptr; S->bandList = nullptr; S->bandCap = 0;
+        (void)hipFree(S->bandKey);
+        (void)hipFree(S->bandSorted);
+        S->bandMask = nullptr; S->bandList = nullptr; S->bandKey = nullptr; S->bandSorted = nullptr;
+        S->bandCap = 0;
     }
     e = hipMalloc(&S->bandMask, n);
     if (e == hipSuccess) e = hipMalloc(&S->bandList, n * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&S->bandKey, n * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&S->bandSorted, n * sizeof(unsigned));
     if (e != hipSuccess) return e;
     S->bandCap = n;
     if (!S->aux)
@@ -340,6 +347,8 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.rec = rec;
         BA.mask = Sc->bandMask;
         BA.list = Sc->bandList;
+        BA.listKey = Sc->bandKey;
+        BA.sorted = Sc->bandSorted;
         BA.count = counter + kBandCountSlot;
         BA.queue = counter + kBandQueueSlot;
         BA.tilesX = (P.width + 7) / 8;
@@ -376,6 +385,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         e = bhk::launchFrameReset(counter, (int)(kCounterBytes / sizeof(unsigned)), BA.tileKey, nTiles,
                                   Sc->tileHist, st);
         if (e == hipSuccess) e = bhk::launchBandClassify(BA, st);
+        if (e == hipSuccess) e = bhk::launchBandSort(BA, st);
         if (e == hipSuccess && BA.tileKey) e = bhk::launchTileOrder(BA.tileKey, nTiles, Sc->tileHist, Sc->tileOrder, st);
         if (e != hipSuccess) return e;
         if (sur.s)
