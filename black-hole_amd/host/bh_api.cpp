@@ -127,7 +127,11 @@ struct Scratch
 
 std::mutex g_mu;
 std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
-BhSchedule g_sched = {256, 4096, 64, 1, 0};   // wgSize, stepsPerChunk, raysPerRefill, mathMode, diskCoop
+// wgSize, stepsPerChunk, raysPerRefill, mathMode, diskCoop.  One-wavefront
+// workgroups: a resident FP64 band wavefront then takes exactly one
+// wavefront slot from the tile grid (a 4-wavefront workgroup cannot start
+// on a CU whose free slots do not take all four).
+BhSchedule g_sched = {64, 4096, 64, 1, 0};
 BhPrecision g_prec = {1, 4e-2f, 1e-3f};
 constexpr int kBandCountSlot = 16;            // counter[16]: band list length (own 64-byte line)
 constexpr int kCandCountSlot = 32;            // counter[32]: surrogate candidates
