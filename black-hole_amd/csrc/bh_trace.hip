@@ -231,8 +231,13 @@ __global__ __launch_bounds__(256) void traceBatchKernel(const ViewDesc* __restri
 constexpr int kBandChunk = 32;
 constexpr int kBandRefill = 16;
 
+// At most 256 VGPRs (two wavefronts per SIMD): a band wavefront shares its
+// SIMD with the FP32 tile grid, whose wavefronts hold 128 VGPRs each, so the
+// band wavefront's allocation sets how many tile wavefronts it displaces --
+// two at 256 (a few FP64 values spill to scratch), three at the 328 the
+// compiler picks unbounded.
 template <class Ray>
-__global__ __launch_bounds__(64) void traceBandKernel(BandArgs A)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void traceBandKernel(BandArgs A)
 {
     const unsigned n = *A.count;
     // rays sorted longest first: the wavefronts that take the first chunks
