@@ -409,10 +409,12 @@ def render_kernel(P: RenderParams) -> np.ndarray:
     return out.download(np.uint8).reshape(Q.height, Q.width, 4)
 
 
-def trace_records(P: RenderParams) -> np.ndarray:
+def trace_records(P: RenderParams, buf: "DeviceBuffer | None" = None) -> np.ndarray:
+    """Records of one trace of P (into `buf` when given, else a new buffer)."""
     lib = load()
     Q, _ = sanitized(P)
-    buf = DeviceBuffer(Q.width * Q.height * RECORD_DTYPE.itemsize)
+    if buf is None:
+        buf = DeviceBuffer(Q.width * Q.height * RECORD_DTYPE.itemsize)
     check(lib.bhTraceRecords(buf.ptr, C.byref(P), None), "bhTraceRecords")
     synchronize()
     return buf.download(RECORD_DTYPE).reshape(Q.height, Q.width)

@@ -31,11 +31,22 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
         band = !ok || (!(A.B.kerr == 0 && delta == 1.0) && (fabs(delta) < A.B.critBand || polar < A.B.polarBand));
         A.mask[i] = band ? 1 : 0;
         tile = (py >> 3) * A.tilesX + (px >> 3);
-        // cost key of the pixel's tile: rays near the critical curve or a
-        // polar turning point wind the longest; non-negative floats order
-        // like their bit patterns
-        if (!band) key = __float_as_uint((float)fmin(fabs(delta), 10.0 * polar));
-        else bandKey = ok ? __float_as_uint((float)fmin(fabs(delta), 10.0 * polar)) : 0u;
+        // cost key of the ray (smaller: longer), reduced to the tile's
+        // smallest.  For a view traced again, the previous frame's RK4 step
+        // count of the pixel, as 64 / (steps + 1); otherwise the band
+        // statistics: rays near the critical curve or a polar turning point
+        // wind the longest.  Non-negative floats order like their bit
+        // patterns.  Keys only order work: any key gives the same frame.
+        unsigned cost;
+        if (A.prevRec)
+        {
+            int steps = reinterpret_cast<const int*>(A.prevRec + i)[3] >> 8;
+            steps = steps < 0 ? 0 : (steps > (1 << 20) ? (1 << 20) : steps);
+            cost = __float_as_uint(64.0f / (float)(steps + 1));
+        }
+        else cost = ok ? __float_as_uint((float)fmin(fabs(delta), 10.0 * polar)) : 0u;
+        if (!band) key = cost;
+        else bandKey = cost;
         if (A.candList && !band && delta < -A.B.critBand)
         {
             // candidate of the surrogate: escaping ingoing ray inside the training domain

@@ -40,6 +40,8 @@ struct BandArgs
     unsigned* list;         // [W*H] band pixel indices
     unsigned* listKey;      // [W*H] cost key of each listed ray (or null: no ordering)
     unsigned* sorted;       // [W*H] the list, costliest rays first (or null: trace in list order)
+    const BhRayRecord* prevRec;     // [W*H] the previous frame's records of this same view: cost keys from
+                                    // their step counts (or null: keys from the band statistics)
     unsigned* count;        // list length (zero before classification)
     unsigned* queue;        // band-trace queue head (zero before the band trace)
     unsigned* tileKey;      // [tiles] cost key of each 8x8 tile (all ones before classification, or null)

@@ -102,6 +102,11 @@ struct Scratch
     unsigned* bandList = nullptr;
     unsigned* bandKey = nullptr;          // [W*H] cost key per listed band ray
     unsigned* bandSorted = nullptr;       // [W*H] band list, longest rays first
+    // the last traced view (RenderParams less the per-frame sample index and
+    // disk time) and the record buffer it wrote: a frame of the same view
+    // orders its work by those records' step counts
+    RenderParams lastView{};
+    const BhRayRecord* lastRec = nullptr;
     size_t bandCap = 0;
     // tile order of the FP32 tile tracer (longest tiles first): per-tile
     // keys, the order, and 512 counters of the counting sort
@@ -353,6 +358,19 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.list = Sc->bandList;
         BA.listKey = Sc->bandKey;
         BA.sorted = Sc->bandSorted;
+        {
+            // same view as the last frame traced into `rec`?  (Under graph
+            // capture always: replays repeat this view.)
+            RenderParams v = P;
+            v.sampleIndex = 0;
+            v.diskTime = 0.f;
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            const bool capturing = hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+            const bool same = Sc->lastRec == rec && std::memcmp(&v, &Sc->lastView, sizeof(v)) == 0;
+            BA.prevRec = (same || capturing) ? rec : nullptr;
+            Sc->lastView = v;
+            Sc->lastRec = rec;
+        }
         BA.count = counter + kBandCountSlot;
         BA.queue = counter + kBandQueueSlot;
         BA.tilesX = (P.width + 7) / 8;

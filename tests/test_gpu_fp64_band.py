@@ -129,8 +129,12 @@ def test_tile_order_keys_and_permutation(gpu, oracle, band_on, name, frame, w, h
     """The tile queue of a band-on frame: each tile's key is the smallest
     min(|delta|, 10 polar) over its FP32 rays (the oracle's FP64 band
     statistics), the order is a permutation of the tiles, and it visits the
-    256 log2-key buckets in ascending order."""
+    256 log2-key buckets in ascending order.  (A frame of a view not traced
+    just before: a repeated view is keyed by its step counts, below.)"""
     P = scaled_workload(gpu, name, frame, w, h)
+    other = P.copy()
+    other.exposure = P.exposure + 0.5
+    gpu.trace_records(other)
     gpu.trace_records(P)
     tx, ty = (w + 7) // 8, (h + 7) // 8
     got = gpu.last_tile_order(tx * ty)
@@ -158,3 +162,39 @@ def test_tile_order_keys_and_permutation(gpu, oracle, band_on, name, frame, w, h
     # ascending buckets along the queue (1-bucket slack for float log2)
     assert (np.diff(bo) >= -1).all()
     say(name, "tiles", tx * ty, "ordered; first buckets", bo[:4], "last", bo[-4:])
+
+
+@pytest.mark.parametrize("name,frame,w,h", [("kerr09_1080p_high", 0, 480, 270), ("kn_m07_q04_2160p_high", 0, 333, 187)])
+def test_tile_order_from_previous_steps(gpu, oracle, band_on, name, frame, w, h):
+    """A view traced again into the same records: each tile's key is the
+    smallest 64 / (steps + 1) over its FP32 rays, from the step counts the
+    previous frame left in the records (the most steps first), and the
+    second frame's records equal the first's (work order changes no result)."""
+    P = scaled_workload(gpu, name, frame, w, h)
+    Q, _ = gpu.sanitized(P)
+    buf = gpu.DeviceBuffer(Q.width * Q.height * gpu.RECORD_DTYPE.itemsize)
+    try:
+        r1 = gpu.trace_records(P, buf).copy()
+        r2 = gpu.trace_records(P, buf)
+    finally:
+        buf.close()
+    assert np.array_equal(r1.view(np.uint8), r2.view(np.uint8))
+    tx, ty = (w + 7) // 8, (h + 7) // 8
+    got = gpu.last_tile_order(tx * ty)
+    assert got is not None, "frame was not ordered"
+    keys, order = got
+    assert np.array_equal(np.sort(order), np.arange(tx * ty))
+    m, _, _ = oracle.band_mask(P, band_on.critBand, band_on.polarBand)
+    steps = np.clip(r1["flags"] >> 8, 0, 1 << 20).astype(np.float32)
+    k = np.where(m > 0, np.inf, np.float32(64.0) / (steps + np.float32(1.0))).astype(np.float32)
+    pad = np.full((ty * 8, tx * 8), np.inf, np.float32)
+    pad[:h, :w] = k
+    want = pad.reshape(ty, 8, tx, 8).min(axis=(1, 3)).ravel()
+    gk = keys.view(np.float32).copy()
+    gk[keys == 0xFFFFFFFF] = np.inf
+    assert np.array_equal(gk, want)
+    b = (np.log2(np.maximum(gk[order], 1e-30)) + 8.0) * 16.0
+    bo = np.clip(b, 0, 254).astype(int)
+    bo[~np.isfinite(gk[order])] = 255
+    assert (np.diff(bo) >= -1).all()
+    say(name, "tiles", tx * ty, "ordered by previous steps; first buckets", bo[:4], "last", bo[-4:])
