@@ -142,10 +142,6 @@ constexpr int kBandCountSlot = 16;            // counter[16]: band list length (
 constexpr int kCandCountSlot = 32;            // counter[32]: surrogate candidates
 constexpr int kAcceptSlot = 48;               // counter[48]: rays answered by the surrogate
 constexpr int kBandQueueSlot = 64;            // counter[64]: band-trace queue head
-// Tiles are ordered longest first when a frame has at most this many tiles
-// per wavefront of the persistent grid (1080p: 8; 2160p: 32, where the tail
-// is already short and the ordering gains nothing).
-constexpr long long kOrderTilesPerWave = 16;
 constexpr size_t kCounterBytes = 512;
 // learned surrogate attached to the render path (null: off)
 struct SurrogateRoute
@@ -377,11 +373,10 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.tileKey = nullptr;
         Sc->lastOrderTiles = 0;
         const long long nT = (long long)BA.tilesX * ((P.height + 7) / 8);
-        if (tiles && nT <= kOrderTilesPerWave * want * (sc.wgSize / 64))
+        if (tiles)
         {
-            // whole-tile schedule with few tiles per wavefront, where the last
-            // tiles set the frame time: order the tiles by the
-            // classification's cost keys, longest first
+            // whole-tile schedule: the queue hands out the costliest tiles
+            // first, by the classification's cost keys
             e = getTileScratch(Sc, (size_t)nT, st);
             if (e != hipSuccess) return e;
             BA.tileKey = Sc->tileKey;
