@@ -172,16 +172,8 @@ hipError_t launchBumpIndex(int* dynIndex, hipStream_t st);
 hipError_t launchSetIndex(int* dynIndex, int value, hipStream_t st);
 // ... or records -> HDR sample -> tonemapped pixel (single pass, no bloom)
 hipError_t launchShadeFinalize(const RenderParams& P, const BhRayRecord* rec, uchar4* out, hipStream_t st);
-// accum -> half-res bright pass + horizontal blur (into dstH)
-hipError_t launchBloomDownBlurH(const RenderParams& P, const float4* accum, float4* dstH, hipStream_t st);
-// vertical blur of srcH (into dstV, the final bloom image) fused with the composite
-hipError_t launchBlurVComposite(const RenderParams& P, const float4* accum, const float4* srcH, float4* dstV,
-                                uchar4* out, hipStream_t st);
-// composite without bloom
-hipError_t launchComposite(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st);
 // Bloom (when P.bloomEnabled) and composite in one launch, bit-identical to
-// launchBloomDownBlurH + launchBlurVComposite (or launchComposite); writes
-// only out.
+// the reference's downsample, blur and composite launches; writes only out.
 hipError_t launchPostFused(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st);
 // batched post chain over a device ViewDesc table (max sizes bound the grids)
 hipError_t launchResetBatch(const BatchArgs& A, hipStream_t st);

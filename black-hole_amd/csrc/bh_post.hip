@@ -5,14 +5,16 @@
 //   shade      trace record -> sky (4 star lattices + Milky-Way band, once per
 //              escaped ray) -> HDR sample -> accumulation blend   (pixel order,
 //              full wavefronts; trace.cuh:908-926, black_hole_kernel.cu:22-39)
-//   bloom      half-resolution bright pass fused with the horizontal 9-tap blur
-//              (one LDS row segment per workgroup), then the vertical blur fused
-//              into the composite through a 2-D LDS tile
-//              (black_hole_kernel.cu:41-103, post_process.cuh:76-148).
+//   post       bloom and composite in one launch: each workgroup rebuilds the
+//              half-resolution bloom image under its output tile in LDS
+//              (bright pass, horizontal and vertical 9-tap blurs) and
+//              tonemaps its pixels (black_hole_kernel.cu:41-103,
+//              post_process.cuh:76-148).  The batched views use the same
+//              steps as two passes through the half-resolution images.
 //
 // HBM traffic per full-resolution pixel (float4 accum, uchar4 out):
 //   shade 32 (record) + 16 (accum read) + 16 (accum write)
-//   bloom 16 (accum read) + 4 (half-res write); composite 16 + ~4 + 4 (out).
+//   post  16 (accum read) + 4 (out).
 // ---------------------------------------------------------------------------
 #include "bh_common.h"
 #include "bh_internal.h"
@@ -119,13 +121,6 @@ BHK_INL void bloomRowSegment(const RenderParams& P, const float4* accum, float4*
     dstH[y * bw + x] = make_float4(s.x, s.y, s.z, 1.f);
 }
 
-__global__ __launch_bounds__(kSegX) void bloomDownBlurHKernel(RenderParams P, const float4* accum, float4* dstH,
-                                                             int bw, int bh)
-{
-    __shared__ float3 row[kSegX + 8];
-    bloomRowSegment(P, accum, dstH, bw, bh, blockIdx.x, blockIdx.y, row);
-}
-
 __global__ __launch_bounds__(kSegX) void bloomBatchKernel(const ViewDesc* views)
 {
     __shared__ float3 row[kSegX + 8];
@@ -216,13 +211,6 @@ BHK_INL void compositeTile(const RenderParams& P, const float4* accum, const flo
         float4 av = accum[py * P.width + px];
         out[py * P.width + px] = finalizePixel(mk3(av.x, av.y, av.z), bl, P, px, py);
     }
-}
-
-__global__ __launch_bounds__(256) void blurVCompositeKernel(RenderParams P, const float4* accum, const float4* srcH,
-                                                            float4* dstV, uchar4* out)
-{
-    __shared__ CompositeLds L;
-    compositeTile(P, accum, srcH, dstV, out, blockIdx.x, blockIdx.y, L);
 }
 
 // ---------------------------------------------------------------- fused post chain
@@ -403,34 +391,11 @@ hipError_t launchShadeFinalize(const RenderParams& P, const BhRayRecord* rec, uc
     return hipGetLastError();
 }
 
-hipError_t launchBloomDownBlurH(const RenderParams& P, const float4* accum, float4* dstH, hipStream_t st)
-{
-    int bw = (P.width + 1) / 2, bh = (P.height + 1) / 2;
-    dim3 grid(blocksFor(bw, kSegX), bh);
-    hipLaunchKernelGGL(bloomDownBlurHKernel, grid, dim3(kSegX), 0, st, P, accum, dstH, bw, bh);
-    return hipGetLastError();
-}
-
-hipError_t launchBlurVComposite(const RenderParams& P, const float4* accum, const float4* srcH, float4* dstV,
-                                uchar4* out, hipStream_t st)
-{
-    dim3 grid(blocksFor(P.width, kTW), blocksFor(P.height, kTH));
-    hipLaunchKernelGGL(blurVCompositeKernel, grid, dim3(256), 0, st, P, accum, srcH, dstV, out);
-    return hipGetLastError();
-}
-
 hipError_t launchPostFused(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st)
 {
     dim3 grid(blocksFor(P.width, kPW), blocksFor(P.height, kPH));
     hipLaunchKernelGGL(postFusedKernel, grid, dim3(256), 0, st, P, accum, out);
     return hipGetLastError();
-}
-
-hipError_t launchComposite(const RenderParams& P, const float4* accum, uchar4* out, hipStream_t st)
-{
-    RenderParams Q = P;
-    Q.bloomEnabled = 0;
-    return launchBlurVComposite(Q, accum, nullptr, nullptr, out, st);
 }
 
 hipError_t launchResetBatch(const BatchArgs& A, hipStream_t st)
