@@ -610,7 +610,10 @@ BHK_INL float2 launchJitter(const RenderParams& P, float2 hostJit, const int* dy
 }
 
 // ------------------------------------------------------------------ sky (trace.cuh:131-195), exact arithmetic
-BHK_INL v3 starLayer(v3 d, float cellScale, float intensity, float coreSharp)
+// One star lattice (trace.cuh:131-165), in two parts: the probe (the
+// direction's cell, its star offset and magnitude, and whether the direction
+// lies inside the star's core) and the shading of a star that it hits.
+BHK_INL bool starLayerProbe(v3 d, float cellScale, float coreSharp, float& d2o, float& mago, float& tselo)
 {
 #pragma clang fp contract(off)
     v3 p = d * cellScale;
@@ -622,11 +625,18 @@ BHK_INL v3 starLayer(v3 d, float cellScale, float intensity, float coreSharp)
     float d2 = dot(dv, dv);
     float mag = hash12(mk2(id.x + 91.7f * id.z, id.y - 33.1f * id.x));
     float cut = (mag > 0.9f) ? 60.f : 14.f;
-    if (d2 * coreSharp > cut) return mk3(0.f, 0.f, 0.f);
+    d2o = d2;
+    mago = mag;
+    tselo = hash12(mk2(id.y + 7.3f, id.z - 3.1f));
+    return !(d2 * coreSharp > cut);
+}
+
+BHK_INL v3 starShade(float d2, float mag, float tsel, float intensity, float coreSharp)
+{
+#pragma clang fp contract(off)
     float bright = 0.05f * powf(mag, 9.f) + 9.0f * powf(mag, 50.f);
     float core = expf(-d2 * coreSharp);
     float glow = expf(-d2 * coreSharp * 0.10f);
-    float tsel = hash12(mk2(id.y + 7.3f, id.z - 3.1f));
     float kelvin = 2600.f + 12000.f * tsel * tsel;
     v3 col = blackbody<MathExact>(kelvin);
     float lum = 0.2126f * col.x + 0.7152f * col.y + 0.0722f * col.z;
@@ -634,16 +644,20 @@ BHK_INL v3 starLayer(v3 d, float cellScale, float intensity, float coreSharp)
     return col * ((core + 0.08f * glow) * bright * intensity);
 }
 
-BHK_INL v3 backgroundColor(v3 dir, float gCam)
+BHK_INL v3 starLayer(v3 d, float cellScale, float intensity, float coreSharp)
+{
+    float d2, mag, tsel;
+    if (!starLayerProbe(d, cellScale, coreSharp, d2, mag, tsel)) return mk3(0.f, 0.f, 0.f);
+    return starShade(d2, mag, tsel, intensity, coreSharp);
+}
+
+// Milky-Way band, dust lanes and airglow added to the star sum, scaled by
+// the observer's energy factor (trace.cuh:181-195).
+BHK_INL v3 skyDiffuse(v3 c, v3 d, float gCam)
 {
 #pragma clang fp contract(off)
-    v3 d = normalize<MathExact>(dir);
     float Ts = clampf(gCam, 0.5f, 3.f);
     float Is = Ts * Ts * Ts * Ts;
-    v3 c = starLayer(d, 80.f, 1.00f, 320.f);
-    c += starLayer(d, 190.f, 0.55f, 520.f);
-    c += starLayer(d, 420.f, 0.30f, 820.f);
-    c += starLayer(d, 900.f, 0.14f, 1500.f);
     float s = 3.0f / (1.35f + fabsf(d.y));
     v2 q = mk2(d.x * s, d.z * s);
     float neb = fbm(q);
@@ -653,6 +667,97 @@ BHK_INL v3 backgroundColor(v3 dir, float gCam)
     c += (mk3(0.020f, 0.024f, 0.038f) + mk3(0.030f, 0.022f, 0.014f) * neb) * (neb * band * lanes);
     c += mk3(0.0012f, 0.0015f, 0.0026f);
     return c * Is;
+}
+
+BHK_INL v3 backgroundColor(v3 dir, float gCam)
+{
+#pragma clang fp contract(off)
+    v3 d = normalize<MathExact>(dir);
+    v3 c = starLayer(d, 80.f, 1.00f, 320.f);
+    c += starLayer(d, 190.f, 0.55f, 520.f);
+    c += starLayer(d, 420.f, 0.30f, 820.f);
+    c += starLayer(d, 900.f, 0.14f, 1500.f);
+    return skyDiffuse(c, d, gCam);
+}
+
+// backgroundColor for a whole wavefront (every lane calls it; lanes with
+// `active` false get an unused value).  A direction hits a star's core in a
+// few percent of the pixels per lattice, so the star shading, evaluated in
+// place, would run four times per wavefront with most lanes idle.  Here the
+// hits of all lanes and lattices are numbered in lane order and shaded 64 at
+// a time, one per lane; each lane then adds its own hits in lattice order.
+// Every value is formed by the same operations as in backgroundColor, and a
+// missing star adds +0 there, so the result is bit-identical.
+BHK_INL v3 backgroundColorWave(v3 dir, float gCam, bool active)
+{
+#pragma clang fp contract(off)
+    v3 d = normalize<MathExact>(dir);
+    float d2_0, d2_1, d2_2, d2_3, m_0, m_1, m_2, m_3, t_0, t_1, t_2, t_3;
+    unsigned hits = 0u;   // bit L: the direction is inside a star core of lattice L
+    if (starLayerProbe(d, 80.f, 320.f, d2_0, m_0, t_0)) hits |= 1u;
+    if (starLayerProbe(d, 190.f, 520.f, d2_1, m_1, t_1)) hits |= 2u;
+    if (starLayerProbe(d, 420.f, 820.f, d2_2, m_2, t_2)) hits |= 4u;
+    if (starLayerProbe(d, 900.f, 1500.f, d2_3, m_3, t_3)) hits |= 8u;
+    if (!active) hits = 0u;
+    const int lane = (int)__lane_id();
+    const int n = __popc(hits);
+    int incl = n;
+#pragma unroll
+    for (int sh = 1; sh < 64; sh <<= 1)
+    {
+        const int t = __shfl_up(incl, sh);
+        if (lane >= sh) incl += t;
+    }
+    const int total = __shfl(incl, 63);
+    const int start = incl - n;
+    v3 c = mk3(0.f, 0.f, 0.f);
+    for (int base = 0; base < total; base += 64)
+    {
+        // hit k = base + lane: its owner is the first lane whose inclusive
+        // count exceeds k, its lattice the owner's (k - start)-th set bit
+        const int k = base + lane;
+        int lo = 0, hi = 63;
+#pragma unroll
+        for (int it = 0; it < 6; ++it)
+        {
+            const int mid = (lo + hi) >> 1;
+            if (__shfl(incl, mid) > k) hi = mid;
+            else lo = mid + 1;
+        }
+        const int owner = lo;
+        unsigned m = (unsigned)__shfl((int)hits, owner);
+        int j = k - __shfl(start, owner);
+        j = j < 0 ? 0 : (j > 3 ? 3 : j);
+        for (int q = 0; q < j; ++q) m &= m - 1u;
+        const int L = m ? __ffs((int)m) - 1 : 0;
+        const float o0 = __shfl(d2_0, owner), o1 = __shfl(d2_1, owner), o2 = __shfl(d2_2, owner),
+                    o3 = __shfl(d2_3, owner);
+        const float g0 = __shfl(m_0, owner), g1 = __shfl(m_1, owner), g2 = __shfl(m_2, owner),
+                    g3 = __shfl(m_3, owner);
+        const float s0 = __shfl(t_0, owner), s1 = __shfl(t_1, owner), s2 = __shfl(t_2, owner),
+                    s3 = __shfl(t_3, owner);
+        v3 col = mk3(0.f, 0.f, 0.f);
+        if (k < total)
+        {
+            const float hd2 = L == 0 ? o0 : L == 1 ? o1 : L == 2 ? o2 : o3;
+            const float hm = L == 0 ? g0 : L == 1 ? g1 : L == 2 ? g2 : g3;
+            const float ht = L == 0 ? s0 : L == 1 ? s1 : L == 2 ? s2 : s3;
+            const float inten = L == 0 ? 1.00f : L == 1 ? 0.55f : L == 2 ? 0.30f : 0.14f;
+            const float sharp = L == 0 ? 320.f : L == 1 ? 520.f : L == 2 ? 820.f : 1500.f;
+            col = starShade(hd2, hm, ht, inten, sharp);
+        }
+        // each lane adds the hits of this round that are its own, in order
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+        {
+            const int kk = start + jj;
+            const bool mine = jj < n && kk >= base && kk < base + 64;
+            const int src = mine ? kk - base : lane;
+            const float x = __shfl(col.x, src), y = __shfl(col.y, src), z = __shfl(col.z, src);
+            if (mine) c += mk3(x, y, z);
+        }
+    }
+    return skyDiffuse(c, d, gCam);
 }
 
 // ------------------------------------------------------------------ post (post_process.cuh), exact arithmetic

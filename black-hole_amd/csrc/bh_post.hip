@@ -49,12 +49,43 @@ BHK_INL void shadeBlendAt(const RenderParams& P, const BhRayRecord* rec, float4*
     accum[idx] = make_float4(c.x, c.y, c.z, 1.f);
 }
 
+// HDR sample of a record for a whole wavefront (every lane calls it; lanes
+// past the image get an unused value): the sky through
+// backgroundColorWave, otherwise the same operations as sampleFromRecord.
+BHK_INL v3 sampleFromRecordWave(const BhRayRecord* rec, int idx, bool valid, float gCam)
+{
+    float4 a = make_float4(0.f, 0.f, 1.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid)
+    {
+        const float4* r = reinterpret_cast<const float4*>(rec + idx);
+        a = r[0];
+        b = r[1];
+    }
+    const bool escaped = valid && (__float_as_int(a.w) & 1);
+    const v3 sky = backgroundColorWave(mk3(a.x, a.y, a.z), gCam, escaped);
+    const v3 bg = escaped ? sky : mk3(0.f, 0.f, 0.f);
+    v3 hdr;
+    {
+#pragma clang fp contract(off)
+        hdr = mk3(b.x, b.y, b.z) + bg * b.w;
+    }
+    if (!finite3(hdr)) hdr = mk3(0.f, 0.f, 0.f);
+    return hdr;
+}
+
 __global__ __launch_bounds__(256) void shadeBlendKernel(RenderParams P, const BhRayRecord* rec, float4* accum,
                                                         const int* dynIndex)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= P.width * P.height) return;
-    shadeBlendAt(P, rec, accum, idx, cameraEnergyFactor(P), dynIndex ? *dynIndex : P.sampleIndex);
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = idx < P.width * P.height;
+    const v3 cur = sampleFromRecordWave(rec, idx, valid, cameraEnergyFactor(P));
+    if (!valid) return;
+    const int sampleIndex = dynIndex ? *dynIndex : P.sampleIndex;
+    float4 pv = accum[idx];
+    v3 prev = mk3(pv.x, pv.y, pv.z);
+    if (!finite3(prev)) prev = cur;
+    v3 c = accumBlend(prev, cur, P.accumMode, sampleIndex);
+    accum[idx] = make_float4(c.x, c.y, c.z, 1.f);
 }
 
 __global__ void bumpIndexKernel(int* i) { i[0] += 1; }
@@ -70,10 +101,10 @@ __global__ __launch_bounds__(256) void shadeBlendBatchKernel(const ViewDesc* vie
 
 __global__ __launch_bounds__(256) void shadeFinalizeKernel(RenderParams P, const BhRayRecord* rec, uchar4* out)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= P.width * P.height) return;
-    const float gCam = cameraEnergyFactor(P);
-    v3 hdr = sampleFromRecord(rec, idx, gCam);
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = idx < P.width * P.height;
+    const v3 hdr = sampleFromRecordWave(rec, idx, valid, cameraEnergyFactor(P));
+    if (!valid) return;
     int px = idx % P.width, py = idx / P.width;
     out[idx] = finalizePixel(hdr, mk3(0.f, 0.f, 0.f), P, px, py);
 }
