@@ -127,6 +127,11 @@ struct BandK
     double bCrit2;           // spherical models: critical impact parameter squared
     double rScanLo, rScanHi; // Kerr models: radial range of the critical-curve scan
     double critBand, polarBand;
+    // the camera's Boyer-Lindquist position, basis and static-observer
+    // metric terms (trace.cuh:213-247, 704-739): the same for every ray of
+    // the frame, evaluated once on the host in double
+    double camR0, camCth0, camTh0, camPh0, camS, camC, camSp, camCp, camS02;
+    double camSig0, camDel0, camGtt, camGtp, camGpp, camA, camPt0, camPrScale, camPthScale;
     int kerr, maxSteps, disk, pad;
 };
 
@@ -137,39 +142,28 @@ struct RayInv64
     double E, Lz, carter, r0, th0, ph0, pr, pth;
 };
 
-BHK_INL RayInv64 kerrInvariants64(const BandK& B, d3 o, d3 dir)
+// Invariants of a ray leaving the camera in direction `dir`: only the
+// projections of dir on the camera's basis and what follows from them are
+// per ray; the camera terms come from the frame constants.
+BHK_INL RayInv64 kerrInvariants64(const BandK& B, d3 dir)
 {
 #pragma clang fp contract(off)
     RayInv64 I;
-    const double a = B.a, M = B.M, Q = B.Q;
-    const double r0 = dlen(o);
-    const double cth0 = fmin(fmax(o.y / r0, -1.0), 1.0);
-    const double th0 = acos(cth0);
-    const double ph0 = atan2(-o.z, o.x);
-    const double s = sin(th0), c = cos(th0), sp = sin(ph0), cp = cos(ph0);
+    const double a = B.a;
+    const double s = B.camS, c = B.camC, sp = B.camSp, cp = B.camCp;
     const d3 er = dk3(s * cp, c, -s * sp);
     const d3 eth = dk3(c * cp, -s, -c * sp);
     const d3 eph = dk3(-sp, 0.0, -cp);
     const double drc = ddot(dir, er), dtc = ddot(dir, eth), dpc = ddot(dir, eph);
-    double s0 = sin(th0);
-    if (s0 < F2D(1e-4)) s0 = F2D(1e-4);
-    const double s02 = s0 * s0;
-    const double Sig0 = r0 * r0 + a * a * cth0 * cth0;
-    const double Del0 = r0 * r0 - 2.0 * M * r0 + a * a + Q * Q;
-    const double m2r0 = 2.0 * M * r0 - Q * Q;
-    double gtt = -(1.0 - m2r0 / Sig0);
-    if (gtt > F2D(-1e-5)) gtt = F2D(-1e-5);
-    const double gtp = -a * m2r0 * s02 / Sig0;
-    const double gpp = (r0 * r0 + a * a + m2r0 * a * a * s02 / Sig0) * s02;
-    const double A = 1.0 / sqrt(fmax(gpp - gtp * gtp / gtt, F2D(1e-8)));
-    const double ptUp = 1.0 / sqrt(-gtt) - dpc * A * gtp / gtt;
-    const double pfUp = dpc * A;
-    I.E = -(gtt * ptUp + gtp * pfUp);
-    I.Lz = gtp * ptUp + gpp * pfUp;
-    I.r0 = r0; I.th0 = th0; I.ph0 = ph0;
-    I.pr = drc * sqrt(Sig0 / fmax(Del0, F2D(1e-6)));
-    I.pth = dtc * sqrt(Sig0);
-    I.carter = I.pth * I.pth + cth0 * cth0 * (I.Lz * I.Lz / s02 - a * a * I.E * I.E);
+    const double ptUp = B.camPt0 - dpc * B.camA * B.camGtp / B.camGtt;
+    const double pfUp = dpc * B.camA;
+    I.E = -(B.camGtt * ptUp + B.camGtp * pfUp);
+    I.Lz = B.camGtp * ptUp + B.camGpp * pfUp;
+    I.r0 = B.camR0; I.th0 = B.camTh0; I.ph0 = B.camPh0;
+    I.pr = drc * B.camPrScale;
+    I.pth = dtc * B.camPthScale;
+    const double cth0 = B.camCth0;
+    I.carter = I.pth * I.pth + cth0 * cth0 * (I.Lz * I.Lz / B.camS02 - a * a * I.E * I.E);
     return I;
 }
 
@@ -249,7 +243,7 @@ BHK_INL bool bandStats(const BandK& B, d3 o, d3 dir, double& delta, double& pola
     delta = 0.0;
     if (B.kerr)
     {
-        const RayInv64 I = kerrInvariants64(B, o, dir);
+        const RayInv64 I = kerrInvariants64(B, dir);
         if (!(I.E > 1e-12) || !isfinite(I.E) || !isfinite(I.Lz)) return false;
         const double xi = I.Lz / I.E, eta = I.carter / (I.E * I.E);
         delta = criticalDelta(B, xi, eta);
@@ -299,7 +293,7 @@ template <class Pol, class D> struct SphRay64
         trans = 1.0f;
         {
             // photon E, Lz in double, handed to the float disk shading (trace.cuh:213-247)
-            const RayInv64 I = kerrInvariants64(B, origin, dir);
+            const RayInv64 I = kerrInvariants64(B, dir);
             double Ed = I.E, Ld = I.Lz;
             if (!(Ed > F2D(1e-6)) || !isfinite(Ed)) { Ed = 1.0; Ld = 0.0; }
             E = (float)Ed;
@@ -484,7 +478,7 @@ template <class Pol, class D> struct KerrRay64
         escDir = dir;
         acc = mk3(0.f, 0.f, 0.f);
         trans = 1.0f;
-        const RayInv64 I = kerrInvariants64(B, origin, dir);
+        const RayInv64 I = kerrInvariants64(B, dir);
         E = I.E; Lz = I.Lz;
         r = I.r0; th = I.th0; ph = I.ph0; pr = I.pr; pth = I.pth;
         D::sincos_(th, sth, cth);

@@ -72,6 +72,45 @@ BandK makeBandK(const RenderParams& P, float critBand, float polarBand)
     B.rScanHi = std::max(5.0 * B.M, B.rScanLo * 1.5);
     B.critBand = critBand;
     B.polarBand = polarBand;
+    // camera terms of the ray invariants (trace.cuh:213-247, 704-739), in
+    // double from the float camera position, with contraction off (this
+    // file is compiled with -ffp-contract=off)
+    {
+        const double a = B.a, M = B.M, Q = B.Q;
+        const double ox = P.camPos.x, oy = P.camPos.y, oz = P.camPos.z;
+        const double r0 = std::sqrt(ox * ox + oy * oy + oz * oz);
+        const double cth0 = std::fmin(std::fmax(oy / r0, -1.0), 1.0);
+        const double th0 = std::acos(cth0);
+        const double ph0 = std::atan2(-oz, ox);
+        double s0 = std::sin(th0);
+        if (s0 < (double)1e-4f) s0 = (double)1e-4f;
+        const double s02 = s0 * s0;
+        const double Sig0 = r0 * r0 + a * a * cth0 * cth0;
+        const double Del0 = r0 * r0 - 2.0 * M * r0 + a * a + Q * Q;
+        const double m2r0 = 2.0 * M * r0 - Q * Q;
+        double gtt = -(1.0 - m2r0 / Sig0);
+        if (gtt > (double)-1e-5f) gtt = (double)-1e-5f;
+        const double gtp = -a * m2r0 * s02 / Sig0;
+        const double gpp = (r0 * r0 + a * a + m2r0 * a * a * s02 / Sig0) * s02;
+        B.camR0 = r0;
+        B.camCth0 = cth0;
+        B.camTh0 = th0;
+        B.camPh0 = ph0;
+        B.camS = std::sin(th0);
+        B.camC = std::cos(th0);
+        B.camSp = std::sin(ph0);
+        B.camCp = std::cos(ph0);
+        B.camS02 = s02;
+        B.camSig0 = Sig0;
+        B.camDel0 = Del0;
+        B.camGtt = gtt;
+        B.camGtp = gtp;
+        B.camGpp = gpp;
+        B.camA = 1.0 / std::sqrt(std::fmax(gpp - gtp * gtp / gtt, (double)1e-8f));
+        B.camPt0 = 1.0 / std::sqrt(-gtt);
+        B.camPrScale = std::sqrt(Sig0 / std::fmax(Del0, (double)1e-6f));
+        B.camPthScale = std::sqrt(Sig0);
+    }
     return B;
 }
 
