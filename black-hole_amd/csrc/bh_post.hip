@@ -254,11 +254,12 @@ BHK_INL void compositeTile(const RenderParams& P, const float4* accum, const flo
 //   blurV   vertical 9-tap of blurH on the kPHX x kPHY texels,
 // then upsamples and tonemaps its pixels.  Every value is formed with the
 // same operations, in the same order and with the same edge clamps as the
-// two-pass chain above (bloomRowSegment, compositeTile), so the frame is
+// two-pass form above (bloomRowSegment, compositeTile), so the frame is
 // bit-identical to it.  The halo texels are recomputed by neighbouring tiles
-// instead of going through HBM: the accumulation is read once from HBM
-// (neighbours' halo reads hit the caches) and the uchar4 frame written once,
-// the compulsory 20 B per pixel; the half-res images are not written.
+// instead of going through memory as half-resolution images: the kernel
+// writes only the uchar4 frame, and the accumulation rows that neighbouring
+// tiles share are re-read through the caches (a tile's halo region is about
+// 2.1 times its own pixels).
 constexpr int kPW = 64, kPH = 32;
 constexpr int kPHX = kPW / 2 + 2;     // half-res columns under the bilinear taps
 constexpr int kPHY = kPH / 2 + 2;     // half-res rows under the bilinear taps
