@@ -184,10 +184,9 @@ __global__ __launch_bounds__(kTileBuckets) void tileScatterKernel(const unsigned
 // polar turning point), each wavefront holds rays of similar length: only
 // the first few run as long as the longest ray, the rest end early and give
 // their SIMD slots (each resident band wavefront displaces one wavefront of
-// the FP32 tile grid) back to the tile kernel.  One workgroup
-// counting-sorts the list into 256 buckets of 1/8 octave of the key
-// (2^-24 .. 2^8); order within a bucket is arbitrary, which changes no
-// result (rays are independent).
+// the FP32 tile grid) back to the tile kernel.  The list is counting-sorted
+// into 256 buckets of 1/8 octave of the key (2^-24 .. 2^8); order within a
+// bucket is arbitrary, which changes no result (rays are independent).
 constexpr int kBandBuckets = 256;
 
 BHK_INL int bandBucket(unsigned keyBits)
@@ -198,36 +197,64 @@ BHK_INL int bandBucket(unsigned keyBits)
     return b <= 0.f ? 0 : (b >= (float)(kBandBuckets - 1) ? kBandBuckets - 1 : (int)b);
 }
 
-__global__ __launch_bounds__(1024) void bandSortKernel(const unsigned* count, const unsigned* list,
-                                                       const unsigned* key, unsigned* sorted)
+// Three launches, as for the tiles: per-workgroup LDS histograms added into
+// hist[0..255], one workgroup's exclusive scan into the cursors
+// hist[256..511] (tileScanKernel), and a scatter in which each workgroup
+// ranks a 256-entry chunk in LDS and reserves its range of every bucket with
+// one atomic.  The list length is read on the device; the grid strides over
+// it.
+constexpr unsigned kBandSortBlocks = 64;
+
+__global__ __launch_bounds__(kBandBuckets) void bandHistKernel(const unsigned* count, const unsigned* key,
+                                                              unsigned* hist)
 {
-    __shared__ unsigned h[kBandBuckets], s[kBandBuckets];
+    __shared__ unsigned h[kBandBuckets];
     const int t = threadIdx.x;
     const unsigned n = *count;
-    if (t < kBandBuckets) h[t] = 0u;
+    h[t] = 0u;
     __syncthreads();
-    for (unsigned i = t; i < n; i += blockDim.x) atomicAdd(h + bandBucket(key[i]), 1u);
+    for (unsigned i = blockIdx.x * (unsigned)kBandBuckets + t; i < n; i += gridDim.x * (unsigned)kBandBuckets)
+        atomicAdd(h + bandBucket(key[i]), 1u);
     __syncthreads();
-    // exclusive scan of the bucket counts (inclusive Hillis-Steele, then
-    // shifted), every thread at every barrier
-    unsigned v = t < kBandBuckets ? h[t] : 0u;
-    if (t < kBandBuckets) s[t] = v;
-    __syncthreads();
-    for (int d = 1; d < kBandBuckets; d <<= 1)
+    if (h[t]) atomicAdd(hist + t, h[t]);
+}
+
+__global__ __launch_bounds__(kBandBuckets) void bandScatterKernel(const unsigned* count, const unsigned* list,
+                                                                 const unsigned* key, unsigned* hist,
+                                                                 unsigned* sorted)
+{
+    __shared__ unsigned h[kBandBuckets], base[kBandBuckets];
+    const int t = threadIdx.x;
+    const unsigned n = *count;
+    // chunk start is the same for the whole workgroup, so every thread
+    // reaches every barrier
+    for (unsigned c0 = blockIdx.x * (unsigned)kBandBuckets; c0 < n; c0 += gridDim.x * (unsigned)kBandBuckets)
     {
-        const unsigned add = (t < kBandBuckets && t >= d) ? s[t - d] : 0u;
+        h[t] = 0u;
         __syncthreads();
-        if (t < kBandBuckets) s[t] += add;
+        const unsigned i = c0 + t;
+        int b = 0;
+        unsigned rank = 0u;
+        if (i < n)
+        {
+            b = bandBucket(key[i]);
+            rank = atomicAdd(h + b, 1u);
+        }
+        __syncthreads();
+        if (h[t]) base[t] = atomicAdd(hist + kBandBuckets + t, h[t]);
+        __syncthreads();
+        if (i < n) sorted[base[b] + rank] = list[i];
         __syncthreads();
     }
-    if (t < kBandBuckets) h[t] = s[t] - v;   // cursor: first slot of bucket t
-    __syncthreads();
-    for (unsigned i = t; i < n; i += blockDim.x) sorted[atomicAdd(h + bandBucket(key[i]), 1u)] = list[i];
 }
 
 hipError_t launchBandSort(const BandArgs& A, hipStream_t st)
 {
-    hipLaunchKernelGGL(bandSortKernel, dim3(1), dim3(1024), 0, st, A.count, A.list, A.listKey, A.sorted);
+    hipLaunchKernelGGL(bandHistKernel, dim3(kBandSortBlocks), dim3(kBandBuckets), 0, st, A.count, A.listKey,
+                       A.listHist);
+    hipLaunchKernelGGL(tileScanKernel, dim3(1), dim3(kTileBuckets), 0, st, A.listHist);
+    hipLaunchKernelGGL(bandScatterKernel, dim3(kBandSortBlocks), dim3(kBandBuckets), 0, st, A.count, A.list,
+                       A.listKey, A.listHist, A.sorted);
     return hipGetLastError();
 }
 
@@ -235,10 +262,12 @@ hipError_t launchBandSort(const BandArgs& A, hipStream_t st)
 // (tile queue head, band list length and queue head, surrogate counters)
 // and, with tile ordering, set every tile key to "no FP32 ray" and clear
 // the tile histogram.
-__global__ void frameResetKernel(unsigned* counter, int nWords, unsigned* tileKey, int nTiles, unsigned* hist)
+__global__ void frameResetKernel(unsigned* counter, int nWords, unsigned* tileKey, int nTiles, unsigned* hist,
+                                 unsigned* listHist)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nWords) counter[i] = 0u;
+    if (listHist && i < 2 * kBandBuckets) listHist[i] = 0u;
     if (tileKey)
     {
         if (i < nTiles) tileKey[i] = 0xFFFFFFFFu;
@@ -247,16 +276,16 @@ __global__ void frameResetKernel(unsigned* counter, int nWords, unsigned* tileKe
 }
 
 hipError_t launchFrameReset(unsigned* counter, int nWords, unsigned* tileKey, int nTiles, unsigned* hist,
-                            hipStream_t st)
+                            unsigned* listHist, hipStream_t st)
 {
-    int n = nWords;
+    int n = nWords > 2 * kBandBuckets ? nWords : 2 * kBandBuckets;
     if (tileKey)
     {
         n = nTiles > n ? nTiles : n;
         n = 2 * kTileBuckets > n ? 2 * kTileBuckets : n;
     }
     hipLaunchKernelGGL(frameResetKernel, dim3((n + 255) / 256), dim3(256), 0, st, counter, nWords, tileKey, nTiles,
-                       hist);
+                       hist, listHist);
     return hipGetLastError();
 }
 

@@ -40,6 +40,7 @@ struct BandArgs
     unsigned* list;         // [W*H] band pixel indices
     unsigned* listKey;      // [W*H] cost key of each listed ray (or null: no ordering)
     unsigned* sorted;       // [W*H] the list, costliest rays first (or null: trace in list order)
+    unsigned* listHist;     // [512] bucket counts and cursors of the sort (zero before classification)
     const BhRayRecord* prevRec;     // [W*H] the previous frame's records of this same view: cost keys from
                                     // their step counts (or null: keys from the band statistics)
     unsigned* count;        // list length (zero before classification)
@@ -181,7 +182,7 @@ hipError_t launchResetBatch(const BatchArgs& A, hipStream_t st);
 // keys: fill (before classification), then a counting sort into `order`
 // (hist: 512 counters of scratch).
 hipError_t launchFrameReset(unsigned* counter, int nWords, unsigned* tileKey, int nTiles, unsigned* hist,
-                            hipStream_t st);
+                            unsigned* listHist, hipStream_t st);
 hipError_t launchBandSort(const BandArgs& A, hipStream_t st);
 hipError_t launchTileOrder(const unsigned* tileKey, int nTiles, unsigned* hist, unsigned* order, hipStream_t st);
 hipError_t launchShadeBlendBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);

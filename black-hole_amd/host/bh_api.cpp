@@ -141,6 +141,7 @@ struct Scratch
     unsigned* bandList = nullptr;
     unsigned* bandKey = nullptr;          // [W*H] cost key per listed band ray
     unsigned* bandSorted = nullptr;       // [W*H] band list, longest rays first
+    unsigned* bandHist = nullptr;         // [512] band sort: bucket counts, then cursors
     // the last traced view (RenderParams less the per-frame sample index and
     // disk time) and the record buffer it wrote: a frame of the same view
     // orders its work by those records' step counts
@@ -334,13 +335,16 @@ hipError_t getBandScratch(Scratch* S, size_t n, hipStream_t st)
         (void)hipFree(S->bandList);
         (void)hipFree(S->bandKey);
         (void)hipFree(S->bandSorted);
+        (void)hipFree(S->bandHist);
         S->bandMask = nullptr; S->bandList = nullptr; S->bandKey = nullptr; S->bandSorted = nullptr;
+        S->bandHist = nullptr;
         S->bandCap = 0;
     }
     e = hipMalloc(&S->bandMask, n);
     if (e == hipSuccess) e = hipMalloc(&S->bandList, n * sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc(&S->bandKey, n * sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc(&S->bandSorted, n * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&S->bandHist, 512 * sizeof(unsigned));
     if (e != hipSuccess) return e;
     S->bandCap = n;
     if (!S->aux)
@@ -393,6 +397,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.list = Sc->bandList;
         BA.listKey = Sc->bandKey;
         BA.sorted = Sc->bandSorted;
+        BA.listHist = Sc->bandHist;
         {
             // same view as the last frame traced into `rec`?  (Under graph
             // capture always: replays repeat this view.)
@@ -439,7 +444,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         // every counter word of the frame (the tile queue head included) and
         // the tile keys in one launch
         e = bhk::launchFrameReset(counter, (int)(kCounterBytes / sizeof(unsigned)), BA.tileKey, nTiles,
-                                  Sc->tileHist, st);
+                                  Sc->tileHist, BA.listHist, st);
         if (e == hipSuccess) e = bhk::launchBandClassify(BA, st);
         if (e == hipSuccess) e = bhk::launchBandSort(BA, st);
         if (e == hipSuccess && BA.tileKey) e = bhk::launchTileOrder(BA.tileKey, nTiles, Sc->tileHist, Sc->tileOrder, st);
