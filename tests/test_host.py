@@ -252,3 +252,12 @@ def test_null_and_invalid_arguments_rejected_without_gpu(bt):
     assert lib.launchRenderPipeline(None, None, None, None, C.byref(P), None) != 0
     assert lib.bhProbeGrid(None, C.byref(P), 4, 4, None) != 0
     assert lib.bhFramebufferCreate(0, 10, C.byref(C.c_void_p())) != 0
+    # the post chain alone: output and accumulation buffers are required, and
+    # the bloom scratch too when bloom is on (as in launchRenderPipeline)
+    dummy = C.c_void_p(0x1000)
+    assert lib.bhPostChain(None, dummy, dummy, dummy, C.byref(P), None) != 0
+    assert lib.bhPostChain(dummy, None, dummy, dummy, C.byref(P), None) != 0
+    Q = P.copy()
+    Q.bloomEnabled = 1
+    assert lib.bhPostChain(dummy, dummy, None, None, C.byref(Q), None) != 0
+    assert lib.bhFramebufferPost(None, C.byref(P), None) != 0
