@@ -382,6 +382,21 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     const int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, sc.diskCoop, tiles)
                                 : bhk::traceOccupancyExact(kerr, sc.wgSize, sc.diskCoop, tiles);
     const long long want = (long long)occ * numCUs(dev);
+    // Same view as the last frame traced into `rec` (RenderParams apart from
+    // the sample index and disk time)?  Then its records predict this
+    // frame's work.  Under graph capture always: replays repeat this view.
+    // The prediction only orders the work; any prediction gives the same frame.
+    bool repeatView;
+    {
+        RenderParams v = P;
+        v.sampleIndex = 0;
+        v.diskTime = 0.f;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        const bool capturing = hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+        repeatView = capturing || (Sc->lastRec == rec && std::memcmp(&v, &Sc->lastView, sizeof(v)) == 0);
+        Sc->lastView = v;
+        Sc->lastRec = rec;
+    }
     bhk::BandArgs BA;
     BA.tileKey = nullptr;
     if (band)
@@ -398,19 +413,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         BA.listKey = Sc->bandKey;
         BA.sorted = Sc->bandSorted;
         BA.listHist = Sc->bandHist;
-        {
-            // same view as the last frame traced into `rec`?  (Under graph
-            // capture always: replays repeat this view.)
-            RenderParams v = P;
-            v.sampleIndex = 0;
-            v.diskTime = 0.f;
-            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-            const bool capturing = hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
-            const bool same = Sc->lastRec == rec && std::memcmp(&v, &Sc->lastView, sizeof(v)) == 0;
-            BA.prevRec = (same || capturing) ? rec : nullptr;
-            Sc->lastView = v;
-            Sc->lastRec = rec;
-        }
+        BA.prevRec = repeatView ? rec : nullptr;
         BA.count = counter + kBandCountSlot;
         BA.queue = counter + kBandQueueSlot;
         BA.tilesX = (P.width + 7) / 8;
