@@ -27,7 +27,22 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
         const float2 jit = launchJitter(A.P, A.jitter, A.dynIndex);
         const d3 o = dk3(A.P.camPos.x, A.P.camPos.y, A.P.camPos.z);
         double delta, polar;
-        const bool ok = bandStats(A.B, o, cameraDir64(A.P, px, py, jit), delta, polar);
+        const d3 dir = cameraDir64(A.P, px, py, jit);
+        // Kerr-family frames with a horizon: an FP32 estimate settles the
+        // rays far from the band; the rest are evaluated in FP64
+        bool exact = true;
+        if (A.B.kerr && A.B.rPlus > 1e-6)
+        {
+            float d32, p32;
+            if (bandStatsF32(A.B, dir, d32, p32) && !(fabsf(d32) < (float)A.B.critBand + 0.01f) &&
+                !(p32 < (float)A.B.polarBand + 1e-3f))
+            {
+                delta = d32;
+                polar = p32;
+                exact = false;
+            }
+        }
+        const bool ok = exact ? bandStats(A.B, o, dir, delta, polar) : true;
         band = !ok || (!(A.B.kerr == 0 && delta == 1.0) && (fabs(delta) < A.B.critBand || polar < A.B.polarBand));
         A.mask[i] = band ? 1 : 0;
         tile = (py >> 3) * A.tilesX + (px >> 3);
