@@ -264,12 +264,13 @@ BHK_INL bool bandStats(const BandK& B, d3 o, d3 dir, double& delta, double& pola
 // FP32 estimate of bandStats for a Kerr-family frame with a horizon: the
 // same invariants (camera terms from the frame constants), 32-point scan,
 // golden-section search and polar turning point, in float with hardware
-// reciprocals.  Checked on the CPU against the FP64 values on the headline
-// Kerr and Kerr-Newman frames and on KN/Kerr views with strong charge or
-// retrograde spin: errors below 2e-5 in delta and 4e-6 in sin^2 of the
-// polar turning angle.  The classification trusts it only farther than 0.01
-// (delta) and 1e-3 (polar) from the band's edges, and evaluates every other
-// ray in FP64.  Returns false when the estimate is unusable.
+// reciprocals.  Against the FP64 values (tests/test_band_prefilter.py, a
+// float32 restatement, on the headline Kerr and Kerr-Newman views and on
+// views with strong charge or retrograde spin) its errors stay below 2e-5 in
+// delta and 4e-6 in sin^2 of the polar turning angle.  The classification
+// trusts it only farther than 0.01 (delta) and 1e-3 (polar) from the band's
+// edges, and evaluates every other ray in FP64.  Returns false when the
+// estimate is unusable.
 BHK_INL bool bandStatsF32(const BandK& B, d3 dir, float& delta, float& polar)
 {
 #pragma clang fp contract(off)
