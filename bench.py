@@ -213,7 +213,7 @@ def other_configs(bt, lib, fp64):
     """The other workloads at their full sizes (not the headline metric):
     trace time and frame rate of one-frame launches, their exit error against
     a full-frame FP64 trace (when `fp64`), and the 64-view uneven batch
-    (views rendered as consecutive pipelines on one stream)."""
+    (views rendered as single-view pipelines on the library's worker streams)."""
     import ctypes as C
 
     import numpy as np
@@ -263,7 +263,7 @@ def other_configs(bt, lib, fp64):
     out["uneven_views_64"] = {"rays": rays, "batch_ms": round(dt * 1e3, 2), "mrays_s": round(rays / dt / 1e6, 2),
                               "view_latency_ms_p50": round(float(np.median(lat)), 2),
                               "view_latency_ms_max": round(float(max(lat)), 2),
-                              "launch": "consecutive single-view pipelines (persistent multi-view kernel disabled)"}
+                              "launch": "single-view pipelines on 4 worker streams (persistent multi-view kernel disabled)"}
     return out
 
 

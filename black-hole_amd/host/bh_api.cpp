@@ -16,6 +16,7 @@
 // ---------------------------------------------------------------------------
 #include <hip/hip_runtime.h>
 #include <atomic>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -118,6 +119,10 @@ BandK makeBandK(const RenderParams& P, float critBand, float polarBand)
 
 namespace {
 
+// Batches run their views as independent single-view pipelines on this many
+// worker streams, so small views share the machine.
+constexpr int kBatchStreams = 4;
+
 struct Scratch
 {
     BhRayRecord* rec = nullptr;
@@ -136,6 +141,9 @@ struct Scratch
     int viewCap = 0;
     int lastViews = 0;
     std::vector<hipEvent_t> viewEvents;   // [0] batch start, [v+1] view v done
+    // batched views run on library worker streams forked from this stream
+    hipStream_t batchStreams[kBatchStreams] = {};
+    hipEvent_t batchDone[kBatchStreams] = {};
     // FP64 band pass: per-pixel mask, band pixel list (count in counter[16])
     unsigned char* bandMask = nullptr;
     unsigned* bandList = nullptr;
@@ -597,8 +605,8 @@ hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bl
 // The persistent multi-view trace kernel (traceBatchKernel: one queue of
 // (view, tile) items over all views) hung on its first GPU runs and stays
 // switched off until the cause is shown; see renderBatchPersistent.  Batches
-// therefore run as consecutive single-view pipelines on the caller's stream,
-// which use only the tile kernel of launchRenderPipeline.
+// therefore run as single-view pipelines (the kernels of
+// launchRenderPipeline) on worker streams forked from the caller's stream.
 constexpr bool kPersistentBatchEnabled = false;
 
 hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const* outs, float4* const* accums,
@@ -716,14 +724,56 @@ hipError_t renderBatch(int n, const RenderParams* views, uchar4* const* outs, fl
         }
         S->lastViews = n;
     }
+    // worker streams (default priority) forked from the caller's stream
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (int k = 0; k < kBatchStreams; ++k)
+        {
+            if (!S->batchStreams[k])
+            {
+                e = hipStreamCreateWithFlags(&S->batchStreams[k], hipStreamNonBlocking);
+                if (e != hipSuccess) { S->batchStreams[k] = nullptr; return e; }
+            }
+            if (!S->batchDone[k])
+            {
+                e = hipEventCreateWithFlags(&S->batchDone[k], hipEventDisableTiming);
+                if (e != hipSuccess) { S->batchDone[k] = nullptr; return e; }
+            }
+        }
+    }
     e = hipEventRecord(S->viewEvents[0], st);
     if (e != hipSuccess) return e;
-    for (int v = 0; v < n; ++v)
+    for (int k = 0; k < kBatchStreams; ++k)
     {
-        e = renderPipeline(outs[v], accums[v], bloomAs ? bloomAs[v] : nullptr, bloomBs ? bloomBs[v] : nullptr,
-                           views[v], st, false);
+        e = hipStreamWaitEvent(S->batchStreams[k], S->viewEvents[0], 0);
         if (e != hipSuccess) return e;
-        e = hipEventRecord(S->viewEvents[(size_t)v + 1], st);
+    }
+    // largest views first, each on the worker stream with the fewest pixels
+    // so far
+    std::vector<int> ord((size_t)n);
+    for (int v = 0; v < n; ++v) ord[(size_t)v] = v;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        return (long long)views[x].width * views[x].height > (long long)views[y].width * views[y].height;
+    });
+    long long load[kBatchStreams] = {};
+    for (int v : ord)
+    {
+        int k = 0;
+        for (int j = 1; j < kBatchStreams; ++j)
+            if (load[j] < load[k]) k = j;
+        load[k] += (long long)std::max(views[v].width, 1) * std::max(views[v].height, 1);
+        hipStream_t ws = S->batchStreams[k];
+        e = renderPipeline(outs[v], accums[v], bloomAs ? bloomAs[v] : nullptr, bloomBs ? bloomBs[v] : nullptr,
+                           views[v], ws, false);
+        if (e != hipSuccess) return e;
+        e = hipEventRecord(S->viewEvents[(size_t)v + 1], ws);
+        if (e != hipSuccess) return e;
+    }
+    // the caller's stream continues once every worker stream has finished
+    for (int k = 0; k < kBatchStreams; ++k)
+    {
+        e = hipEventRecord(S->batchDone[k], S->batchStreams[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, S->batchDone[k], 0);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

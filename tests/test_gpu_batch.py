@@ -3,9 +3,10 @@ models and disk settings must reproduce per-view launchRenderPipeline output
 bit for bit, and match the oracle.
 
 The persistent single-queue multi-view kernel hung on its first two GPU runs
-(cause not yet shown); bhRenderBatch now runs the views as consecutive
-single-view pipelines and these tests exercise that path.  The persistent
-kernel's own launch path returns hipErrorNotSupported.
+(cause not yet shown); bhRenderBatch now runs the views as single-view
+pipelines on worker streams forked from the caller's stream (views finish in
+any order), and these tests exercise that path.  The persistent kernel's own
+launch path returns hipErrorNotSupported.
 """
 import numpy as np
 import pytest
@@ -90,7 +91,7 @@ def test_sequential_batch_matches_single_views(gpu, oracle):
     B.render()
     gpu.synchronize()
     lat = B.latency_ms()
-    assert all(x >= 0 for x in lat) and all(b >= a for a, b in zip(lat, lat[1:])), lat
+    assert all(x >= 0 for x in lat), lat
     for v, P in enumerate(views):
         rgba, acc = B.readback(v)
         ref_rgba, ref_acc = gpu.render_pipeline(P)
