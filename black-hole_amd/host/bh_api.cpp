@@ -121,7 +121,7 @@ namespace {
 
 // Batches run their views as independent single-view pipelines on this many
 // worker streams, so small views share the machine.
-constexpr int kBatchStreams = 4;
+constexpr int kBatchStreams = 2;
 
 struct Scratch
 {

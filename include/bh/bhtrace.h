@@ -210,7 +210,7 @@ bh_error_t bhTraceRecords(BhRayRecord* devOut, BH_PARAMS_IN p, bh_stream_t strea
  * bloomBs[v] float4[ceil(W/2)*ceil(H/2)] (the arrays may be NULL when no view
  * has bloom).  `views` is a host array; each view is copied and sanitized.
  * Results equal per-view launchRenderPipeline calls bit for bit.  The views
- * run as single-view pipelines on four library worker streams forked from
+ * run as single-view pipelines on two library worker streams forked from
  * `stream` (largest views first, each on the least-loaded worker), and
  * `stream` continues when all have finished; the single-queue persistent
  * multi-view kernel is disabled in this build. */
