@@ -266,7 +266,7 @@ def other_configs(bt, lib, fp64):
     out["uneven_views_64"] = {"rays": rays, "batch_ms": round(dt * 1e3, 2), "mrays_s": round(rays / dt / 1e6, 2),
                               "view_latency_ms_p50": round(float(np.median(lat)), 2),
                               "view_latency_ms_max": round(float(max(lat)), 2),
-                              "launch": "single-view pipelines on 2 worker streams (persistent multi-view kernel disabled)"}
+                              "launch": "single-view pipelines on 6 worker streams (persistent multi-view kernel disabled)"}
     return out
 
 

@@ -210,8 +210,9 @@ bh_error_t bhTraceRecords(BhRayRecord* devOut, BH_PARAMS_IN p, bh_stream_t strea
  * bloomBs[v] float4[ceil(W/2)*ceil(H/2)] (the arrays may be NULL when no view
  * has bloom).  `views` is a host array; each view is copied and sanitized.
  * Results equal per-view launchRenderPipeline calls bit for bit.  The views
- * run as single-view pipelines on two library worker streams forked from
- * `stream` (largest views first, each on the least-loaded worker), and
+ * run as single-view pipelines on six library worker streams forked from
+ * `stream` (largest views first, each on the least-loaded worker; each worker
+ * and its pipelines' auxiliary stream have a hardware queue of their own), and
  * `stream` continues when all have finished; the single-queue persistent
  * multi-view kernel is disabled in this build. */
 bh_error_t bhRenderBatch(int nViews, const RenderParams* views, bh_uchar4* const* outs,
