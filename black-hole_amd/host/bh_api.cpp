@@ -221,7 +221,8 @@ int numCUs(int dev)
 // queues per process and shares them among streams in an order set by every
 // stream created before, so two batch streams can land on one queue and run
 // one after the other.  A stream created with a CU mask gets a queue outside
-// that pool; the mask enables every CU, so dispatch is unchanged.
+// that pool; the mask enables every CU, so dispatch is unchanged.  HIP creates
+// it as a blocking stream: it also orders against the legacy null stream.
 hipError_t createQueueStream(int dev, hipStream_t* s)
 {
     const int n = numCUs(dev);

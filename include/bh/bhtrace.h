@@ -212,7 +212,9 @@ bh_error_t bhTraceRecords(BhRayRecord* devOut, BH_PARAMS_IN p, bh_stream_t strea
  * Results equal per-view launchRenderPipeline calls bit for bit.  The views
  * run as single-view pipelines on six library worker streams forked from
  * `stream` (largest views first, each on the least-loaded worker; each worker
- * and its pipelines' auxiliary stream have a hardware queue of their own), and
+ * and its pipelines' auxiliary stream have a hardware queue of their own;
+ * HIP makes such streams blocking, so the batch also orders against work on
+ * the legacy null stream), and
  * `stream` continues when all have finished; the single-queue persistent
  * multi-view kernel is disabled in this build. */
 bh_error_t bhRenderBatch(int nViews, const RenderParams* views, bh_uchar4* const* outs,
