@@ -638,6 +638,26 @@ void bho_background(const float* dir, float gCam, float* rgb)
     V3<float> c = backgroundColor(mk3(dir[0], dir[1], dir[2]), gCam);
     rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
 }
+// One star lattice (trace.cuh:131-162) at the point d (not normalised here,
+// so tests can place d * cellScale anywhere in lattice space).
+void bho_star_layer(const float* d, float cellScale, float intensity, float coreSharp, float* rgb)
+{
+    V3<float> c = starLayer(mk3(d[0], d[1], d[2]), cellScale, intensity, coreSharp);
+    rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
+// The volumetric disk on one chord [a, b] (trace.cuh:269-403) for photon
+// (E, Lz) and hole spin/charge (a, Q) from ctx4 = {E, Lz, a, Q}; io4 holds
+// the accumulated radiance and transmittance, updated in place.
+void bho_disk_segment(const RenderParams* Pp, const float* ctx4, const float* a3, const float* b3, float tEmit,
+                      float* io4)
+{
+    PT<float> P(*Pp);
+    DiskCtx<float> ctx{ctx4[0], ctx4[1], ctx4[2], ctx4[3]};
+    V3<float> acc = mk3(io4[0], io4[1], io4[2]);
+    float trans = io4[3];
+    sampleDiskSegment<float>(P, ctx, mk3(a3[0], a3[1], a3[2]), mk3(b3[0], b3[1], b3[2]), tEmit, acc, trans);
+    io4[0] = acc.x; io4[1] = acc.y; io4[2] = acc.z; io4[3] = trans;
+}
 void bho_finalize(const float* hdr, const float* bloom, const RenderParams* P, int px, int py, unsigned char* out)
 {
     finalizePixel<float>(mk3(hdr[0], hdr[1], hdr[2]), mk3(bloom[0], bloom[1], bloom[2]), *P, px, py, out);

@@ -53,6 +53,8 @@ class Oracle:
             "bho_aces": (None, [_vp, _vp]),
             "bho_background": (None, [_vp, C.c_float, _vp]),
             "bho_finalize": (None, [_vp, _vp, _P, C.c_int, C.c_int, _vp]),
+            "bho_star_layer": (None, [_vp, C.c_float, C.c_float, C.c_float, _vp]),
+            "bho_disk_segment": (None, [_P, _vp, _vp, _vp, C.c_float, _vp]),
             "bho_emitter_g": (C.c_float, [C.c_float] * 6),
             "bho_flux_weight": (C.c_float, [C.c_float, C.c_float]),
             "bho_omega": (C.c_float, [C.c_float] * 4),
@@ -205,6 +207,26 @@ class Oracle:
         o = np.zeros(3, np.float32)
         self.L.bho_background(_ptr(a), gcam, _ptr(o))
         return o
+
+    def star_layer(self, d, cell_scale, intensity, core_sharp):
+        a = np.asarray(d, np.float32)
+        o = np.zeros(3, np.float32)
+        self.L.bho_star_layer(_ptr(a), cell_scale, intensity, core_sharp, _ptr(o))
+        return o
+
+    def finalize(self, P, hdr, bloom, px, py):
+        o = np.zeros(4, np.uint8)
+        self.L.bho_finalize(_ptr(np.asarray(hdr, np.float32)), _ptr(np.asarray(bloom, np.float32)), C.byref(P),
+                            px, py, _ptr(o))
+        return o
+
+    def disk_segment(self, P, E, Lz, a_spin, Q, a, b, t_emit, acc=(0.0, 0.0, 0.0), trans=1.0):
+        """One chord of the volumetric disk: returns (radiance[3], transmittance)."""
+        ctx = np.array([E, Lz, a_spin, Q], np.float32)
+        io = np.array([acc[0], acc[1], acc[2], trans], np.float32)
+        self.L.bho_disk_segment(C.byref(P), _ptr(ctx), _ptr(np.asarray(a, np.float32)),
+                                _ptr(np.asarray(b, np.float32)), t_emit, _ptr(io))
+        return io[:3].copy(), float(io[3])
 
     def jitter(self, mode, idx):
         o = np.zeros(2, np.float32)
