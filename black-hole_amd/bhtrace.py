@@ -144,6 +144,7 @@ SIGNATURES = [
     ("bhWorkloadParams", C.c_int, [C.c_char_p, C.c_int, _P]),
     ("bhWorkloadViews", C.c_int, [C.c_char_p, C.c_uint64, C.c_int, _P]),
     ("bhRenderBatch", C.c_int, [C.c_int, _P, _vp, _vp, _vp, _vp, _vp]),
+    ("bhRenderBatchPersistent", C.c_int, [C.c_int, _P, _vp, _vp, _vp, _vp, _vp]),
     ("bhBatchViewLatency", C.c_int, [C.POINTER(C.c_float), C.c_int, _vp]),
     ("bhProbeGrid", C.c_int, [_vp, _P, C.c_int, C.c_int, _vp]),
     ("bhProbeDirs", C.c_int, [_vp, _vp, C.c_int, _P, _vp]),
@@ -160,6 +161,7 @@ SIGNATURES = [
     ("bhSetSchedule", C.c_int, [C.POINTER(BhSchedule)]),
     ("bhAutotune", C.c_int, [_P, C.c_int, C.POINTER(BhSchedule), C.POINTER(C.c_float), _vp]),
     ("bhAutotuneClear", None, []),
+    ("bhScheduleFor", None, [_P, C.POINTER(BhSchedule)]),
     ("bhAccumCreate", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("bhAccumDestroy", None, [_vp]),
     ("bhAccumColor", _vp, [_vp]),
@@ -206,6 +208,7 @@ SIGNATURES = [
     ("bhCopyDevice", C.c_int, [_vp, _vp, C.c_size_t]),
     ("bhDeviceSynchronize", C.c_int, []),
     ("bhStreamCreate", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("bhStreamRelease", C.c_int, [_vp]),
     ("bhStreamDestroy", C.c_int, [_vp]),
     ("bhStreamSynchronize", C.c_int, [_vp]),
     ("bhErrorString", C.c_char_p, [C.c_int]),
@@ -584,6 +587,13 @@ def restore_schedule(s: BhSchedule) -> None:
     check(load().bhSetSchedule(C.byref(s)), "bhSetSchedule")
 
 
+def schedule_for(P: RenderParams) -> BhSchedule:
+    """The schedule a frame of P runs under (tuned for its class, or the default)."""
+    s = BhSchedule()
+    load().bhScheduleFor(C.byref(P), C.byref(s))
+    return s
+
+
 def autotune(P: RenderParams, reps: int = 2):
     """Run the library's autotuner on P; returns (chosen BhSchedule, best trace ms)."""
     s, ms = BhSchedule(), C.c_float()
@@ -809,6 +819,10 @@ class BatchBuffers:
     def render(self, stream=None):
         check(load().bhRenderBatch(self.n, self.views, self.outs, self.accs, self.bAs, self.bBs, stream),
               "bhRenderBatch")
+
+    def render_persistent(self, stream=None) -> int:
+        """bhRenderBatchPersistent; returns its HIP error code."""
+        return load().bhRenderBatchPersistent(self.n, self.views, self.outs, self.accs, self.bAs, self.bBs, stream)
 
     def latency_ms(self, stream=None):
         out = (C.c_float * self.n)()

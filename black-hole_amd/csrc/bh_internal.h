@@ -188,4 +188,29 @@ hipError_t launchTileOrder(const unsigned* tileKey, int nTiles, unsigned* hist, 
 hipError_t launchShadeBlendBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);
 hipError_t launchBloomBatch(const ViewDesc* views, int nViews, int maxW, int maxH, hipStream_t st);
 
+// Every device buffer one frame of launchRenderPipeline uses beyond the
+// caller's (records, counters, band pass, tile order), handed in by a caller
+// that owns them instead of the library's per-stream scratch.
+struct FrameBufs
+{
+    BhRayRecord* rec;          // [W*H]
+    unsigned* counter;         // [kFrameCounterWords]
+    unsigned char* bandMask;   // [W*H]
+    unsigned* bandList;        // [W*H]
+    unsigned* bandKey;         // [W*H]
+    unsigned* bandSorted;      // [W*H]
+    unsigned* bandHist;        // [512]
+    unsigned* tileKey;         // [ceil(W/8)*ceil(H/8)]
+    unsigned* tileOrder;       // [ceil(W/8)*ceil(H/8)]
+    unsigned* tileHist;        // [512]
+};
+constexpr int kFrameCounterWords = 128;   // 512 bytes of queue heads and list lengths
+
+// launchRenderPipeline on caller-owned scratch: the same kernels and results
+// (the learned surrogate is not applied), the band and tile traces one after
+// the other on `st`, no kernel timers, and no allocation or stream/event
+// creation (host/bh_api.cpp).
+hipError_t renderPipelineFixed(uchar4* out, float4* accum, float4* bloomA, float4* bloomB, const RenderParams& p,
+                               hipStream_t st, const FrameBufs& fb);
+
 }  // namespace bhk

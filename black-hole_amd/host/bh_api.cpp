@@ -126,6 +126,24 @@ namespace {
 // and 165 ms again on 12, whose 24 queues the command processor time-slices.
 constexpr int kBatchStreams = 6;
 
+// The last view traced into a record buffer (RenderParams less the
+// per-frame sample index and disk time) and that buffer: a frame of the same
+// view orders its work by those records' step counts.
+struct ViewMemo
+{
+    RenderParams lastView{};
+    const BhRayRecord* lastRec = nullptr;
+};
+
+// Records of one view of a batch, kept from batch to batch so that a
+// repeated view orders its work by its own previous records.
+struct BatchSlot
+{
+    BhRayRecord* rec = nullptr;
+    size_t cap = 0;
+    ViewMemo memo;
+};
+
 struct Scratch
 {
     BhRayRecord* rec = nullptr;
@@ -153,11 +171,8 @@ struct Scratch
     unsigned* bandKey = nullptr;          // [W*H] cost key per listed band ray
     unsigned* bandSorted = nullptr;       // [W*H] band list, longest rays first
     unsigned* bandHist = nullptr;         // [512] band sort: bucket counts, then cursors
-    // the last traced view (RenderParams less the per-frame sample index and
-    // disk time) and the record buffer it wrote: a frame of the same view
-    // orders its work by those records' step counts
-    RenderParams lastView{};
-    const BhRayRecord* lastRec = nullptr;
+    ViewMemo memo;                        // last view traced into `rec`
+    std::vector<BatchSlot> slots;         // batched views: records per view of the last batch
     size_t bandCap = 0;
     // tile order of the FP32 tile tracer (longest tiles first): per-tile
     // keys, the order, and 512 counters of the counting sort
@@ -193,7 +208,7 @@ constexpr int kBandCountSlot = 16;            // counter[16]: band list length (
 constexpr int kCandCountSlot = 32;            // counter[32]: surrogate candidates
 constexpr int kAcceptSlot = 48;               // counter[48]: rays answered by the surrogate
 constexpr int kBandQueueSlot = 64;            // counter[64]: band-trace queue head
-constexpr size_t kCounterBytes = 512;
+constexpr size_t kCounterBytes = bhk::kFrameCounterWords * sizeof(unsigned);
 // learned surrogate attached to the render path (null: off)
 struct SurrogateRoute
 {
@@ -228,7 +243,12 @@ hipError_t createQueueStream(int dev, hipStream_t* s)
     const int n = numCUs(dev);
     std::vector<uint32_t> mask((size_t)(n + 31) / 32, 0xffffffffu);
     if (n % 32) mask.back() = (1u << (n % 32)) - 1u;
-    return hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data());
+    if (hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data()) == hipSuccess) return hipSuccess;
+    // without CU-mask support: a pooled non-blocking stream (same results;
+    // the runtime may then share its hardware queue with other streams)
+    (void)hipGetLastError();
+    *s = nullptr;
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
 }
 
 // Scratch for the current device and `st`, holding at least `nRec` records.
@@ -394,14 +414,26 @@ float2 jitterOf(int accumMode, int sampleIndex)
     return make_float2(x - std::floor(x), y - std::floor(y));
 }
 
+// Memo of the views traced by the fixed-buffer path, one per device (that
+// path keeps its records in the library's static scratch of that device).
+ViewMemo g_fixedMemo[64];
+
+// Enqueues the trace of one frame into `rec`.  Buffers come from the
+// stream's scratch `Sc`, or, with `fixed`, from caller-owned memory: then
+// nothing is allocated or created, the surrogate is not applied, no timers
+// are recorded, and the FP64 band and FP32 tile traces run one after the
+// other on `st` (no auxiliary stream).  `memo` holds the last view traced
+// into `rec` (null: the stream's own).
 hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, const BhSchedule& sc,
-                        hipStream_t st, const int* dynIndex = nullptr, bool allowTimers = true)
+                        hipStream_t st, const int* dynIndex = nullptr, bool allowTimers = true,
+                        ViewMemo* memo = nullptr, const bhk::FrameBufs* fixed = nullptr)
 {
-    unsigned* counter = Sc->counter;
+    unsigned* counter = fixed ? fixed->counter : Sc->counter;
     const BhPrecision pr = currentPrecision();
     const bool band = pr.fp64Band != 0 && P.width > 0 && P.height > 0;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (!memo) memo = fixed ? &g_fixedMemo[(dev >= 0 && dev < 64) ? dev : 0] : &Sc->memo;
     const int kerr = isKerr(P.model) ? 1 : 0;
     const int tiles = sc.raysPerRefill >= 64 ? 1 : 0;
     const int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, sc.diskCoop, tiles)
@@ -418,49 +450,67 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         v.diskTime = 0.f;
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         const bool capturing = hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
-        repeatView = capturing || (Sc->lastRec == rec && std::memcmp(&v, &Sc->lastView, sizeof(v)) == 0);
-        Sc->lastView = v;
-        Sc->lastRec = rec;
+        repeatView = capturing || (memo->lastRec == rec && std::memcmp(&v, &memo->lastView, sizeof(v)) == 0);
+        memo->lastView = v;
+        memo->lastRec = rec;
     }
+    const SurrogateRoute sur = fixed ? SurrogateRoute{} : currentSurrogate();
     bhk::BandArgs BA;
     BA.tileKey = nullptr;
+    unsigned char* bandMask = nullptr;
+    unsigned* tileOrder = nullptr;
     if (band)
     {
         const size_t n = (size_t)P.width * (size_t)P.height;
-        hipError_t e = getBandScratch(Sc, n, st);
-        if (e != hipSuccess) return e;
+        const long long nT = (long long)((P.width + 7) / 8) * ((P.height + 7) / 8);
+        hipError_t e = hipSuccess;
+        unsigned* tileHist = nullptr;
+        if (fixed)
+        {
+            bandMask = fixed->bandMask;
+            BA.list = fixed->bandList;
+            BA.listKey = fixed->bandKey;
+            BA.sorted = fixed->bandSorted;
+            BA.listHist = fixed->bandHist;
+            if (tiles) { BA.tileKey = fixed->tileKey; tileOrder = fixed->tileOrder; tileHist = fixed->tileHist; }
+        }
+        else
+        {
+            e = getBandScratch(Sc, n, st);
+            if (e != hipSuccess) return e;
+            bandMask = Sc->bandMask;
+            BA.list = Sc->bandList;
+            BA.listKey = Sc->bandKey;
+            BA.sorted = Sc->bandSorted;
+            BA.listHist = Sc->bandHist;
+            Sc->lastOrderTiles = 0;
+            if (tiles)
+            {
+                // whole-tile schedule: the queue hands out the costliest tiles
+                // first, by the classification's cost keys
+                e = getTileScratch(Sc, (size_t)nT, st);
+                if (e != hipSuccess) return e;
+                BA.tileKey = Sc->tileKey;
+                tileOrder = Sc->tileOrder;
+                tileHist = Sc->tileHist;
+                Sc->lastOrderTiles = (int)nT;
+            }
+        }
         BA.P = P;
         BA.B = bhk::makeBandK(P, pr.critBand, pr.polarBand);
         BA.D = bhk::makeDisk(P, BA.B.kerr ? P.aSpin : 0.f, P.Qc);
         BA.rec = rec;
-        BA.mask = Sc->bandMask;
-        BA.list = Sc->bandList;
-        BA.listKey = Sc->bandKey;
-        BA.sorted = Sc->bandSorted;
-        BA.listHist = Sc->bandHist;
+        BA.mask = bandMask;
         BA.prevRec = repeatView ? rec : nullptr;
         BA.count = counter + kBandCountSlot;
         BA.queue = counter + kBandQueueSlot;
         BA.tilesX = (P.width + 7) / 8;
-        BA.tileKey = nullptr;
-        Sc->lastOrderTiles = 0;
-        const long long nT = (long long)BA.tilesX * ((P.height + 7) / 8);
-        if (tiles)
-        {
-            // whole-tile schedule: the queue hands out the costliest tiles
-            // first, by the classification's cost keys
-            e = getTileScratch(Sc, (size_t)nT, st);
-            if (e != hipSuccess) return e;
-            BA.tileKey = Sc->tileKey;
-            Sc->lastOrderTiles = (int)nT;
-        }
         BA.jitter = jitterOf(P.accumMode, P.sampleIndex);
         BA.dynIndex = dynIndex;
         BA.candInputs = nullptr;
         BA.candList = nullptr;
         BA.candCount = counter + kCandCountSlot;
         BA.psiMax = (float)(65.0 * 3.14159265358979323846 / 180.0);
-        const SurrogateRoute sur = currentSurrogate();
         if (sur.s)
         {
             e = getCandScratch(Sc, n, st);
@@ -468,14 +518,13 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             BA.candInputs = Sc->candInputs;
             BA.candList = Sc->candList;
         }
-        const int nTiles = BA.tilesX * ((P.height + 7) / 8);
+        const int nTiles = (int)nT;
         // every counter word of the frame (the tile queue head included) and
         // the tile keys in one launch
-        e = bhk::launchFrameReset(counter, (int)(kCounterBytes / sizeof(unsigned)), BA.tileKey, nTiles,
-                                  Sc->tileHist, BA.listHist, st);
+        e = bhk::launchFrameReset(counter, bhk::kFrameCounterWords, BA.tileKey, nTiles, tileHist, BA.listHist, st);
         if (e == hipSuccess) e = bhk::launchBandClassify(BA, st);
         if (e == hipSuccess) e = bhk::launchBandSort(BA, st);
-        if (e == hipSuccess && BA.tileKey) e = bhk::launchTileOrder(BA.tileKey, nTiles, Sc->tileHist, Sc->tileOrder, st);
+        if (e == hipSuccess && BA.tileKey) e = bhk::launchTileOrder(BA.tileKey, nTiles, tileHist, tileOrder, st);
         if (e != hipSuccess) return e;
         if (sur.s)
         {
@@ -495,17 +544,15 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             R.sigmaMax = sur.sigmaMax;
             R.diskProbMax = sur.diskProbMax;
             R.rec = rec;
-            R.mask = Sc->bandMask;
+            R.mask = bandMask;
             R.accepted = counter + kAcceptSlot;
-            int dv = 0;
-            if (hipGetDevice(&dv) != hipSuccess) dv = 0;
-            e = bhk::launchSurrogateRoute(R, (int)n, numCUs(dv), st);
+            e = bhk::launchSurrogateRoute(R, (int)n, numCUs(dev), st);
             if (e != hipSuccess) return e;
         }
     }
     bhk::TraceArgs A;
-    A.bandMask = band ? Sc->bandMask : nullptr;
-    A.tileOrder = (band && BA.tileKey) ? Sc->tileOrder : nullptr;
+    A.bandMask = bandMask;
+    A.tileOrder = BA.tileKey ? tileOrder : nullptr;
     A.P = P;
     A.rec = rec;
     A.counter = counter;
@@ -524,6 +571,17 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     if (S.blocks < 1) S.blocks = 1;
     hipError_t e = band ? hipSuccess : bhk::launchResetCounter(counter, st);   // with the band: reset above
     if (e != hipSuccess) return e;
+    const int bandBlocks = 4 * numCUs(dev);
+    if (fixed)
+    {
+        if (band)
+        {
+            e = sc.mathMode ? bhk::launchBandTraceFast(BA, bandBlocks, st)
+                            : bhk::launchBandTraceExact(BA, bandBlocks, st);
+            if (e != hipSuccess) return e;
+        }
+        return sc.mathMode ? bhk::launchTraceFast(A, S, st) : bhk::launchTraceExact(A, S, st);
+    }
     const bool timed = allowTimers && g_timers.load() != 0;
     if (timed && !Sc->tileEv[0])
     {
@@ -549,8 +607,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     e = hipEventRecord(Sc->fork, st);
     if (e != hipSuccess) return e;
     if (timed) (void)hipEventRecord(Sc->bandEv[0], st);
-    const int blocks = 4 * numCUs(dev);
-    e = sc.mathMode ? bhk::launchBandTraceFast(BA, blocks, st) : bhk::launchBandTraceExact(BA, blocks, st);
+    e = sc.mathMode ? bhk::launchBandTraceFast(BA, bandBlocks, st) : bhk::launchBandTraceExact(BA, bandBlocks, st);
     if (e != hipSuccess) return e;
     if (timed) (void)hipEventRecord(Sc->bandEv[1], st);
     e = hipStreamWaitEvent(Sc->aux, Sc->fork, 0);
@@ -575,8 +632,56 @@ void recordEvent(Scratch* S, int i, hipStream_t st)
     (void)hipEventRecord(S->ev[i], st);
 }
 
+// Autotune cache: schedule chosen per workload class (renderPipeline looks
+// a frame's class up here; frames of untuned classes use g_sched).
+struct TuneKey
+{
+    int kerr, w, h, disk, maxSteps, band, math;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(kerr, w, h, disk, maxSteps, band, math) <
+               std::tie(o.kerr, o.w, o.h, o.disk, o.maxSteps, o.band, o.math);
+    }
+};
+std::map<TuneKey, std::pair<BhSchedule, float>> g_tuned;
+
+TuneKey tuneKeyOf(const RenderParams& P, int band, int math)
+{
+    return TuneKey{isKerr(P.model) ? 1 : 0, P.width, P.height, P.diskEnabled, P.maxSteps, band, math};
+}
+
+// Schedule of a (sanitized) frame: the autotuner's choice for its class
+// under the current precision and math mode, else the process default.
+BhSchedule scheduleFor(const RenderParams& P)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tuned.find(tuneKeyOf(P, g_prec.fp64Band, g_sched.mathMode));
+    return it != g_tuned.end() ? it->second.first : g_sched;
+}
+
+// Records of one frame: the stream's scratch, or a batch view's own slot
+// (grown on demand; the stream must be idle on the old buffer first).
+hipError_t slotRecords(BatchSlot* slot, size_t n, hipStream_t st)
+{
+    if (n <= slot->cap) return hipSuccess;
+    if (slot->rec)
+    {
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        (void)hipFree(slot->rec);
+        slot->rec = nullptr;
+        slot->cap = 0;
+    }
+    hipError_t e = hipMalloc(&slot->rec, n * sizeof(BhRayRecord));
+    if (e != hipSuccess) { slot->rec = nullptr; return e; }
+    slot->cap = n;
+    slot->memo = ViewMemo{};
+    return hipSuccess;
+}
+
 hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bloomB, const RenderParams& p,
-                          hipStream_t st, bool singlePass, const int* dynIndex = nullptr, bool allowTimers = true)
+                          hipStream_t st, bool singlePass, const int* dynIndex = nullptr, bool allowTimers = true,
+                          BatchSlot* slot = nullptr)
 {
     RenderParams P = p;
     bhSanitizeRenderParams(&P);
@@ -585,22 +690,31 @@ hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bl
     if (!singlePass && P.bloomEnabled && (!bloomA || !bloomB)) return hipErrorInvalidValue;
     const size_t n = (size_t)P.width * (size_t)P.height;
     Scratch* S = nullptr;
-    hipError_t e = getScratch(st, n, &S);
+    hipError_t e = getScratch(st, slot ? 0 : n, &S);
     if (e != hipSuccess) return e;
-    const BhSchedule sc = currentSchedule();
+    BhRayRecord* rec = S->rec;
+    ViewMemo* memo = nullptr;
+    if (slot)
+    {
+        e = slotRecords(slot, n, st);
+        if (e != hipSuccess) return e;
+        rec = slot->rec;
+        memo = &slot->memo;
+    }
+    const BhSchedule sc = scheduleFor(P);
     const bool timed = allowTimers && g_timers.load() != 0;
     if (timed) recordEvent(S, 0, st);
-    e = enqueueTrace(P, S->rec, S, sc, st, dynIndex, allowTimers);
+    e = enqueueTrace(P, rec, S, sc, st, dynIndex, allowTimers, memo);
     if (e != hipSuccess) return e;
     if (timed) recordEvent(S, 1, st);
     if (singlePass)
     {
-        e = bhk::launchShadeFinalize(P, S->rec, out, st);
+        e = bhk::launchShadeFinalize(P, rec, out, st);
         if (timed) { recordEvent(S, 2, st); recordEvent(S, 3, st); }
     }
     else
     {
-        e = bhk::launchShadeBlend(P, S->rec, accum, dynIndex, st);
+        e = bhk::launchShadeBlend(P, rec, accum, dynIndex, st);
         if (e != hipSuccess) return e;
         if (timed) recordEvent(S, 2, st);
         // bloom + composite in one launch; bloomA/bloomB stay the caller's
@@ -618,6 +732,28 @@ hipError_t renderPipeline(uchar4* out, float4* accum, float4* bloomA, float4* bl
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
+
+}  // namespace
+
+namespace bhk {
+
+hipError_t renderPipelineFixed(uchar4* out, float4* accum, float4* bloomA, float4* bloomB, const RenderParams& p,
+                               hipStream_t st, const FrameBufs& fb)
+{
+    RenderParams P = p;
+    bhSanitizeRenderParams(&P);
+    if (!out || !accum) return hipErrorInvalidValue;
+    if (P.bloomEnabled && (!bloomA || !bloomB)) return hipErrorInvalidValue;
+    hipError_t e = enqueueTrace(P, fb.rec, nullptr, scheduleFor(P), st, nullptr, false, nullptr, &fb);
+    if (e == hipSuccess) e = launchShadeBlend(P, fb.rec, accum, nullptr, st);
+    if (e == hipSuccess) e = launchPostFused(P, accum, out, st);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+}  // namespace bhk
+
+namespace {
 
 // The persistent multi-view trace kernel (traceBatchKernel: one queue of
 // (view, tile) items over all views) hung on its first GPU runs and stays
@@ -719,13 +855,29 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
 }
 
 
+// Checks every view of a batch before any work is enqueued (the arguments
+// launchRenderPipeline checks, on each view's sanitized copy).
+hipError_t validateBatch(int n, const RenderParams* views, uchar4* const* outs, float4* const* accums,
+                         float4* const* bloomAs, float4* const* bloomBs)
+{
+    if (n < 1 || !views || !outs || !accums) return hipErrorInvalidValue;
+    for (int v = 0; v < n; ++v)
+    {
+        RenderParams P = views[v];
+        bhSanitizeRenderParams(&P);
+        if (!outs[v] || !accums[v]) return hipErrorInvalidValue;
+        if (P.bloomEnabled && (!bloomAs || !bloomBs || !bloomAs[v] || !bloomBs[v])) return hipErrorInvalidValue;
+    }
+    return hipSuccess;
+}
+
 hipError_t renderBatch(int n, const RenderParams* views, uchar4* const* outs, float4* const* accums,
                        float4* const* bloomAs, float4* const* bloomBs, hipStream_t st)
 {
-    if (n < 1 || !views || !outs || !accums) return hipErrorInvalidValue;
-    if (kPersistentBatchEnabled) return renderBatchPersistent(n, views, outs, accums, bloomAs, bloomBs, st);
+    hipError_t e = validateBatch(n, views, outs, accums, bloomAs, bloomBs);
+    if (e != hipSuccess) return e;
     Scratch* S = nullptr;
-    hipError_t e = getScratch(st, 0, &S);
+    e = getScratch(st, 0, &S);
     if (e != hipSuccess) return e;
     {
         std::lock_guard<std::mutex> lk(g_mu);
@@ -739,11 +891,12 @@ hipError_t renderBatch(int n, const RenderParams* views, uchar4* const* outs, fl
                 S->viewEvents.push_back(ev);
             }
         }
+        if ((int)S->slots.size() < n) S->slots.resize((size_t)n);
         S->lastViews = n;
     }
     // worker streams (default priority) forked from the caller's stream; they
     // and the auxiliary streams of their pipelines each get a hardware queue
-    // of their own
+    // of their own where the runtime allows it
     int dev = 0;
     e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
@@ -786,13 +939,15 @@ hipError_t renderBatch(int n, const RenderParams* views, uchar4* const* outs, fl
         if (e != hipSuccess) return e;
     }
     // largest views first, each on the worker stream with the fewest pixels
-    // so far
+    // so far; each view traces into its own records, so a view repeated in
+    // the next batch orders its work by its previous step counts
     std::vector<int> ord((size_t)n);
     for (int v = 0; v < n; ++v) ord[(size_t)v] = v;
     std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
         return (long long)views[x].width * views[x].height > (long long)views[y].width * views[y].height;
     });
     long long load[kBatchStreams] = {};
+    hipError_t err = hipSuccess;
     for (int v : ord)
     {
         int k = 0;
@@ -800,19 +955,101 @@ hipError_t renderBatch(int n, const RenderParams* views, uchar4* const* outs, fl
             if (load[j] < load[k]) k = j;
         load[k] += (long long)std::max(views[v].width, 1) * std::max(views[v].height, 1);
         hipStream_t ws = S->batchStreams[k];
-        e = renderPipeline(outs[v], accums[v], bloomAs ? bloomAs[v] : nullptr, bloomBs ? bloomBs[v] : nullptr,
-                           views[v], ws, false);
-        if (e != hipSuccess) return e;
-        e = hipEventRecord(S->viewEvents[(size_t)v + 1], ws);
-        if (e != hipSuccess) return e;
+        err = renderPipeline(outs[v], accums[v], bloomAs ? bloomAs[v] : nullptr, bloomBs ? bloomBs[v] : nullptr,
+                             views[v], ws, false, nullptr, true, &S->slots[(size_t)v]);
+        if (err == hipSuccess) err = hipEventRecord(S->viewEvents[(size_t)v + 1], ws);
+        if (err != hipSuccess) break;
     }
     // the caller's stream continues once every worker stream has finished
+    // (also after an error: work already enqueued stays ordered before the
+    // caller's later work)
     for (int k = 0; k < kBatchStreams; ++k)
     {
         e = hipEventRecord(S->batchDone[k], S->batchStreams[k]);
         if (e == hipSuccess) e = hipStreamWaitEvent(st, S->batchDone[k], 0);
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess && err == hipSuccess) err = e;
     }
+    return err;
+}
+
+
+// Frees every buffer, event and stream a Scratch holds (its streams must be
+// idle).
+void freeScratch(Scratch& S)
+{
+    (void)hipFree(S.rec);
+    (void)hipFree(S.counter);
+    if (S.evValid)
+        for (auto& ev : S.ev) (void)hipEventDestroy(ev);
+    (void)hipFree(S.views);
+    (void)hipHostFree(S.hostViews);
+    (void)hipFree(S.viewDone);
+    (void)hipFree(S.stamps);
+    if (S.staged) (void)hipEventDestroy(S.staged);
+    for (hipEvent_t ev : S.viewEvents) (void)hipEventDestroy(ev);
+    for (int k = 0; k < kBatchStreams; ++k)
+        if (S.batchDone[k]) (void)hipEventDestroy(S.batchDone[k]);
+    (void)hipFree(S.bandMask);
+    (void)hipFree(S.bandList);
+    (void)hipFree(S.bandKey);
+    (void)hipFree(S.bandSorted);
+    (void)hipFree(S.bandHist);
+    (void)hipFree(S.tileKey);
+    (void)hipFree(S.tileOrder);
+    (void)hipFree(S.tileHist);
+    (void)hipFree(S.candInputs);
+    (void)hipFree(S.candList);
+    for (auto& ev : S.bandEv)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : S.tileEv)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& sl : S.slots) (void)hipFree(sl.rec);
+    if (S.aux)
+    {
+        (void)hipStreamSynchronize(S.aux);
+        (void)hipStreamDestroy(S.aux);
+    }
+    if (S.fork) (void)hipEventDestroy(S.fork);
+    if (S.join) (void)hipEventDestroy(S.join);
+}
+
+// Releases what the library holds for `st` on the current device: its
+// scratch, and the worker streams of its batches with their scratch and
+// auxiliary streams.  Waits for the stream's work first.
+hipError_t releaseStream(hipStream_t st)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    std::vector<Scratch> gone;
+    std::vector<hipStream_t> workers;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_scratch.find({dev, st});
+        if (it == g_scratch.end()) return hipSuccess;
+        gone.push_back(it->second);
+        g_scratch.erase(it);
+        for (int k = 0; k < kBatchStreams; ++k)
+        {
+            hipStream_t ws = gone[0].batchStreams[k];
+            if (!ws) continue;
+            workers.push_back(ws);
+            auto w = g_scratch.find({dev, ws});
+            if (w != g_scratch.end())
+            {
+                gone.push_back(w->second);
+                g_scratch.erase(w);
+            }
+        }
+        if (g_lastTimed.first == dev && (g_lastTimed.second == st ||
+                                         std::find(workers.begin(), workers.end(), g_lastTimed.second) != workers.end()))
+            g_lastTimed = {-1, nullptr};
+    }
+    for (hipStream_t ws : workers) (void)hipStreamSynchronize(ws);
+    for (auto& S : gone) freeScratch(S);
+    for (hipStream_t ws : workers) (void)hipStreamDestroy(ws);
     return hipSuccess;
 }
 
@@ -836,21 +1073,6 @@ struct BhAccumulator
     hipGraphExec_t exec = nullptr;
 };
 
-namespace {
-
-// Autotune cache: schedule chosen per workload class.
-struct TuneKey
-{
-    int kerr, w, h, disk, maxSteps, band, math;
-    bool operator<(const TuneKey& o) const
-    {
-        return std::tie(kerr, w, h, disk, maxSteps, band, math) <
-               std::tie(o.kerr, o.w, o.h, o.disk, o.maxSteps, o.band, o.math);
-    }
-};
-std::map<TuneKey, std::pair<BhSchedule, float>> g_tuned;
-
-}  // namespace
 
 extern "C" {
 
@@ -877,7 +1099,7 @@ bh_error_t bhTraceRecords(BhRayRecord* devOut, const RenderParams& p, bh_stream_
     Scratch* S = nullptr;
     hipError_t e = getScratch(st, 0, &S);
     if (e != hipSuccess) return e;
-    e = enqueueTrace(P, devOut, S, currentSchedule(), st);
+    e = enqueueTrace(P, devOut, S, scheduleFor(P), st);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -917,6 +1139,16 @@ bh_error_t bhRenderBatch(int nViews, const RenderParams* views, bh_uchar4* const
     return (bh_error_t)renderBatch(nViews, views, reinterpret_cast<uchar4* const*>(outs),
                                    reinterpret_cast<float4* const*>(accums), reinterpret_cast<float4* const*>(bloomAs),
                                    reinterpret_cast<float4* const*>(bloomBs), (hipStream_t)stream);
+}
+
+bh_error_t bhRenderBatchPersistent(int nViews, const RenderParams* views, bh_uchar4* const* outs,
+                                   bh_float4* const* accums, bh_float4* const* bloomAs, bh_float4* const* bloomBs,
+                                   bh_stream_t stream)
+{
+    return (bh_error_t)renderBatchPersistent(nViews, views, reinterpret_cast<uchar4* const*>(outs),
+                                             reinterpret_cast<float4* const*>(accums),
+                                             reinterpret_cast<float4* const*>(bloomAs),
+                                             reinterpret_cast<float4* const*>(bloomBs), (hipStream_t)stream);
 }
 
 bh_error_t bhBatchViewLatency(float* msOut, int nViews, bh_stream_t stream)
@@ -1051,7 +1283,7 @@ void bhAccumDestroy(BhAccumulator* acc)
     if (acc->graph) (void)hipGraphDestroy(acc->graph);
     if (acc->capStream)
     {
-        (void)hipStreamSynchronize(acc->capStream);
+        (void)releaseStream(acc->capStream);
         (void)hipStreamDestroy(acc->capStream);
     }
     (void)hipFree(acc->index);
@@ -1150,14 +1382,12 @@ bh_error_t bhAutotune(const RenderParams& p, int reps, BhSchedule* chosen, float
     hipStream_t st = (hipStream_t)stream;
     const BhSchedule cur = currentSchedule();
     const BhPrecision pr = currentPrecision();
-    const TuneKey key{isKerr(P.model) ? 1 : 0, P.width, P.height, P.diskEnabled, P.maxSteps, pr.fp64Band,
-                      cur.mathMode};
+    const TuneKey key = tuneKeyOf(P, pr.fp64Band, cur.mathMode);
     {
         std::lock_guard<std::mutex> lk(g_mu);
         auto it = g_tuned.find(key);
         if (it != g_tuned.end())
         {
-            g_sched = it->second.first;
             if (chosen) *chosen = it->second.first;
             if (bestMs) *bestMs = it->second.second;
             return hipSuccess;
@@ -1204,12 +1434,19 @@ bh_error_t bhAutotune(const RenderParams& p, int reps, BhSchedule* chosen, float
     if (e != hipSuccess) return e;
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        g_sched = best;
         g_tuned[key] = {best, bestT};
     }
     if (chosen) *chosen = best;
     if (bestMs) *bestMs = bestT;
     return hipSuccess;
+}
+
+void bhScheduleFor(const RenderParams& p, BhSchedule* s)
+{
+    if (!s) return;
+    RenderParams P = p;
+    bhSanitizeRenderParams(&P);
+    *s = scheduleFor(P);
 }
 
 void bhAutotuneClear(void)
@@ -1399,7 +1636,13 @@ bh_error_t bhStreamCreate(bh_stream_t* s)
     if (s) *s = (bh_stream_t)st;
     return e;
 }
-bh_error_t bhStreamDestroy(bh_stream_t s) { return hipStreamDestroy((hipStream_t)s); }
+bh_error_t bhStreamRelease(bh_stream_t s) { return releaseStream((hipStream_t)s); }
+bh_error_t bhStreamDestroy(bh_stream_t s)
+{
+    hipError_t e = releaseStream((hipStream_t)s);
+    hipError_t d = hipStreamDestroy((hipStream_t)s);
+    return e != hipSuccess ? e : d;
+}
 bh_error_t bhStreamSynchronize(bh_stream_t s) { return hipStreamSynchronize((hipStream_t)s); }
 const char* bhErrorString(bh_error_t e) { return hipGetErrorString((hipError_t)e); }
 int bhDeviceCount(void)

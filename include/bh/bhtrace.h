@@ -202,24 +202,39 @@ typedef struct BhRayRecord
 bh_error_t bhTraceRecords(BhRayRecord* devOut, BH_PARAMS_IN p, bh_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
-/* Batched views: one persistent launch over many views                     */
+/* Batched views                                                            */
 /* ------------------------------------------------------------------------ */
 /* Renders `nViews` views of any sizes and models, each with the full
  * pipeline of launchRenderPipeline (accumulation, bloom, tonemap) into its
  * own buffers: outs[v] uchar4[W*H], accums[v] float4[W*H], bloomAs[v] /
  * bloomBs[v] float4[ceil(W/2)*ceil(H/2)] (the arrays may be NULL when no view
- * has bloom).  `views` is a host array; each view is copied and sanitized.
- * Results equal per-view launchRenderPipeline calls bit for bit.  The views
- * run as single-view pipelines on six library worker streams forked from
- * `stream` (largest views first, each on the least-loaded worker; each worker
- * and its pipelines' auxiliary stream have a hardware queue of their own;
- * HIP makes such streams blocking, so the batch also orders against work on
- * the legacy null stream), and
- * `stream` continues when all have finished; the single-queue persistent
- * multi-view kernel is disabled in this build. */
+ * has bloom).  `views` is a host array; each view is copied and sanitized,
+ * and every view is checked before any work is enqueued
+ * (hipErrorInvalidValue).  Results equal per-view launchRenderPipeline calls
+ * bit for bit.  The views run as single-view pipelines, each under the
+ * schedule bhScheduleFor gives it, on six library worker streams forked
+ * from `stream` (largest views first, each on the least-loaded worker), and
+ * `stream` continues when all have finished (also after an error).
+ * Library state per caller stream: the six workers and the auxiliary streams
+ * of their pipelines are created with a CU mask of every CU, which gives each
+ * its own hardware queue (12 queues; HIP makes such streams blocking, so the
+ * batch also orders against the legacy null stream); where CU-masked streams
+ * cannot be created, pooled non-blocking streams are used.  Each view index
+ * keeps its own trace records (32 bytes per pixel) between batches, so a view
+ * repeated at the same index orders its work by its previous step counts.
+ * bhStreamRelease frees all of it. */
 bh_error_t bhRenderBatch(int nViews, const RenderParams* views, bh_uchar4* const* outs,
                          bh_float4* const* accums, bh_float4* const* bloomAs,
                          bh_float4* const* bloomBs, bh_stream_t stream);
+/* The same batch as one persistent launch: one device queue of (view, tile)
+ * items over all views, traced by one kernel, then batched shade and post
+ * kernels.  Disabled in this build: it returns hipErrorNotSupported without
+ * enqueueing anything (the kernel hung on its first GPU runs; the cause has
+ * not been shown).  No reference counterpart (the reference renders one
+ * view per launch, black_hole_kernel.cu:108-134). */
+bh_error_t bhRenderBatchPersistent(int nViews, const RenderParams* views, bh_uchar4* const* outs,
+                                   bh_float4* const* accums, bh_float4* const* bloomAs,
+                                   bh_float4* const* bloomBs, bh_stream_t stream);
 /* Per-view completion time (ms after the batch began on the GPU) of the
  * last bhRenderBatch on `stream`; synchronizes the stream. */
 bh_error_t bhBatchViewLatency(float* msOut, int nViews, bh_stream_t stream);
@@ -294,10 +309,16 @@ bh_error_t bhSetSchedule(const BhSchedule* s);   /* -1 (hipErrorInvalidValue-lik
 /* Times the trace kernel of `p` under 18 candidate schedules (workgroup
  * size 64/128/256 x whole tiles / lane refill of 32 rays every 256 steps /
  * 16 rays every 64 steps x cooperative disk sampling off/on; `reps` timed
- * frames each after one warm-up) on `stream`, makes the fastest the current
- * schedule and caches it per (model class, size, disk, step budget, band,
- * math mode).  Output does not depend on the schedule. */
+ * frames each after one warm-up) on `stream` and caches the fastest for
+ * p's workload class (model class, size, disk, step budget, band, math
+ * mode).  Every later frame of that class (launchRenderPipeline,
+ * launchRenderKernel, bhRenderBatch views, bhTraceRecords) runs under it;
+ * other classes keep the process default of bhSetSchedule, which tuning
+ * does not change.  Output does not depend on the schedule. */
 bh_error_t bhAutotune(BH_PARAMS_IN p, int reps, BhSchedule* chosen, float* bestMs, bh_stream_t stream);
+/* The schedule a frame of `p` runs under: its class's tuned schedule, else
+ * the process default. */
+void bhScheduleFor(BH_PARAMS_IN p, BhSchedule* s);
 void bhAutotuneClear(void);
 
 /* ------------------------------------------------------------------------ */
@@ -463,6 +484,11 @@ bh_error_t  bhDeviceFill(void* dst, unsigned char value, size_t bytes);
 bh_error_t  bhCopyDevice(void* dst, const void* src, size_t bytes);   /* device to device */
 bh_error_t  bhDeviceSynchronize(void);
 bh_error_t  bhStreamCreate(bh_stream_t* s);
+/* Frees what the library holds for a stream (per-stream scratch, batch
+ * workers and their queues, batch view records) after waiting for the
+ * stream's work; the stream itself stays valid.  bhStreamDestroy does this
+ * before destroying the stream. */
+bh_error_t  bhStreamRelease(bh_stream_t s);
 bh_error_t  bhStreamDestroy(bh_stream_t s);
 bh_error_t  bhStreamSynchronize(bh_stream_t s);
 const char* bhErrorString(bh_error_t e);

@@ -103,18 +103,35 @@ def test_accumulation_vs_oracle(gpu, oracle):
 
 
 def test_autotune(gpu):
+    """Tuning is per workload class: the tuned schedule applies to frames of
+    the tuned view's class only, the process default and other classes keep
+    theirs, and no output changes."""
     P = scaled_workload(gpu, "kerr09_1080p_high", 0, 320, 180)
+    B = scaled_workload(gpu, "kerr09_1080p_high", 0, 256, 144)      # another class (size)
     before = gpu.get_schedule()
     base, _ = gpu.render_pipeline(P)
+    base_b, _ = gpu.render_pipeline(B)
     try:
         gpu.load().bhAutotuneClear()
+        assert gpu.schedule_for(P).as_dict() == before.as_dict()
         s, ms = gpu.autotune(P, 1)
         say("autotune chose", s.as_dict(), "%.3f ms" % ms)
         assert s.wgSize in (64, 128, 256) and 1 <= s.raysPerRefill <= 64 and ms > 0
-        assert gpu.get_schedule().as_dict() == s.as_dict()
+        assert gpu.schedule_for(P).as_dict() == s.as_dict()
+        assert gpu.get_schedule().as_dict() == before.as_dict()
+        assert gpu.schedule_for(B).as_dict() == before.as_dict()
+        Q = P.copy()
+        Q.sampleIndex, Q.diskTime = 3, 0.25                            # same class
+        assert gpu.schedule_for(Q).as_dict() == s.as_dict()
         tuned, _ = gpu.render_pipeline(P)
         assert np.array_equal(tuned, base)
+        tuned_b, _ = gpu.render_pipeline(B)
+        assert np.array_equal(tuned_b, base_b)
+        # a forced different schedule for the class gives the same frame too
+        gpu.set_schedule(wg=256 if s.wgSize != 256 else 128)
+        assert gpu.schedule_for(P).as_dict() == s.as_dict()
         s2, ms2 = gpu.autotune(P, 1)          # cached
         assert s2.as_dict() == s.as_dict() and ms2 == ms
     finally:
+        gpu.load().bhAutotuneClear()
         gpu.restore_schedule(before)

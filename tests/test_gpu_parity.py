@@ -194,3 +194,55 @@ def test_full_kn_2160p_sampled_rows(gpu, oracle):
     ang = angle_between(g["escDir"].reshape(-1, 3)[both], esc.reshape(-1, 3)[both])
     say("kn rows: agree %.6f  angle<=1e-3 %.6f" % ((go == oo).mean(), (ang <= 1e-3).mean()))
     assert (ang <= 1e-3).mean() >= 0.999
+
+
+# ---------------------------------------------------------------- against pure FP32 reference semantics
+# The product traces rays near the critical curve and polar turning points in
+# FP64 (the default precision); the reference traces every ray in FP32
+# (trace.cuh:466-890).  These tests hold the default product to the parity bar
+# against the oracle with its band rule off -- the reference's own
+# arithmetic -- so the band's departure from the reference is measured.
+class _BandOff:
+    def __init__(self, gpu, oracle):
+        self.gpu, self.oracle = gpu, oracle
+
+    def __enter__(self):
+        self.oracle.set_band(0)
+        return self.oracle
+
+    def __exit__(self, *exc):
+        p = self.gpu.get_precision()
+        self.oracle.set_band(p.fp64Band, p.critBand, p.polarBand)
+
+
+def test_default_product_vs_fp32_reference_kerr09_1080p(gpu, oracle):
+    P = gpu.workload("kerr09_1080p_high", 0)
+    assert gpu.get_precision().fp64Band == 1
+    say("kerr09 1920x1080 t=0: default product (FP64 band on) vs oracle with the band off")
+    rgba, _ = gpu.render_pipeline(P)
+    band_rays, _ = gpu.last_band_stats()
+    with _BandOff(gpu, oracle) as O:
+        ref, _ = O.render_pipeline(P)
+    st = lsb_stats(rgba, ref)
+    say("band rays", band_rays, "lsb", st)
+    assert band_rays > 0
+    assert st["le1"] >= LSB_BAR, st
+
+
+def test_default_product_vs_fp32_reference_kn_reduced(gpu, oracle):
+    P = scaled_workload(gpu, "kn_m07_q04_2160p_high", 0, 960, 540)
+    say("kn 960x540: default product vs oracle with the band off")
+    rgba, _ = gpu.render_pipeline(P)
+    rec = gpu.trace_records(P)
+    with _BandOff(gpu, oracle) as O:
+        ref, _ = O.render_pipeline(P)
+        hdr, esc, steps, disk = O.trace_frame(P, 0, 0, 1)
+    st = lsb_stats(rgba, ref)
+    go = (rec["flags"] & 1).ravel()
+    oo = hdr[..., 3].ravel().astype(np.int32)
+    both = (go == 1) & (oo == 1)
+    ang = angle_between(rec["escDir"].reshape(-1, 3)[both], esc.reshape(-1, 3)[both])
+    say("lsb", st, "outcome agreement %.6f, exit angle <= 1e-3 on %.6f" % ((go == oo).mean(), (ang <= 1e-3).mean()))
+    assert st["le1"] >= LSB_BAR, st
+    assert (go == oo).mean() >= 0.999
+    assert (ang <= 1e-3).mean() >= 0.999
