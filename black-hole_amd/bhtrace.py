@@ -121,6 +121,7 @@ _vp = C.c_void_p
 SIGNATURES = [
     ("launchRenderPipeline", C.c_int, [_vp, _vp, _vp, _vp, _P, _vp]),
     ("launchRenderKernel", C.c_int, [_vp, _P, _vp]),
+    ("launchRenderPipelineNoAlloc", C.c_int, [_vp, _vp, _vp, _vp, _P, _vp]),
     ("bhDefaultRenderParams", None, [_P]),
     ("bhSanitizeRenderParams", C.c_int, [_P]),
     ("bhValidateAndDerive", None, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),

@@ -54,6 +54,26 @@ bh_error_t launchRenderPipeline(bh_uchar4* out, bh_float4* accum,
  * RenderParams&, cudaStream_t) (black_hole_kernel.cu:151-163). */
 bh_error_t launchRenderKernel(bh_uchar4* out, BH_PARAMS_IN p, bh_stream_t stream);
 
+/* launchRenderPipeline that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, semantics and frames as launchRenderPipeline, except:
+ *   - the library's scratch (trace records, counters, FP64 band lists, tile
+ *     order) comes from one static 96 MiB device array per GPU, held until
+ *     the process ends; frames whose scratch does not fit -- 45 bytes per
+ *     pixel plus 8 per 8x8 tile and ~5 KiB, i.e. up to 1920x1080 or about
+ *     2.2 Mpixels of any shape -- are rejected with hipErrorInvalidValue
+ *     before anything is enqueued;
+ *   - no allocation, stream or event is created (the first call on a device
+ *     looks up the array's address); the FP64 band and FP32 tile traces run
+ *     one after the other on `stream`; kernel timers are not recorded; an
+ *     attached learned surrogate is not applied;
+ *   - every call on a GPU shares that GPU's array: calls that overlap on one
+ *     GPU (several streams or host threads) race. */
+bh_error_t launchRenderPipelineNoAlloc(bh_uchar4* out, bh_float4* accum,
+                                       bh_float4* bloomA, bh_float4* bloomB,
+                                       BH_PARAMS_IN p, bh_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Host-side parameter helpers                                              */
 /* ------------------------------------------------------------------------ */
