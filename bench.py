@@ -55,6 +55,14 @@ OTHER_FLOPS = {"schw_256_balanced": (2202725945, 22345464),
                "kn_m07_q04_2160p_high": (1845561825607, 76329881318)}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """One progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +74,8 @@ def parse():
     ap.add_argument("--no-fp64-error", action="store_true",
                     help="skip the per-frame exit error against a full-frame FP64 trace")
     ap.add_argument("--cpu-row-step", type=int, default=4)
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the full-frame LSB comparison of the headline frame with the CPU oracle")
     ap.add_argument("--math", type=int, default=None, help="0 exact, 1 fast (default: library default)")
     ap.add_argument("--band", type=int, default=None, help="FP64 critical-band pass 0/1 (default: library default)")
     ap.add_argument("--surrogate-steps", type=int, default=3000,
@@ -103,8 +113,10 @@ def surrogate_leg(bt, lib, frames, steps, sigma, plain_ms):
     t0 = time.perf_counter()
     s = bt.Surrogate(seed=2)
     s.dataset(4096, 4096, seed=2)
+    progress("surrogate: training set traced")
     loss = s.train(steps, batch=65536, lr=2e-3)
     bt.synchronize()
+    progress(f"surrogate: {steps} Adam steps")
     train_s = time.perf_counter() - t0
     P = frames[0]
     plain = bt.trace_records(P)
@@ -250,6 +262,7 @@ def other_configs(bt, lib, fp64):
                                      "frac_of_nonpacked_fp32": round(ach / FP32_NONPACKED_TFLOPS, 4)}
         if fp64:
             out[name]["fp64_error"] = fp64_error_leg(bt, [P])[0]
+        progress(f"{name}: {out[name]['mrays_s']} Mrays/s")
     views = bt.workload_views("uneven_views", 1, 64)
     B = bt.BatchBuffers(views)
     B.render()
@@ -270,38 +283,115 @@ def other_configs(bt, lib, fp64):
     return out
 
 
-def cpu_baseline(bt, P, row_step, others):
-    """CPU oracle (float, reference semantics) on every `row_step`-th row;
-    with `others`, also a row sample of each of the other workloads."""
+def parity_leg(bt, P):
+    """The headline frame (t = 0) of the product against the CPU oracle:
+    share of pixels whose channels all lie within 1 LSB, against the oracle
+    with its FP64 band rule off (the reference's all-FP32 arithmetic) and
+    with the library's band rule."""
+    import numpy as np
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     O = pyoracle.Oracle()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    rgba, _ = bt.render_pipeline(P)
+    prec = bt.get_precision()
+    out = {}
+    for key, band in (("vs_fp32_reference_semantics", 0), ("vs_oracle_with_band_rule", 1)):
+        O.set_band(band and prec.fp64Band, prec.critBand, prec.polarBand)
+        t0 = time.perf_counter()
+        ref, _ = O.render_pipeline(P)
+        progress(f"parity: oracle frame ({key}) in {time.perf_counter() - t0:.1f} s")
+        d = np.abs(rgba[..., :3].astype(np.int32) - ref[..., :3].astype(np.int32)).max(-1)
+        out[key] = {"le1_lsb": round(float((d <= 1).mean()), 6), "exact": round(float((d == 0).mean()), 6),
+                    "pixels_over_1lsb": int((d > 1).sum()), "max_lsb": int(d.max())}
+    O.set_band(0)
+    out["frame"] = f"{WORKLOAD} t=0, {P.width}x{P.height}, product with FP64 band {'on' if prec.fp64Band else 'off'}"
+    return out
+
+
+def host_cpus():
+    """(logical CPUs this process may run on, physical cores among them,
+    total logical CPUs, CPU model name) from the affinity mask and sysfs /
+    procfs topology."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "core_id") as f1, open(base + "physical_package_id") as f2:
+                cores.add((f2.read().strip(), f1.read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return len(cpus), len(cores), os.cpu_count(), model
+
+
+def cpu_baseline(bt, P, row_step, others, host):
+    """CPU oracle (float, reference semantics: the FP32 integrators, disk, sky
+    and post chain, g++ -O3 -ffp-contract=off, OpenMP dynamic over 16x8-pixel
+    tiles) on every `row_step`-th row: one warm-up pass, then the median of
+    three timed passes over the same rows, one thread per physical core of
+    this host bound close (OMP_PROC_BIND=close, OMP_PLACES=cores, set at the
+    start of main before the OpenMP runtime starts).  With `others`, the same for a row sample of
+    each of the other workloads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    O = pyoracle.Oracle()
+    logical, physical, nproc, model = host
+    threads = physical
+    progress(f"cpu baseline: {model}, nproc {nproc}, affinity {logical} CPUs, {physical} physical cores")
 
     def sample(Q, step, reps=1):
-        t = time.perf_counter()
-        for _ in range(reps):
-            hdr, _, _, _ = O.trace_frame(Q, 0, 0, step, threads)
-        dt = time.perf_counter() - t
+        O.trace_frame(Q, 0, 0, step, threads)            # warm-up
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            for _ in range(reps):
+                hdr, _, _, _ = O.trace_frame(Q, 0, 0, step, threads)
+            times.append(time.perf_counter() - t)
+        dt = sorted(times)[1]
         rays = hdr.shape[0] * hdr.shape[1] * reps
         return rays / dt / 1e6, rays, dt
 
     v, rays, dt = sample(P, row_step)
+    progress(f"cpu baseline: {v:.4f} Mrays/s on {threads} threads")
     res = {"value": round(v, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-           "sample": f"{WORKLOAD} t=0, every {row_step}th row ({rays} rays, {dt:.1f} s wall)"}
+           "sample": f"{WORKLOAD} t=0, every {row_step}th row ({rays} rays), median of 3 passes after a warm-up "
+                     f"({dt:.2f} s each, {rays / dt / 1e6 / threads:.4f} Mrays/s per core)",
+           "host": {"cpu_model": model, "nproc": nproc, "affinity_cpus": logical, "physical_cores": physical,
+                    "omp_proc_bind": "close", "omp_places": "cores",
+                    "flags": "g++ -O3 -ffp-contract=off -fno-fast-math -fopenmp (portable x86-64, no -march=native)"}}
     if others:
         res["other_configs"] = {}
         for name, step, reps in (("schw_256_balanced", 1, 8), ("kn_m07_q04_2160p_high", 16, 1)):
             v, rays, dt = sample(bt.workload(name, 0), step, reps)
             what = "every row" if step == 1 else f"every {step}th row"
             res["other_configs"][name] = {"value": round(v, 4), "unit": "Mrays/s",
-                                          "sample": f"t=0, {what} x{reps} ({rays} rays, {dt:.1f} s wall)"}
+                                          "sample": f"t=0, {what} x{reps} ({rays} rays), median of 3 after a "
+                                                    f"warm-up ({dt:.2f} s)"}
     return res
 
 
 def main():
     a = parse()
+    # the host's CPUs, read before the oracle's OpenMP runtime starts (with
+    # OMP_PROC_BIND it binds this thread to one core, which then is all the
+    # affinity mask shows)
+    host = host_cpus()
+    # thread binding of the CPU oracle's OpenMP runtime (read when the
+    # runtime starts, i.e. when the oracle library is first loaded)
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(pixels shard across GPUs by rows, see DESIGN.md)", file=sys.stderr)
@@ -396,14 +486,22 @@ def main():
     res["post_roofline"]["post_only_ms"] = round(hot, 4)
     res["post_roofline"]["cache_flushed"] = {k: v for k, v in post_roofline(W, H, cold).items()
                                              if k in ("achieved", "frac", "ms")}
+    progress(f"headline: {mrays:.1f} Mrays/s, {ms_per_step:.3f} ms/frame")
     if not a.no_fp64_error:
         res["fp64_error"] = fp64_error_leg(bt, frames)
+        progress("fp64 exit error done")
     if not a.no_other_configs:
         res["other_configs"] = other_configs(bt, lib, not a.no_fp64_error)
+        progress("other configs done")
     if a.surrogate_steps > 0:
         res["surrogate"] = surrogate_leg(bt, lib, frames, a.surrogate_steps, a.surrogate_sigma, avg_trace)
+        progress("surrogate done")
+    if not a.no_parity:
+        res["parity"] = parity_leg(bt, frames[0])
+        progress("parity done")
     if not a.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(bt, frames[0], a.cpu_row_step, not a.no_other_configs)
+        res["cpu_baseline"] = cpu_baseline(bt, frames[0], a.cpu_row_step, not a.no_other_configs, host)
+        progress("cpu baseline done")
     print(json.dumps(res), flush=True)
 
 

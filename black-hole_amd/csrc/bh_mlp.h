@@ -43,6 +43,14 @@ struct MlpTarget
     float pad[2];
 };
 
+// Epilogue of a GEMM launch.
+enum GemmEpi
+{
+    kEpiStore = 0,      // C (+)= acc (f32)
+    kEpiBiasAct = 1,    // z = acc + bias[col]; Zout = z (f32, when set); Aout = bf16(act ? silu(z) : z)
+    kEpiSiluBack = 2,   // g = acc * silu'(Zin); dZout = bf16(g); db[col] += sum over rows of g
+};
+
 struct GemmArgs
 {
     const __bf16* A;
@@ -52,13 +60,18 @@ struct GemmArgs
     int lda, ldb, ldc;
     int transA, transB;
     int accumulate;
+    int epi;                // GemmEpi; the fused epilogues need splitK == 1 and use ldc for every row stride
+    int act;                // kEpiBiasAct: 1 SiLU, 0 identity
+    const float* bias;      // kEpiBiasAct: [N]
+    float* Zout;            // kEpiBiasAct: [M x N] pre-activations (or null)
+    __bf16* Aout;           // kEpiBiasAct: [M x N] activations
+    const float* Zin;       // kEpiSiluBack: [M x N] pre-activations of the layer
+    __bf16* dZout;          // kEpiSiluBack: [M x N]
+    float* db;              // kEpiSiluBack: [N] bias gradient (accumulated)
 };
 
 hipError_t launchGemm(const GemmArgs& g, int splitK, hipStream_t st);
 hipError_t launchFeatures(const float* x, int M, __bf16* X0, hipStream_t st);
-hipError_t launchBiasAct(const float* Y, const float* bias, int M, int N, float* Z, __bf16* A, int silu,
-                         hipStream_t st);
-hipError_t launchSiluBack(const float* dA, const float* Z, int M, int N, __bf16* dZ, float* db, hipStream_t st);
 hipError_t launchLossHead(const float* Y, const float* bias, const MlpTarget* T, int M, __bf16* dOut,
                           __bf16* dOutTrunk, float* lossOut, float* dbHead, hipStream_t st);
 hipError_t launchHeadOut(const float* Y, const float* bias, int M, float* out, hipStream_t st);

@@ -27,55 +27,186 @@ BHK_MLP_INL __bf16 tobf(float x) { return (__bf16)x; }
 // ------------------------------------------------------------------ GEMM
 // C[M x N] (+)= op(A)[M x K] . op(B)[K x N].  op(A) = A (row-major, lda) or
 // A^T (A stored K x M); op(B) = B (row-major K x N) or B^T (B stored N x K).
-// One wavefront per 16 x 16 output tile; gridDim.z splits K (accumulated
-// with atomics when split > 1 or `accumulate`).
-__global__ __launch_bounds__(64) void gemmKernel(GemmArgs g)
+//
+// Workgroups of 4 wavefronts own a 64 x 64 output tile (each wavefront a
+// 32 x 32 quarter: 2 x 2 MFMA tiles) and walk K in chunks of 64: both
+// operand tiles are staged in LDS with k contiguous ([row][k], rows padded
+// to 72 elements), so every MFMA fragment is one 16-byte LDS read whatever
+// the operand's storage order.  An operand stored k-contiguous is copied
+// with 16-byte loads and stores; one stored k-major is read 8 elements of a
+// row at a time (16-byte loads) and transposed on its way into LDS.  The
+// vector path needs 8-element-aligned rows and extents (all the network's
+// shapes); other shapes load element by element with bounds checks.
+// gridDim.z splits K (partial tiles accumulated with atomics).
+//
+// Fused epilogues (kEpiBiasAct, kEpiSiluBack) replace the separate
+// bias/SiLU and SiLU-backward passes: same expressions, same rounding
+// points, no f32 round trip through HBM.
+constexpr int kGBM = 64, kGBN = 64, kGBK = 64;
+constexpr int kGLds = kGBK + 8;    // LDS row stride (bf16): 144 bytes, 16-byte aligned
+
+template <bool kVec>
+BHK_MLP_INL void stageTile(const __bf16* __restrict__ X, int ld, int trans, int rows0, int nRows, int k0, int K,
+                           __bf16* __restrict__ T, int tid)
 {
-    const int lane = threadIdx.x;
-    const int tn = blockIdx.x, tm = blockIdx.y;
-    const int r16 = lane & 15, kq = (lane >> 4) * 8;
-    const int row = tm * 16 + r16;         // A row of this lane
-    const int col = tn * 16 + r16;         // B column of this lane
-    const int kPer = ((g.K + 31) / 32 + gridDim.z - 1) / gridDim.z * 32;
-    const int k0 = blockIdx.z * kPer, k1 = min(g.K, k0 + kPer);
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kb = k0; kb < k1; kb += 32)
-    {
-        bf16x8 a, b;
+    // T[r][k] = op(X)[rows0 + r][k0 + k], r < 64, k < 64: 512 groups of 8
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
+    for (int it = 0; it < 2; ++it)
+    {
+        const int q = tid + 256 * it;
+        if (!trans)
         {
-            const int k = kb + kq + j;
-            __bf16 av = tobf(0.f), bv = tobf(0.f);
-            if (k < k1)
+            // X[row][k]: 8 consecutive k of one row
+            const int r = q >> 3, kg = (q & 7) * 8;
+            const int row = rows0 + r, k = k0 + kg;
+            bf16x8 v;
+            if (kVec)
             {
-                if (row < g.M) av = g.transA ? g.A[(size_t)k * g.lda + row] : g.A[(size_t)row * g.lda + k];
-                if (col < g.N) bv = g.transB ? g.B[(size_t)col * g.ldb + k] : g.B[(size_t)k * g.ldb + col];
-            }
-            a[j] = av;
-            b[j] = bv;
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-    }
-    const int oc = tn * 16 + r16;
+                if (row < nRows && k < K) v = *reinterpret_cast<const bf16x8*>(X + (size_t)row * ld + k);
+                else
+                {
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-    {
-        const int orow = tm * 16 + (lane >> 4) * 4 + v;
-        if (orow < g.M && oc < g.N)
+                    for (int j = 0; j < 8; ++j) v[j] = tobf(0.f);
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = (row < nRows && k + j < K) ? X[(size_t)row * ld + k + j] : tobf(0.f);
+            }
+            *reinterpret_cast<bf16x8*>(T + r * kGLds + kg) = v;
+        }
+        else
         {
-            float* dst = g.C + (size_t)orow * g.ldc + oc;
-            if (g.accumulate || gridDim.z > 1) atomicAdd(dst, acc[v]);
-            else *dst = acc[v];
+            // X[k][row]: 8 consecutive rows of one k, transposed into T
+            const int kr = q >> 3, rg = (q & 7) * 8;
+            const int k = k0 + kr, row = rows0 + rg;
+            bf16x8 v;
+            if (kVec)
+            {
+                if (k < K && row < nRows) v = *reinterpret_cast<const bf16x8*>(X + (size_t)k * ld + row);
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) v[j] = tobf(0.f);
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = (k < K && row + j < nRows) ? X[(size_t)k * ld + row + j] : tobf(0.f);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) T[(rg + j) * kGLds + kr] = v[j];
         }
     }
 }
 
+template <bool kVec>
+__global__ __launch_bounds__(256) void gemmKernel(GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 As[kGBM * kGLds];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[kGBN * kGLds];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * kGBM, n0 = blockIdx.x * kGBN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    const int chunks = (g.K + kGBK - 1) / kGBK;
+    const int per = (chunks + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int c0 = (int)blockIdx.z * per, c1 = min(chunks, c0 + per);
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c = c0; c < c1; ++c)
+    {
+        const int k0 = c * kGBK;
+        stageTile<kVec>(g.A, g.lda, g.transA, m0, g.M, k0, g.K, As, tid);
+        stageTile<kVec>(g.B, g.ldb, !g.transB, n0, g.N, k0, g.K, Bs, tid);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kGBK; ks += 32)
+        {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8*>(As + (wm + i * 16 + r16) * kGLds + ks + kq);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const bf16x8*>(Bs + (wn + j * 16 + r16) * kGLds + ks + kq);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D layout: column r16 of each 16 x 16 tile, rows 4 (lane / 16) + v
+    const bool atom = g.accumulate || gridDim.z > 1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+    {
+        const int col = n0 + wn + j * 16 + r16;
+        const bool cok = col < g.N;
+        float colSum = 0.f;
+        const float bias = (g.epi == kEpiBiasAct && cok) ? g.bias[col] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + v;
+                if (!cok || row >= g.M) continue;
+                const size_t o = (size_t)row * g.ldc + col;
+                const float x = acc[i][j][v];
+                if (g.epi == kEpiStore)
+                {
+                    if (atom) atomicAdd(g.C + o, x);
+                    else g.C[o] = x;
+                }
+                else if (g.epi == kEpiBiasAct)
+                {
+                    const float z = x + bias;
+                    if (g.Zout) g.Zout[o] = z;
+                    const float y = g.act ? z / (1.f + __expf(-z)) : z;
+                    g.Aout[o] = tobf(y);
+                }
+                else
+                {
+                    const float z = g.Zin[o];
+                    const float sg = 1.f / (1.f + __expf(-z));
+                    const float gr = x * (sg * (1.f + z * (1.f - sg)));
+                    g.dZout[o] = tobf(gr);
+                    colSum += gr;
+                }
+            }
+        if (g.epi == kEpiSiluBack)
+        {
+            // the four lanes of a column hold its 32 rows of this wavefront
+            colSum += __shfl_xor(colSum, 16);
+            colSum += __shfl_xor(colSum, 32);
+            if (lane < 16 && cok && colSum != 0.f) atomicAdd(g.db + col, colSum);
+        }
+    }
+}
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
 hipError_t launchGemm(const GemmArgs& g, int splitK, hipStream_t st)
 {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return hipSuccess;
-    dim3 grid((g.N + 15) / 16, (g.M + 15) / 16, splitK < 1 ? 1 : splitK);
-    hipLaunchKernelGGL(gemmKernel, grid, dim3(64), 0, st, g);
+    if (splitK < 1) splitK = 1;
+    if (g.epi != kEpiStore && (splitK > 1 || g.accumulate)) return hipErrorInvalidValue;
+    // vector staging: every operand row starts 16-byte aligned and its
+    // contiguous extent is a multiple of 8 elements
+    const int contA = g.transA ? g.M : g.K, contB = g.transB ? g.K : g.N;
+    const bool vec = aligned16(g.A) && aligned16(g.B) && g.lda % 8 == 0 && g.ldb % 8 == 0 && contA % 8 == 0 &&
+                     contB % 8 == 0;
+    dim3 grid((g.N + kGBN - 1) / kGBN, (g.M + kGBM - 1) / kGBM, splitK);
+    if (vec) hipLaunchKernelGGL(gemmKernel<true>, grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL(gemmKernel<false>, grid, dim3(256), 0, st, g);
     return hipGetLastError();
 }
 
@@ -109,69 +240,18 @@ hipError_t launchFeatures(const float* x, int M, __bf16* X0, hipStream_t st)
     return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ bias + SiLU
-// Z = Y + b (kept in FP32 for the backward pass when Z != null), A = silu(Z)
-// (or Z itself when silu == 0) in bf16.  Y may alias Z.
-__global__ __launch_bounds__(256) void biasActKernel(const float* Y, const float* bias, int M, int N, float* Z,
-                                                     __bf16* A, int silu)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)M * N) return;
-    const float z = Y[i] + bias[i % N];
-    if (Z) Z[i] = z;
-    const float a = silu ? z / (1.f + __expf(-z)) : z;
-    A[i] = tobf(a);
-}
-
-hipError_t launchBiasAct(const float* Y, const float* bias, int M, int N, float* Z, __bf16* A, int silu,
-                         hipStream_t st)
-{
-    const size_t n = (size_t)M * N;
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(biasActKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Y, bias, M, N, Z, A, silu);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------------ SiLU backward
-// dZ = dA * silu'(Z) in bf16 (GEMM operand) and f32 column sums into db.
-__global__ __launch_bounds__(256) void siluBackKernel(const float* dA, const float* Z, int M, int N, __bf16* dZ,
-                                                      float* db)
-{
-    __shared__ float colSum[kMlpHidden];
-    for (int c = threadIdx.x; c < N; c += blockDim.x) colSum[c] = 0.f;
-    __syncthreads();
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < (size_t)M * N)
-    {
-        const float z = Z[i];
-        const float sg = 1.f / (1.f + __expf(-z));
-        const float g = dA[i] * (sg * (1.f + z * (1.f - sg)));
-        dZ[i] = tobf(g);
-        atomicAdd(&colSum[i % N], g);
-    }
-    __syncthreads();
-    for (int c = threadIdx.x; c < N; c += blockDim.x)
-        if (colSum[c] != 0.f) atomicAdd(&db[c], colSum[c]);
-}
-
-hipError_t launchSiluBack(const float* dA, const float* Z, int M, int N, __bf16* dZ, float* db, hipStream_t st)
-{
-    const size_t n = (size_t)M * N;
-    if (!n || N > kMlpHidden) return n ? hipErrorInvalidValue : hipSuccess;
-    hipLaunchKernelGGL(siluBackKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dA, Z, M, N, dZ, db);
-    return hipGetLastError();
-}
-
 // ------------------------------------------------------------------ loss head
 // Output columns: 0, 1 exit deflection minus the spherical base (rad), 2
 // disk-hit logit, 3 hit radius / 10, 4 s = log of the expected squared exit
 // error over sigma0^2.  Per row (mean over the M rows of the batch):
 //   all rows:               BCE(disk logit, disk)
 //   escaped, not band:      kRegW x Huber(|e|; kClip) (quadratic inside the
-//                           clip radius, linear beyond), and
-//                           0.1 (s - log(min(|e|, kClip)^2 / sigma0^2 + 1e-6))^2
-//                           with the error term held fixed (s learns the
-//                           error it will make)
+//                           clip radius, linear beyond), and the negative
+//                           log-likelihood 0.1 (s + q e^-s) of an exponential
+//                           law for q = min(|e|, kClip)^2 / sigma0^2 + 1e-6,
+//                           held fixed: minimal at e^s = q, so s learns
+//                           log E[q], the mean squared error it will make
+//                           (e^-s clamped at |s| <= 20)
 //   escaped, not band, disk: (hitR_pred - hitR / 10)^2
 // dOut gets dL/dY for the head; dOutTrunk the same with the s column zero,
 // so the log-variance head does not shape the hidden layers.
@@ -215,10 +295,10 @@ __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const floa
                 g[0] = 2.f * kMlpRegW * kMlpClip * e0 / en * invM;
                 g[1] = 2.f * kMlpRegW * kMlpClip * e1 / en * invM;
             }
-            const float tgt = __logf(c2 / (kMlpSigma0 * kMlpSigma0) + 1e-6f);
-            const float ds = y[4] - tgt;
-            lReg += 0.1f * ds * ds;
-            g[4] = 0.2f * ds * invM;
+            const float q = c2 / (kMlpSigma0 * kMlpSigma0) + 1e-6f;
+            const float qe = q * __expf(-fminf(fmaxf(y[4], -20.f), 20.f));
+            lReg += 0.1f * (y[4] + qe);
+            g[4] = 0.1f * (1.f - (fabsf(y[4]) < 20.f ? qe : 0.f)) * invM;
             if (t.disk > 0.5f)
             {
                 const float eh = y[3] - t.hitR * 0.1f;

@@ -405,6 +405,22 @@ hipError_t getBandScratch(Scratch* S, size_t n, hipStream_t st)
 
 inline bool isKerr(int model) { return model == BH_KERR || model == BH_KERR_NEWMAN; }
 
+// A camera inside the disk's emitting layer: its own position passes the
+// volumetric sampler's geometric gates (r_in <= rc <= r_out and |y| <= 2.6 H
+// with H = (0.040 + 0.015 rc / r_out) rc; trace.cuh:298-310).  Every camera
+// ray then starts inside the disk, so none can be answered without disk
+// light, and the learned surrogate is not consulted for the frame.
+bool cameraInDiskLayer(const RenderParams& P)
+{
+    if (!P.diskEnabled) return false;
+    const float x = P.camPos.x, y = P.camPos.y, z = P.camPos.z;
+    const float rc = std::sqrt(x * x + z * z);
+    const float rin = std::fmax(P.diskInner, 1e-4f);
+    if (rc < rin || rc > P.diskOuter) return false;
+    const float H = (0.040f + 0.015f * (rc * (1.f / std::fmax(P.diskOuter, rin)))) * rc;
+    return std::fabs(y / std::fmax(H, 1e-4f)) <= 2.6f;
+}
+
 float2 jitterOf(int accumMode, int sampleIndex)
 {
     // R2 sequence (post_process.cuh:47-53), evaluated without contraction
@@ -454,7 +470,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         memo->lastView = v;
         memo->lastRec = rec;
     }
-    const SurrogateRoute sur = fixed ? SurrogateRoute{} : currentSurrogate();
+    const SurrogateRoute sur = (fixed || cameraInDiskLayer(P)) ? SurrogateRoute{} : currentSurrogate();
     bhk::BandArgs BA;
     BA.tileKey = nullptr;
     unsigned char* bandMask = nullptr;

@@ -82,8 +82,7 @@ struct BhSurrogate
     float* Y5 = nullptr;         // [cap x 16]
     __bf16* dOut = nullptr;      // [cap x 16]
     __bf16* dOutT = nullptr;     // [cap x 16], log-variance column zero
-    float* dA = nullptr;         // [cap x 128]
-    __bf16* dZ = nullptr;        // [cap x 128]
+    __bf16* dZ[2] = {};          // [cap x 128] each: gradients at the pre-activations (ping-pong)
     float* loss = nullptr;       // per-step loss terms [lossCap x 3]
     int lossCap = 0;
     // training set
@@ -104,9 +103,8 @@ void freeWorkspace(BhSurrogate* s)
     (void)hipFree(s->dOut);
     (void)hipFree(s->dOutT);
     s->dOutT = nullptr;
-    (void)hipFree(s->dA);
-    (void)hipFree(s->dZ);
-    s->X0 = nullptr; s->Y5 = nullptr; s->dOut = nullptr; s->dA = nullptr; s->dZ = nullptr;
+    for (int k = 0; k < 2; ++k) { (void)hipFree(s->dZ[k]); s->dZ[k] = nullptr; }
+    s->X0 = nullptr; s->Y5 = nullptr; s->dOut = nullptr;
     s->cap = 0;
 }
 
@@ -126,8 +124,8 @@ hipError_t ensureWorkspace(BhSurrogate* s, int rows, hipStream_t st)
     if (e == hipSuccess) e = dalloc(&s->Y5, n * kMlpOutPad);
     if (e == hipSuccess) e = dalloc(&s->dOut, n * kMlpOutPad);
     if (e == hipSuccess) e = dalloc(&s->dOutT, n * kMlpOutPad);
-    if (e == hipSuccess) e = dalloc(&s->dA, n * kMlpHidden);
-    if (e == hipSuccess) e = dalloc(&s->dZ, n * kMlpHidden);
+    if (e == hipSuccess) e = dalloc(&s->dZ[0], n * kMlpHidden);
+    if (e == hipSuccess) e = dalloc(&s->dZ[1], n * kMlpHidden);
     if (e != hipSuccess) { freeWorkspace(s); return e; }
     s->cap = rows;
     return hipSuccess;
@@ -142,11 +140,15 @@ bhk::GemmArgs gemm(const __bf16* A, int lda, int transA, const __bf16* B, int ld
     g.C = C; g.ldc = ldc;
     g.M = M; g.N = N; g.K = K;
     g.accumulate = acc;
+    g.epi = bhk::kEpiStore;
+    g.act = 0;
+    g.bias = nullptr; g.Zout = nullptr; g.Aout = nullptr; g.Zin = nullptr; g.dZout = nullptr; g.db = nullptr;
     return g;
 }
 
 // Forward pass of `rows` rows of inputs x (device [rows x 6]).  Keeps the
 // pre-activations and activations in the workspace; Y5 = head before bias.
+// Hidden layers: GEMM with the bias + SiLU epilogue fused.
 hipError_t forward(BhSurrogate* s, const float* x, int rows, hipStream_t st)
 {
     hipError_t e = bhk::launchFeatures(x, rows, s->X0, st);
@@ -154,15 +156,31 @@ hipError_t forward(BhSurrogate* s, const float* x, int rows, hipStream_t st)
     for (int l = 0; l < kMlpLayers && e == hipSuccess; ++l)
     {
         const int in = bhk::mlpLayerIn(l), out = bhk::mlpLayerOut(l);
-        float* Y = (l < 4) ? s->Z[l] : s->Y5;
-        e = bhk::launchGemm(gemm(X, in, 0, s->pb + kL.w[l], in, 1, Y, out, rows, out, in, 0), 1, st);
-        if (e == hipSuccess && l < 4)
+        bhk::GemmArgs g = gemm(X, in, 0, s->pb + kL.w[l], in, 1, (l < 4) ? nullptr : s->Y5, out, rows, out, in, 0);
+        if (l < 4)
         {
-            e = bhk::launchBiasAct(Y, s->p + kL.b[l], rows, out, Y, s->A[l], 1, st);
-            X = s->A[l];
+            g.epi = bhk::kEpiBiasAct;
+            g.act = 1;
+            g.bias = s->p + kL.b[l];
+            g.Zout = s->Z[l];
+            g.Aout = s->A[l];
         }
+        e = bhk::launchGemm(g, 1, st);
+        if (l < 4) X = s->A[l];
     }
     return e;
+}
+
+// dZ_{l-1} = (dZ_l . W_l) * silu'(Z_{l-1}) and db_{l-1} in one GEMM
+bhk::GemmArgs backData(const __bf16* dY, int ldy, const __bf16* W, int in, int outN, const float* Zprev,
+                       __bf16* dZout, float* db, int rows)
+{
+    bhk::GemmArgs g = gemm(dY, ldy, 0, W, in, 0, nullptr, in, rows, in, outN, 0);
+    g.epi = bhk::kEpiSiluBack;
+    g.Zin = Zprev;
+    g.dZout = dZout;
+    g.db = db;
+    return g;
 }
 
 // Backward pass after forward(): gradients accumulate into s->g (zeroed by
@@ -171,25 +189,24 @@ hipError_t backward(BhSurrogate* s, const bhk::MlpTarget* T, int rows, float* lo
 {
     hipError_t e =
         bhk::launchLossHead(s->Y5, s->p + kL.b[4], T, rows, s->dOut, s->dOutT, lossSlot, s->g + kL.b[4], st);
-    const int split = std::max(1, std::min(64, rows / 2048));
-    // head: dW4 = dOut^T A3, dA3 = dOut W4
+    // weight gradients reduce over the batch: split it into chunks of 512 rows
+    const int split = std::max(1, std::min(128, rows / 512));
+    // head: dW4 = dOut^T A3; dZ3 = (dOut W4) * silu'(Z3), db3
     if (e == hipSuccess)
         e = bhk::launchGemm(gemm(s->dOut, kMlpOutPad, 1, s->A[3], kMlpHidden, 0, s->g + kL.w[4], kMlpHidden,
                                  kMlpOutPad, kMlpHidden, rows, 1), split, st);
     if (e == hipSuccess)
-        e = bhk::launchGemm(gemm(s->dOutT, kMlpOutPad, 0, s->pb + kL.w[4], kMlpHidden, 0, s->dA, kMlpHidden, rows,
-                                 kMlpHidden, kMlpOutPad, 0), 1, st);
-    for (int l = 3; l >= 0 && e == hipSuccess; --l)
+        e = bhk::launchGemm(backData(s->dOutT, kMlpOutPad, s->pb + kL.w[4], kMlpHidden, kMlpOutPad, s->Z[3],
+                                     s->dZ[1], s->g + kL.b[3], rows), 1, st);
+    for (int l = 3, cur = 1; l >= 0 && e == hipSuccess; --l, cur ^= 1)
     {
         const int in = bhk::mlpLayerIn(l);
-        e = bhk::launchSiluBack(s->dA, s->Z[l], rows, kMlpHidden, s->dZ, s->g + kL.b[l], st);
         const __bf16* X = (l == 0) ? s->X0 : s->A[l - 1];
-        if (e == hipSuccess)
-            e = bhk::launchGemm(gemm(s->dZ, kMlpHidden, 1, X, in, 0, s->g + kL.w[l], in, kMlpHidden, in, rows, 1),
-                                split, st);
+        e = bhk::launchGemm(gemm(s->dZ[cur], kMlpHidden, 1, X, in, 0, s->g + kL.w[l], in, kMlpHidden, in, rows, 1),
+                            split, st);
         if (e == hipSuccess && l > 0)
-            e = bhk::launchGemm(gemm(s->dZ, kMlpHidden, 0, s->pb + kL.w[l], in, 0, s->dA, kMlpHidden, rows, in,
-                                     kMlpHidden, 0), 1, st);
+            e = bhk::launchGemm(backData(s->dZ[cur], kMlpHidden, s->pb + kL.w[l], in, kMlpHidden, s->Z[l - 1],
+                                         s->dZ[cur ^ 1], s->g + kL.b[l - 1], rows), 1, st);
     }
     return e;
 }

@@ -107,8 +107,11 @@ REG_W = 1e3
 def loss_grad(p, x, t, bf=False, tgt=None):
     """Loss sums [regression + log-variance, disk BCE, hit radius] over rows
     and the gradient of the mean loss w.r.t. the parameter vector (library
-    layout).  The log-variance target log(min(|e|, clip)^2 / sigma0^2 + 1e-6)
-    is held fixed in the gradient; pass `tgt` to fix it explicitly (finite
+    layout).  The log-variance head s is fitted by the negative
+    log-likelihood 0.1 (s + q e^-s) of an exponential law for the squared
+    exit error, q = min(|e|, clip)^2 / sigma0^2 + 1e-6, whose minimum is
+    e^s = q, so s estimates log E[q] (exp clamped at |s| <= 20); q is held
+    fixed in the gradient; pass `tgt` (q) to fix it explicitly (finite
     differences).  Returns (losses, grad, tgt)."""
     r = bf16 if bf else (lambda a: np.asarray(a, np.float64))
     y, c = forward(p, x, bf)
@@ -125,15 +128,17 @@ def loss_grad(p, x, t, bf=False, tgt=None):
     c2 = np.minimum(en2, CLIP ** 2)
     inside = en2 <= CLIP ** 2
     if tgt is None:
-        tgt = np.log(c2 / float(SIGMA0) ** 2 + 1e-6)
-    ds = y[:, 4] - tgt
+        tgt = c2 / float(SIGMA0) ** 2 + 1e-6
+    s4 = y[:, 4]
+    live = np.abs(s4) < 20.0
+    qe = tgt * np.exp(-np.clip(s4, -20.0, 20.0))
     en = np.sqrt(en2)
     huber = np.where(inside, en2, 2.0 * CLIP * en - CLIP ** 2)
-    l_reg = np.where(reg, REG_W * huber + 0.1 * ds ** 2, 0.0)
+    l_reg = np.where(reg, REG_W * huber + 0.1 * (s4 + qe), 0.0)
     scale_g = np.where(inside, 1.0, CLIP / np.maximum(en, 1e-300))
     g[:, 0] = np.where(reg, 2.0 * REG_W * e0 * scale_g / n, 0.0)
     g[:, 1] = np.where(reg, 2.0 * REG_W * e1 * scale_g / n, 0.0)
-    g[:, 4] = np.where(reg, 0.2 * ds / n, 0.0)
+    g[:, 4] = np.where(reg, 0.1 * (1.0 - np.where(live, qe, 0.0)) / n, 0.0)
     hit = reg & (disk > 0.5)
     eh = y[:, 3] - t["hitR"] * np.float32(0.1)
     l_hit = np.where(hit, eh ** 2, 0.0)
@@ -171,5 +176,6 @@ def mean_loss(p, x, t, tgt, with_var=True):
     if not with_var:
         y, _ = forward(p, x, False)
         reg = (t["escaped"] > 0.5) & (t["band"] < 0.5)
-        total -= np.where(reg, 0.1 * (y[:, 4] - tgt) ** 2, 0.0).sum()
+        s4 = y[:, 4]
+        total -= np.where(reg, 0.1 * (s4 + tgt * np.exp(-np.clip(s4, -20.0, 20.0))), 0.0).sum()
     return total / x.shape[0]

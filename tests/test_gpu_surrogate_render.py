@@ -61,8 +61,28 @@ def accepted_mask(rec):
     return ((rec["flags"] >> 8) == 0) & ((rec["flags"] & 1) == 1)
 
 
+def test_camera_in_disk_layer_skips_the_network(gpu):
+    """The headline camera sits inside the disk's emitting layer (|dz| = 1.7
+    scale heights at r = 15): every ray starts in the disk, so the network is
+    not consulted at all and the frame is the plain frame."""
+    P = scaled_workload(gpu, "kerr09_1080p_high", 0, W, H)
+    base, _ = gpu.render_pipeline(P)
+    s = gpu.Surrogate(seed=5)
+    try:
+        gpu.set_surrogate(s, 1.0, 1.0)
+        img, _ = gpu.render_pipeline(P)
+        cand, acc = gpu.last_surrogate_stats()
+    finally:
+        gpu.set_surrogate(None)
+        s.close()
+    say("camera in the disk layer: candidates", cand, "accepted", acc)
+    assert cand == 0 and acc == 0
+    assert np.array_equal(img, base)
+
+
 def test_untrained_network_changes_nothing(gpu):
     P = scaled_workload(gpu, "kerr09_1080p_high", 0, W, H)
+    P.diskEnabled = 0
     base, _ = gpu.render_pipeline(P)
     s = gpu.Surrogate(seed=5)
     try:
