@@ -194,6 +194,7 @@ SIGNATURES = [
     ("bhSurrogateLossGrad", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp]),
     ("bhSetSurrogate", C.c_int, [_vp, C.c_float, C.c_float]),
     ("bhLastSurrogateStats", C.c_int, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), _vp]),
+    ("bhLastSurrogateKernelMs", C.c_int, [C.POINTER(C.c_float), _vp]),
     ("bhMlpGemmTest", C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
                                 C.c_int, C.c_int, _vp]),
     ("bhWritePPM", C.c_int, [C.c_char_p, _vp, C.c_int, C.c_int]),

@@ -19,6 +19,11 @@ constexpr int kMlpLayers = 5;                         // 4 hidden + output
 constexpr int kMlpOut = 5;                            // deflection x2, disk logit, hit radius, log-variance
 constexpr int kMlpOutPad = 16;
 constexpr float kMlpSigma0 = 1e-3f;                   // scale of the predicted exit-direction spread (rad)
+// The exit outputs (columns 0, 1) are in milliradians: targets of a few
+// mrad are then O(1) outputs, whose last-layer weights Adam moves by steps
+// of the learning rate without swamping them (in radians the late steps of
+// a 1e-4 learning rate alone moved the outputs by ~1e-3 rad).
+constexpr float kMlpOutPerRad = 1000.f;
 
 // Frequencies 1.25^f * pi / 2, f = 0..15 (pi/2 .. 44.6 per unit input).
 __host__ __device__ inline float mlpFreq(int f)

@@ -241,22 +241,24 @@ hipError_t launchFeatures(const float* x, int M, __bf16* X0, hipStream_t st)
 }
 
 // ------------------------------------------------------------------ loss head
-// Output columns: 0, 1 exit deflection minus the spherical base (rad), 2
+// Output columns: 0, 1 exit deflection minus the spherical base (mrad), 2
 // disk-hit logit, 3 hit radius / 10, 4 s = log of the expected squared exit
 // error over sigma0^2.  Per row (mean over the M rows of the batch):
 //   all rows:               BCE(disk logit, disk)
-//   escaped, not band:      kRegW x Huber(|e|; kClip) (quadratic inside the
-//                           clip radius, linear beyond), and the negative
-//                           log-likelihood 0.1 (s + q e^-s) of an exponential
-//                           law for q = min(|e|, kClip)^2 / sigma0^2 + 1e-6,
+//   escaped, not band:      kRegW x Huber(|e|; kClip) with e in mrad
+//                           (quadratic inside the clip radius, linear
+//                           beyond), and the negative log-likelihood
+//                           0.1 (s + q e^-s) of an exponential law for
+//                           q = min(|e|, kClip)^2 + 1e-6 (|e| / sigma0 with
+//                           sigma0 = 1 mrad, the output unit),
 //                           held fixed: minimal at e^s = q, so s learns
 //                           log E[q], the mean squared error it will make
 //                           (e^-s clamped at |s| <= 20)
 //   escaped, not band, disk: (hitR_pred - hitR / 10)^2
 // dOut gets dL/dY for the head; dOutTrunk the same with the s column zero,
 // so the log-variance head does not shape the hidden layers.
-constexpr float kMlpClip = 0.02f;
-constexpr float kMlpRegW = 1e3f;
+constexpr float kMlpClip = 20.f;      // mrad
+constexpr float kMlpRegW = 1e-3f;   // (mrad^2: the balance against the disk BCE of a 1e3 weight on rad^2)
 
 __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const float* bias, const MlpTarget* T, int M,
                                                       __bf16* dOut, __bf16* dOutTrunk, float* lossOut)
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const floa
         g[2] = (p - t.disk) * invM;
         if (t.escaped > 0.5f && t.band < 0.5f)
         {
-            const float e0 = y[0] - t.d0, e1 = y[1] - t.d1;
+            const float e0 = y[0] - t.d0 * kMlpOutPerRad, e1 = y[1] - t.d1 * kMlpOutPerRad;
             const float en2 = e0 * e0 + e1 * e1;
             const float c2 = fminf(en2, kMlpClip * kMlpClip);
             if (en2 <= kMlpClip * kMlpClip)
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const floa
                 g[0] = 2.f * kMlpRegW * kMlpClip * e0 / en * invM;
                 g[1] = 2.f * kMlpRegW * kMlpClip * e1 / en * invM;
             }
-            const float q = c2 / (kMlpSigma0 * kMlpSigma0) + 1e-6f;
+            const float q = c2 + 1e-6f;
             const float qe = q * __expf(-fminf(fmaxf(y[4], -20.f), 20.f));
             lReg += 0.1f * (y[4] + qe);
             g[4] = 0.1f * (1.f - (fabsf(y[4]) < 20.f ? qe : 0.f)) * invM;

@@ -1,11 +1,14 @@
 // ---------------------------------------------------------------------------
 // bh_route.hip -- surrogate inference fused with the router.
 //
-// Each wavefront runs the whole network for 32 candidate rays at a time:
-// Fourier features into LDS, then the five layers as 16x16x32 BF16 MFMA
-// tiles (activations round-trip through LDS in the A-operand layout, weights
-// read as B operands from global memory / L2, FP32 accumulation, bias and
-// SiLU in the epilogue), and finally the routing decision per ray: a ray is
+// A workgroup of four wavefronts runs the whole network for 128 candidate
+// rays at a time (32 per wavefront): Fourier features into LDS, then the
+// five layers as 16x16x32 BF16 MFMA tiles.  Each layer's weights are staged
+// once per pass into LDS by the whole workgroup (16-byte copies, rows padded
+// by 8 elements) and read from there as B operands by all four wavefronts;
+// activations round-trip through each wavefront's own LDS rows in the
+// A-operand layout, with FP32 accumulation, bias and SiLU in the epilogue.
+// Finally the routing decision per ray: a ray is
 // accepted when its predicted disk-hit probability and exit-direction spread
 // are both below the thresholds; it then gets its record (exit direction
 // from the predicted deflection, no disk light) and mask value 2, so the RK4
@@ -24,7 +27,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kRouteRows = 32;                 // rays per wavefront pass
 constexpr int kStrideA = kMlpIn + 8;           // LDS row strides (bf16), 16-byte multiples
 constexpr int kStrideB = kMlpHidden + 8;
-constexpr int kRouteWaves = 2;                 // wavefronts per workgroup
+constexpr int kRouteWaves = 4;                 // wavefronts per workgroup
 
 struct alignas(16) RouteLds
 {
@@ -32,10 +35,13 @@ struct alignas(16) RouteLds
     __bf16 b[kRouteRows * kStrideB];
     float out[kRouteRows * 8];
 };
+// staged weights of one layer: out rows of in + 8 elements (layer 0 is the largest)
+constexpr int kRouteWElems = kMlpHidden * kStrideA;
 
 __global__ __launch_bounds__(64 * kRouteWaves) void surrogateRouteKernel(RouteArgs A)
 {
     __shared__ RouteLds lds[kRouteWaves];
+    __shared__ __attribute__((aligned(16))) __bf16 Wlds[kRouteWElems];
     const unsigned n = *A.count;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     RouteLds& L = lds[wave];
@@ -66,36 +72,63 @@ __global__ __launch_bounds__(64 * kRouteWaves) void surrogateRouteKernel(RouteAr
             if (half)
                 for (int c = kMlpFF + kMlpInputs; c < kMlpIn; ++c) o[c] = (__bf16)0.f;
         }
-        __syncthreads();
         // ---- layers
         for (int l = 0; l < kMlpLayers; ++l)
         {
             const int in = mlpLayerIn(l), out = mlpLayerOut(l);
+            const int ws = in + 8;
             const __bf16* src = (l & 1) ? L.b : L.a;
             const int ss = (l & 1) ? kStrideB : kStrideA;
             __bf16* dst = (l & 1) ? L.a : L.b;
             const int ds = (l & 1) ? kStrideA : kStrideB;
-            const __bf16* Wl = A.W + A.wOff[l];
             const float* bl = A.Pf + A.bOff[l];
-            for (int ot = 0; ot < out / 16; ++ot)
+            // stage the layer's weights (every wavefront is past the previous
+            // layer's reads of Wlds)
+            __syncthreads();
             {
-                f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-                const __bf16* wrow = Wl + (size_t)(ot * 16 + r16) * in + kq;
-                for (int kb = 0; kb < in; kb += 32)
+                const __bf16* Wl = A.W + A.wOff[l];
+                const int vecPerRow = in / 8;
+                for (int q = threadIdx.x; q < out * vecPerRow; q += 64 * kRouteWaves)
                 {
-                    const bf16x8 bw = *reinterpret_cast<const bf16x8*>(wrow + kb);
-                    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(src + r16 * ss + kb + kq);
-                    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(src + (16 + r16) * ss + kb + kq);
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw, acc1, 0, 0, 0);
+                    const int o = q / vecPerRow, c = (q - o * vecPerRow) * 8;
+                    *reinterpret_cast<bf16x8*>(Wlds + o * ws + c) =
+                        *reinterpret_cast<const bf16x8*>(Wl + (size_t)o * in + c);
                 }
+            }
+            __syncthreads();
+            // all output tiles at once: 2 x nOt independent accumulators, the
+            // A fragments of a k step read once
+            const int nOt = out / 16;
+            f32x4 acc[kMlpHidden / 16][2];
+#pragma unroll
+            for (int ot = 0; ot < kMlpHidden / 16; ++ot)
+                acc[ot][0] = acc[ot][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int kb = 0; kb < in; kb += 32)
+            {
+                const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(src + r16 * ss + kb + kq);
+                const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(src + (16 + r16) * ss + kb + kq);
+#pragma unroll
+                for (int ot = 0; ot < kMlpHidden / 16; ++ot)
+                {
+                    if (ot < nOt)
+                    {
+                        const bf16x8 bw = *reinterpret_cast<const bf16x8*>(Wlds + (ot * 16 + r16) * ws + kb + kq);
+                        acc[ot][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw, acc[ot][0], 0, 0, 0);
+                        acc[ot][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw, acc[ot][1], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int ot = 0; ot < kMlpHidden / 16; ++ot)
+            {
+                if (ot >= nOt) continue;
                 const int col = ot * 16 + r16;
                 const float bias = bl[col];
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
                 {
                     const int row0 = (lane >> 4) * 4 + v;
-                    const float z0 = acc0[v] + bias, z1 = acc1[v] + bias;
+                    const float z0 = acc[ot][0][v] + bias, z1 = acc[ot][1][v] + bias;
                     if (l < kMlpLayers - 1)
                     {
                         dst[row0 * ds + col] = (__bf16)(z0 / (1.f + __expf(-z0)));
@@ -108,8 +141,8 @@ __global__ __launch_bounds__(64 * kRouteWaves) void surrogateRouteKernel(RouteAr
                     }
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();   // the head outputs of every lane are in L.out
         // ---- routing: one lane per ray
         bool acc = false;
         if (lane < kRouteRows && base + lane < n)
@@ -129,7 +162,9 @@ __global__ __launch_bounds__(64 * kRouteWaves) void surrogateRouteKernel(RouteAr
                 float rc, thc;
                 camBasis(mk3(A.P.camPos.x, A.P.camPos.y, A.P.camPos.z), er, eth, eph, rc, thc);
                 const RayFrame F = rayFrame(d, er, eth, eph, rc);
-                const v3 e = exitFromDeflection(F, y[0] + sphDeflection(A.P.M, A.P.Qc, rc, F.psi, 120.f * A.P.M), y[1]);
+                const v3 e = exitFromDeflection(
+                    F, y[0] * (1.f / kMlpOutPerRad) + sphDeflection(A.P.M, A.P.Qc, rc, F.psi, 120.f * A.P.M),
+                    y[1] * (1.f / kMlpOutPerRad));
                 float4* r = reinterpret_cast<float4*>(A.rec + pix);
                 r[0] = make_float4(e.x, e.y, e.z, __int_as_float(1));
                 r[1] = make_float4(0.f, 0.f, 0.f, 1.f);

@@ -188,6 +188,8 @@ struct Scratch
     size_t candCap = 0;
     hipEvent_t bandEv[2] = {nullptr, nullptr};   // timing: band pass start / end
     hipEvent_t tileEv[2] = {nullptr, nullptr};   // timing: FP32 tile kernel start / end (its own stream)
+    hipEvent_t routeEv[2] = {nullptr, nullptr};  // timing: surrogate inference + routing start / end
+    bool routeTimed = false;                     // routeEv hold the last timed frame's route kernel
     bool tileTimed = false;                      // tileEv hold the last timed frame
     // the FP32 tile trace runs on `aux` concurrently with the FP64 band
     // kernel on the caller's stream: fork after classification, join before
@@ -450,6 +452,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     if (!memo) memo = fixed ? &g_fixedMemo[(dev >= 0 && dev < 64) ? dev : 0] : &Sc->memo;
+    if (Sc) Sc->routeTimed = false;
     const int kerr = isKerr(P.model) ? 1 : 0;
     const int tiles = sc.raysPerRefill >= 64 ? 1 : 0;
     const int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, sc.diskCoop, tiles)
@@ -558,15 +561,27 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             R.Pf = net.P;
             for (int l = 0; l < bhk::kMlpLayers; ++l) { R.wOff[l] = net.wOff[l]; R.bOff[l] = net.bOff[l]; }
             R.sigmaMax = sur.sigmaMax;
-            R.diskProbMax = sur.diskProbMax;
+            // without a disk no ray can gather disk light: the disk-hit head
+            // (trained with the disk on) is not consulted
+            R.diskProbMax = P.diskEnabled ? sur.diskProbMax : 2.f;
             R.rec = rec;
             R.mask = bandMask;
             R.accepted = counter + kAcceptSlot;
+            const bool rt = allowTimers && g_timers.load() != 0;
+            if (rt && !Sc->routeEv[0])
+            {
+                if (hipEventCreate(&Sc->routeEv[0]) != hipSuccess || hipEventCreate(&Sc->routeEv[1]) != hipSuccess)
+                    return hipErrorOutOfMemory;
+            }
+            Sc->routeTimed = rt;
+            if (rt) (void)hipEventRecord(Sc->routeEv[0], st);
             e = bhk::launchSurrogateRoute(R, (int)n, numCUs(dev), st);
+            if (rt && e == hipSuccess) (void)hipEventRecord(Sc->routeEv[1], st);
             if (e != hipSuccess) return e;
         }
     }
     bhk::TraceArgs A;
+    const BhSchedule& scT = sc;
     A.bandMask = bandMask;
     A.tileOrder = BA.tileKey ? tileOrder : nullptr;
     A.P = P;
@@ -575,15 +590,16 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     A.tilesX = (P.width + 7) / 8;
     const int tilesY = (P.height + 7) / 8;
     A.totalIds = A.tilesX * tilesY * 64;
-    A.chunk = sc.stepsPerChunk;
-    A.refillMin = sc.raysPerRefill;
-    A.coop = sc.diskCoop;
+    A.chunk = scT.stepsPerChunk;
+    A.refillMin = scT.raysPerRefill;
+    A.coop = scT.diskCoop;
     A.jitter = jitterOf(P.accumMode, P.sampleIndex);
     A.dynIndex = dynIndex;
-    long long maxUseful = ((long long)A.totalIds + sc.wgSize - 1) / sc.wgSize;
+    long long maxUseful = ((long long)A.totalIds + scT.wgSize - 1) / scT.wgSize;
+    const long long wantT = want;
     bhk::LaunchShape S;
-    S.wgSize = sc.wgSize;
-    S.blocks = (int)(want < maxUseful ? want : maxUseful);
+    S.wgSize = scT.wgSize;
+    S.blocks = (int)(wantT < maxUseful ? wantT : maxUseful);
     if (S.blocks < 1) S.blocks = 1;
     hipError_t e = band ? hipSuccess : bhk::launchResetCounter(counter, st);   // with the band: reset above
     if (e != hipSuccess) return e;
@@ -1018,6 +1034,8 @@ void freeScratch(Scratch& S)
     for (auto& ev : S.bandEv)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : S.tileEv)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : S.routeEv)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& sl : S.slots) (void)hipFree(sl.rec);
     if (S.aux)
@@ -1538,6 +1556,24 @@ bh_error_t bhLastSurrogateStats(unsigned* candidates, unsigned* accepted, bh_str
         e = hipMemcpy(candidates, S->counter + kCandCountSlot, sizeof(unsigned), hipMemcpyDeviceToHost);
     if (e == hipSuccess && accepted)
         e = hipMemcpy(accepted, S->counter + kAcceptSlot, sizeof(unsigned), hipMemcpyDeviceToHost);
+    return e;
+}
+
+bh_error_t bhLastSurrogateKernelMs(float* ms, bh_stream_t stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    Scratch* S = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_scratch.find({dev, st});
+        if (it == g_scratch.end() || !it->second.routeTimed || !it->second.routeEv[0]) return hipErrorNotReady;
+        S = &it->second;
+    }
+    e = hipStreamSynchronize(st);
+    if (e == hipSuccess && ms) e = hipEventElapsedTime(ms, S->routeEv[0], S->routeEv[1]);
     return e;
 }
 

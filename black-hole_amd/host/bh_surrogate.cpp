@@ -258,9 +258,9 @@ bh_error_t bhSurrogateCreate(uint64_t seed, BhSurrogate** out)
     {
         const int in = bhk::mlpLayerIn(l), outN = bhk::mlpLayerOut(l);
         const int realOut = (l == kMlpLayers - 1) ? kMlpOut : outN;
-        // output layer starts near zero: the exit heads predict the residual
-        // over the analytic base, whose typical size is 1e-3 rad
-        const double lim = (l == kMlpLayers - 1) ? 1e-3 * std::sqrt(3.0 / in) : std::sqrt(6.0 / in);
+        // output layer starts small: the exit heads predict the residual
+        // over the analytic base in mrad (typically a few)
+        const double lim = (l == kMlpLayers - 1) ? 0.05 * std::sqrt(3.0 / in) : std::sqrt(6.0 / in);
         for (int o = 0; o < realOut; ++o)
             for (int i = 0; i < in; ++i) h[kL.w[l] + (size_t)o * in + i] = (float)r.uniform(-lim, lim);
     }

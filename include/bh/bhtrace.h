@@ -434,7 +434,9 @@ bh_error_t bhSurrogateDatasetView(BhSurrogate* s, float** inputs, void** targets
  * (may be NULL) receives per-step [regression, disk BCE, hit radius] means. */
 bh_error_t bhSurrogateTrain(BhSurrogate* s, int steps, int batch, float lr, float* lossHost, bh_stream_t stream);
 /* Network outputs for n rows of normalised inputs (device float[n*6] ->
- * device float[n*5]). */
+ * device float[n*5]): exit deflection minus the weak-field term (two
+ * columns, milliradians), disk-hit logit, hit radius / 10, and s, the log of
+ * the expected squared exit error in mrad^2. */
 bh_error_t bhSurrogateForward(BhSurrogate* s, const float* devInputs, int n, float* devOut, bh_stream_t stream);
 /* Loss terms (sums over rows) and the gradient of the mean loss with respect
  * to every parameter for n rows (host outputs; synchronizes). */
@@ -444,12 +446,19 @@ bh_error_t bhSurrogateLossGrad(BhSurrogate* s, const float* devInputs, const voi
  * FP64 band pass on, every escaping non-band camera ray inside the training
  * domain (psi <= 65 deg, camera radius 6..40 rs, q <= 0.6) is evaluated by
  * the network; rays whose predicted disk-hit probability is below
- * diskProbMax and predicted exit spread below sigmaMax (rad) take the
- * predicted exit direction and skip RK4; all others are traced by RK4.  The
- * surrogate must outlive its attachment. */
+ * diskProbMax (not consulted when the disk is off) and predicted exit
+ * spread (the RMS exit error the network expects) below sigmaMax (rad) take
+ * the predicted exit direction and skip RK4; all others are traced by RK4.
+ * Frames whose camera sits inside the disk's emitting layer (every ray
+ * starts in the disk) do not consult the network.  The surrogate must
+ * outlive its attachment. */
 bh_error_t bhSetSurrogate(BhSurrogate* s, float sigmaMax, float diskProbMax);
 /* Candidates and accepted rays of the last frame on `stream` (synchronizes). */
 bh_error_t bhLastSurrogateStats(unsigned* candidates, unsigned* accepted, bh_stream_t stream);
+/* HIP-event time (ms) of the network inference and routing kernel of the
+ * last frame on `stream` traced with kernel timers on and a surrogate
+ * consulted; hipErrorNotReady otherwise.  Synchronizes the stream. */
+bh_error_t bhLastSurrogateKernelMs(float* ms, bh_stream_t stream);
 /* The MFMA GEMM used by the network, exposed for tests: C (+)= op(A) op(B)
  * with bf16 operands (device) and f32 C (device; zero it when splitK > 1). */
 bh_error_t bhMlpGemmTest(const void* A, int lda, int transA, const void* B, int ldb, int transB, float* C, int ldc,

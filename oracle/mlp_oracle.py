@@ -100,8 +100,9 @@ def forward(p, x, bf=False):
     return y, cache
 
 
-CLIP = 0.02
-REG_W = 1e3
+CLIP = 20.0            # mrad
+REG_W = 1e-3
+OUT_PER_RAD = np.float32(1000.0)   # exit outputs in milliradians
 
 
 def loss_grad(p, x, t, bf=False, tgt=None):
@@ -109,7 +110,8 @@ def loss_grad(p, x, t, bf=False, tgt=None):
     and the gradient of the mean loss w.r.t. the parameter vector (library
     layout).  The log-variance head s is fitted by the negative
     log-likelihood 0.1 (s + q e^-s) of an exponential law for the squared
-    exit error, q = min(|e|, clip)^2 / sigma0^2 + 1e-6, whose minimum is
+    exit error, q = min(|e|, clip)^2 + 1e-6 with e in mrad (the exit
+    outputs' unit, sigma0 = 1 mrad), whose minimum is
     e^s = q, so s estimates log E[q] (exp clamped at |s| <= 20); q is held
     fixed in the gradient; pass `tgt` (q) to fix it explicitly (finite
     differences).  Returns (losses, grad, tgt)."""
@@ -122,13 +124,13 @@ def loss_grad(p, x, t, bf=False, tgt=None):
     y2 = y[:, 2]
     l_bce = np.maximum(y2, 0) - y2 * disk + np.log1p(np.exp(-np.abs(y2)))
     g[:, 2] = (1.0 / (1.0 + np.exp(-y2)) - disk) / n
-    e0 = y[:, 0] - t["d0"]
-    e1 = y[:, 1] - t["d1"]
+    e0 = y[:, 0] - (t["d0"].astype(np.float32) * OUT_PER_RAD).astype(np.float64)
+    e1 = y[:, 1] - (t["d1"].astype(np.float32) * OUT_PER_RAD).astype(np.float64)
     en2 = e0 ** 2 + e1 ** 2
     c2 = np.minimum(en2, CLIP ** 2)
     inside = en2 <= CLIP ** 2
     if tgt is None:
-        tgt = c2 / float(SIGMA0) ** 2 + 1e-6
+        tgt = c2 + 1e-6
     s4 = y[:, 4]
     live = np.abs(s4) < 20.0
     qe = tgt * np.exp(-np.clip(s4, -20.0, 20.0))
