@@ -150,6 +150,7 @@ SIGNATURES = [
     ("bhProbeGrid", C.c_int, [_vp, _P, C.c_int, C.c_int, _vp]),
     ("bhProbeDirs", C.c_int, [_vp, _vp, C.c_int, _P, _vp]),
     ("bhTraceRecords", C.c_int, [_vp, _P, _vp]),
+    ("bhMathProbe", C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp]),
     ("bhFramebufferCreate", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("bhFramebufferDestroy", None, [_vp]),
     ("bhFramebufferColor", _vp, [_vp]),
@@ -549,6 +550,20 @@ def probe_grid(P: RenderParams, nx: int, ny: int) -> np.ndarray:
     check(lib.bhProbeGrid(buf.ptr, C.byref(P), nx, ny, None), "bhProbeGrid")
     synchronize()
     return buf.download(PROBE_DTYPE).reshape(ny, nx)
+
+
+def math_probe(fn: int, a, b=None) -> np.ndarray:
+    """bhMathProbe: results [n, 2] (float64) of library function `fn` on the
+    argument pairs (a, b)."""
+    a = np.asarray(a, np.float64).ravel()
+    b = np.zeros_like(a) if b is None else np.broadcast_to(np.asarray(b, np.float64), a.shape)
+    x = np.ascontiguousarray(np.stack([a, b], 1))
+    n = a.size
+    di, do = DeviceBuffer(max(x.nbytes, 16)), DeviceBuffer(max(x.nbytes, 16))
+    di.upload(x)
+    check(load().bhMathProbe(fn, di.ptr, n, do.ptr, None), "bhMathProbe")
+    synchronize()
+    return do.download(np.float64, 2 * n).reshape(n, 2)
 
 
 def probe_dirs(P: RenderParams, dirs: np.ndarray) -> np.ndarray:

@@ -210,6 +210,26 @@ bh_error_t bhProbeGrid(BhProbe* devOut, BH_PARAMS_IN p, int nx, int ny, bh_strea
 bh_error_t bhProbeDirs(BhProbe* devOut, const bh_float3* devDirs, int n,
                        BH_PARAMS_IN p, bh_stream_t stream);
 
+/* The library's own approximations of elementary functions, evaluated on
+ * the GPU as the fast trace kernels evaluate them (verification of their
+ * error bounds; no reference counterpart).  devIn: double[2n] argument
+ * pairs (a, b); devOut: double[2n] result pairs.  Functions: */
+enum
+{
+    BH_MATH_ATAN2_POLY = 0,       /* atan2(y = a, x = b) in float, polynomial (fast policy) */
+    BH_MATH_FAST_SINCOS = 1,      /* hardware sin(a), cos(a) in float */
+    BH_MATH_FAST_RCP = 2,         /* hardware 1/a in float */
+    BH_MATH_FAST_RSQ = 3,         /* hardware 1/sqrt(a) */
+    BH_MATH_FAST_SQRT = 4,        /* hardware sqrt(a) */
+    BH_MATH_FAST_EXP = 5,         /* exp(a) via hardware exp2 */
+    BH_MATH_FAST_LOG = 6,         /* log(a) via hardware log2 */
+    BH_MATH_FAST_POW = 7,         /* a^b via hardware exp2/log2 */
+    BH_MATH_D64_SINCOS = 8,       /* sin(a), cos(a) in double: Cody-Waite reduction and Taylor polynomials */
+    BH_MATH_D64_SINCOS_NEAR = 9,  /* sin, cos of a + b from those of a by angle addition (|b| < 1/16) */
+    BH_MATH_D64_RCP = 10          /* 1/a in double: hardware seed and two Newton steps */
+};
+bh_error_t bhMathProbe(int fn, const double* devIn, int n, double* devOut, bh_stream_t stream);
+
 /* Per-pixel trace record of one frame (no shading): for checking exit
  * directions against an FP64 trace.  devOut: BhRayRecord[W*H]. */
 typedef struct BhRayRecord
