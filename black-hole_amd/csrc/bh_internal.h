@@ -4,6 +4,7 @@
 // ---------------------------------------------------------------------------
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include "../../include/bh/bhtrace.h"
 #include "bh_rays.h"
 #include "bh_band.h"
@@ -102,6 +103,16 @@ struct ViewDesc
     int pad;
     float2 jitter;
 };
+
+// The host stages the table and the kernels read it: one layout in both
+// compilation passes (the offsets the batch kernels' scalar loads use).
+static_assert(sizeof(RenderParams) == 160, "RenderParams layout");
+static_assert(offsetof(ViewDesc, rec) == 160 + sizeof(FrameK) && offsetof(ViewDesc, tilesX) == offsetof(ViewDesc, rec) + 40,
+              "ViewDesc layout");
+static_assert(offsetof(ViewDesc, tileBase) == offsetof(ViewDesc, tilesX) + 4 &&
+                  offsetof(ViewDesc, tileCount) == offsetof(ViewDesc, tilesX) + 8 &&
+                  offsetof(ViewDesc, jitter) == offsetof(ViewDesc, tilesX) + 16 && sizeof(ViewDesc) % 8 == 0,
+              "ViewDesc layout");
 
 struct BatchArgs
 {

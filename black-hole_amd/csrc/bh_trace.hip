@@ -189,13 +189,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Pol::fast ?
 // Wavefronts take (view, tile) items from one queue over all views.  The
 // view index is wavefront-uniform, so its descriptor is read through scalar
 // loads; per-view completion times are stamped for latency reporting.
+//
+// The loop has one lane-0 region per iteration, at its top and before the
+// cross-lane read of the queue item: the completion count of the tile the
+// wavefront finished last, then the next queue item.  (A lane-0 region at
+// the end of the body -- the completion count after the trace -- let the
+// compiler merge it with the next iteration's queue atomic on the loop's
+// exit path: lanes 1..63 re-entered the loop while lane 0 was parked there,
+// read the next item from the inactive lane 0 as 0, and traced tile 0 of
+// view 0 forever.)  Every wavefront also leaves after totalTiles + 1 items
+// at most, an exit condition each lane reaches on its own count.
 template <class SRay, class KRay>
 __global__ __launch_bounds__(256) void traceBatchKernel(const ViewDesc* __restrict__ views, BatchArgs A)
 {
     const unsigned lane = __lane_id();
-    for (;;)
+    int prev = -1;      // view of the tile this wavefront finished last (wavefront-uniform)
+    for (int it = 0; it <= A.totalTiles; ++it)
     {
-        const unsigned t = nextTile(A.counter, lane);
+        unsigned t = 0;
+        if (lane == 0)
+        {
+            if (prev >= 0)
+            {
+                // all of this wavefront's record stores are issued before the count
+                __threadfence();
+                const unsigned d = atomicAdd(&A.viewDone[prev], 1u) + 1u;
+                if (d == (unsigned)views[prev].tileCount) A.stamps[prev] = (unsigned long long)wall_clock64();
+            }
+            t = atomicAdd(A.counter, 1u);
+        }
+        t = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl(t, 0));
         if (t >= (unsigned)A.totalTiles) break;
         int lo = 0, hi = A.nViews - 1;
         while (lo < hi)
@@ -210,13 +233,7 @@ __global__ __launch_bounds__(256) void traceBatchKernel(const ViewDesc* __restri
         if (V.P.model == BH_KERR || V.P.model == BH_KERR_NEWMAN)
             traceTile<KRay, false>(V.P, V.K, V.rec, V.tilesX, V.jitter, tile, lane);
         else traceTile<SRay, false>(V.P, V.K, V.rec, V.tilesX, V.jitter, tile, lane);
-        if (lane == 0)
-        {
-            // all of this wavefront's record stores are issued before the count
-            __threadfence();
-            const unsigned d = atomicAdd(&A.viewDone[v], 1u) + 1u;
-            if (d == (unsigned)V.tileCount) A.stamps[v] = (unsigned long long)wall_clock64();
-        }
+        prev = v;
     }
 }
 

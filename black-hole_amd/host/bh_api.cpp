@@ -161,6 +161,7 @@ struct Scratch
     unsigned long long* stamps = nullptr;
     int viewCap = 0;
     int lastViews = 0;
+    bool lastPersistent = false;          // the last batch was a persistent launch (latency from stamps)
     std::vector<hipEvent_t> viewEvents;   // [0] batch start, [v+1] view v done
     // batched views run on library worker streams forked from this stream
     hipStream_t batchStreams[kBatchStreams] = {};
@@ -787,12 +788,15 @@ hipError_t renderPipelineFixed(uchar4* out, float4* accum, float4* bloomA, float
 
 namespace {
 
-// The persistent multi-view trace kernel (traceBatchKernel: one queue of
-// (view, tile) items over all views) hung on its first GPU runs and stays
-// switched off until the cause is shown; see renderBatchPersistent.  Batches
-// therefore run as single-view pipelines (the kernels of
-// launchRenderPipeline) on worker streams forked from the caller's stream.
-constexpr bool kPersistentBatchEnabled = false;
+// The persistent multi-view launch: one trace kernel over a device queue of
+// (view, tile) items of every view (traceBatchKernel), then batched shade,
+// bloom and composite kernels over a device table of the views.  Every ray is
+// traced in FP32 (the reference's arithmetic; no FP64 band pass), so frames
+// equal launchRenderPipeline's with the band pass off.  (Its first GPU runs
+// hung: the queue loop ended with a lane-0 block that the compiler merged
+// with the next iteration's queue atomic, see traceBatchKernel; the loop was
+// restructured.)
+constexpr bool kPersistentBatchEnabled = true;
 
 hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const* outs, float4* const* accums,
                                  float4* const* bloomAs, float4* const* bloomBs, hipStream_t st)
@@ -854,6 +858,7 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
     A.viewDone = S->viewDone;
     A.stamps = S->stamps;
     S->lastViews = n;
+    S->lastPersistent = true;
     const BhSchedule sc = currentSchedule();
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
@@ -925,6 +930,7 @@ hipError_t renderBatch(int n, const RenderParams* views, uchar4* const* outs, fl
         }
         if ((int)S->slots.size() < n) S->slots.resize((size_t)n);
         S->lastViews = n;
+        S->lastPersistent = false;
     }
     // worker streams (default priority) forked from the caller's stream; they
     // and the auxiliary streams of their pipelines each get a hardware queue
@@ -1191,8 +1197,25 @@ bh_error_t bhBatchViewLatency(float* msOut, int nViews, bh_stream_t stream)
     Scratch* S = nullptr;
     hipError_t e = getScratch(st, 0, &S);
     if (e != hipSuccess) return e;
-    if (!msOut || nViews < 1 || nViews > S->lastViews || (int)S->viewEvents.size() < nViews + 1)
-        return hipErrorInvalidValue;
+    if (!msOut || nViews < 1 || nViews > S->lastViews) return hipErrorInvalidValue;
+    if (S->lastPersistent)
+    {
+        // device wall-clock stamps: each view's last tile traced, and the
+        // batch start (stamps[lastViews])
+        const int n = S->lastViews;
+        std::vector<unsigned long long> h((size_t)n + 1);
+        int dev = 0, khz = 0;
+        e = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = hipMemcpy(h.data(), S->stamps, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+        if (e != hipSuccess) return e;
+        if (khz <= 0) return hipErrorNotSupported;
+        for (int v = 0; v < nViews; ++v)
+            msOut[v] = h[(size_t)v] >= h[(size_t)n] ? (float)((double)(h[(size_t)v] - h[(size_t)n]) / khz) : -1.f;
+        return hipSuccess;
+    }
+    if ((int)S->viewEvents.size() < nViews + 1) return hipErrorInvalidValue;
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return e;
     for (int v = 0; v < nViews; ++v)

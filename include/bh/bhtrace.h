@@ -267,16 +267,22 @@ bh_error_t bhRenderBatch(int nViews, const RenderParams* views, bh_uchar4* const
                          bh_float4* const* accums, bh_float4* const* bloomAs,
                          bh_float4* const* bloomBs, bh_stream_t stream);
 /* The same batch as one persistent launch: one device queue of (view, tile)
- * items over all views, traced by one kernel, then batched shade and post
- * kernels.  Disabled in this build: it returns hipErrorNotSupported without
- * enqueueing anything (the kernel hung on its first GPU runs; the cause has
- * not been shown).  No reference counterpart (the reference renders one
- * view per launch, black_hole_kernel.cu:108-134). */
+ * items over all views, traced by one kernel (wavefronts take the next item
+ * as they finish; the view index is uniform per wavefront), then batched
+ * shade, bloom and composite kernels over a device table of the views, all
+ * on `stream`.  Every ray is traced in FP32 (the reference's arithmetic; the
+ * FP64 band pass does not apply), so each view's frame equals
+ * launchRenderPipeline's with the band pass off, bit for bit.  Per-view
+ * completion times are read with bhBatchViewLatency.  No reference
+ * counterpart (the reference renders one view per launch,
+ * black_hole_kernel.cu:108-134). */
 bh_error_t bhRenderBatchPersistent(int nViews, const RenderParams* views, bh_uchar4* const* outs,
                                    bh_float4* const* accums, bh_float4* const* bloomAs,
                                    bh_float4* const* bloomBs, bh_stream_t stream);
 /* Per-view completion time (ms after the batch began on the GPU) of the
- * last bhRenderBatch on `stream`; synchronizes the stream. */
+ * last bhRenderBatch (the view's whole pipeline) or bhRenderBatchPersistent
+ * (the view's last traced tile, from device wall-clock stamps) on `stream`;
+ * synchronizes the stream. */
 bh_error_t bhBatchViewLatency(float* msOut, int nViews, bh_stream_t stream);
 
 /* ------------------------------------------------------------------------ */

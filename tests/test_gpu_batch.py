@@ -3,11 +3,10 @@ models and disk settings must reproduce per-view launchRenderPipeline output
 bit for bit, and match the oracle.
 
 bhRenderBatch runs the views as single-view pipelines on worker streams
-forked from the caller's stream (views finish in any order); these tests
-exercise that path.  The persistent single-queue multi-view kernel hung on
-its first two GPU runs (cause not yet shown): its entry point,
-bhRenderBatchPersistent, returns hipErrorNotSupported, and the test of its
-output stays skipped.
+forked from the caller's stream (views finish in any order).
+bhRenderBatchPersistent traces all views in one persistent launch over a
+device queue of (view, tile) items, in FP32; it is compared with
+launchRenderPipeline under the band-off precision.
 """
 import ctypes as C
 
@@ -43,31 +42,69 @@ def small_views(bt, n=12):
     return views
 
 
-SKIP_PERSISTENT = pytest.mark.skip(
-    reason="bhRenderBatchPersistent: the persistent multi-view kernel hung on the GPU twice (cause not shown); "
-           "the entry point is disabled in this build")
-HIP_ERROR_NOT_SUPPORTED = 801
 HIP_ERROR_INVALID_VALUE = 1
 
 
-def test_persistent_batch_entry_is_disabled(gpu):
-    """The disabled persistent launch enqueues nothing and says so."""
-    B = gpu.BatchBuffers(small_views(gpu, 4))
-    assert B.render_persistent() == HIP_ERROR_NOT_SUPPORTED
+class _BandOff:
+    """The persistent launch traces every ray in FP32: compare it with
+    launchRenderPipeline under the same (band-off) precision."""
+
+    def __init__(self, gpu):
+        self.gpu = gpu
+
+    def __enter__(self):
+        self.p0 = self.gpu.get_precision()
+        self.gpu.set_precision(band=0)
+
+    def __exit__(self, *exc):
+        self.gpu.restore_precision(self.p0)
 
 
-@SKIP_PERSISTENT
 def test_persistent_batch_equals_single_view_launches(gpu, oracle):
     views = small_views(gpu)
     say("persistent batch of", len(views), "small views")
     B = gpu.BatchBuffers(views)
-    assert B.render_persistent() == 0
-    gpu.synchronize()
-    for v, P in enumerate(views):
-        rgba, acc = B.readback(v)
-        ref_rgba, ref_acc = gpu.render_pipeline(P)
-        assert np.array_equal(rgba, ref_rgba), v
-        assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32)), v
+    with _BandOff(gpu):
+        assert B.render_persistent() == 0
+        gpu.synchronize()
+        lat = B.latency_ms()
+        say("persistent batch done; per-view latency ms", ["%.3f" % x for x in lat])
+        assert all(x > 0 for x in lat), lat
+        for v, P in enumerate(views):
+            rgba, acc = B.readback(v)
+            ref_rgba, ref_acc = gpu.render_pipeline(P)
+            assert np.array_equal(rgba, ref_rgba), v
+            assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32)), v
+    oracle.set_band(0)
+    try:
+        for v, P in enumerate(views):
+            o_rgba, _ = oracle.render_pipeline(P)
+            st = lsb_stats(B.readback(v)[0], o_rgba)
+            assert st["le1"] >= 0.999, (v, st)
+    finally:
+        p = gpu.get_precision()
+        oracle.set_band(p.fp64Band, p.critBand, p.polarBand)
+
+
+def test_persistent_uneven_64_view_batch(gpu):
+    """The seed-1 64-view batch in one persistent launch: every view equals
+    its own (band-off) launchRenderPipeline bit for bit, again on a second
+    launch into the same buffers."""
+    views = gpu.workload_views("uneven_views", 1, 64)
+    B = gpu.BatchBuffers(views)
+    with _BandOff(gpu):
+        for rep in range(2):
+            assert B.render_persistent() == 0
+            gpu.synchronize()
+            lat = B.latency_ms()
+            say("persistent 64-view batch %d: view latency ms min %.2f median %.2f max %.2f" %
+                (rep, min(lat), float(np.median(lat)), max(lat)))
+            assert all(x > 0 for x in lat)
+        for v, P in enumerate(views):
+            rgba, acc = B.readback(v)
+            ref, ref_acc = gpu.render_pipeline(P)
+            assert np.array_equal(rgba, ref), v
+            assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32)), v
 
 
 def test_uneven_64_view_batch(gpu, oracle):
