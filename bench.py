@@ -355,7 +355,42 @@ def other_configs(bt, lib, fp64):
     out["uneven_views_64"] = {"rays": rays, "batch_ms": round(dt * 1e3, 2), "mrays_s": round(rays / dt / 1e6, 2),
                               "view_latency_ms_p50": round(float(np.median(lat)), 2),
                               "view_latency_ms_max": round(float(max(lat)), 2),
-                              "launch": "single-view pipelines on 6 worker streams (persistent multi-view kernel disabled)"}
+                              "launch": "bhRenderBatch: single-view pipelines (FP64 band on) on 6 worker streams"}
+    # the same views in one persistent launch (every ray FP32: the
+    # reference's arithmetic), eager and as a replayed graph
+    p0 = bt.get_precision()
+    try:
+        bt.set_precision(band=0)
+        bt.check(B.render_persistent(), "bhRenderBatchPersistent")
+        bt.synchronize()
+        pe = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            bt.check(B.render_persistent(), "bhRenderBatchPersistent")
+            bt.synchronize()
+            t = time.perf_counter() - t0
+            pe = t if pe is None or t < pe else pe
+        G = B.graph(autotune=1)
+        wg, tuned = G.info()
+        G.launch()
+        bt.synchronize()
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            G.launch()
+        bt.synchronize()
+        tg = (time.perf_counter() - t0) / reps
+        glat = G.latency_ms()
+        G.close()
+    finally:
+        bt.restore_precision(p0)
+    out["uneven_views_64_persistent"] = {
+        "rays": rays, "eager_batch_ms": round(pe * 1e3, 2), "eager_mrays_s": round(rays / pe / 1e6, 2),
+        "graph_batch_ms": round(tg * 1e3, 2), "graph_mrays_s": round(rays / tg / 1e6, 2),
+        "trace_latency_ms_p50": round(float(np.median(glat)), 2), "trace_latency_ms_max": round(float(max(glat)), 2),
+        "trace_wg_size": wg, "autotune_ms_wg64_128_256": [round(x, 2) for x in tuned],
+        "launch": "bhRenderBatchPersistent / bhBatchGraph: one persistent trace over a device queue of (view, tile) "
+                  "items + batched shade/bloom/composite; every ray FP32 (band off)"}
     return out
 
 

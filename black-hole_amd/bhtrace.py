@@ -147,6 +147,11 @@ SIGNATURES = [
     ("bhRenderBatch", C.c_int, [C.c_int, _P, _vp, _vp, _vp, _vp, _vp]),
     ("bhRenderBatchPersistent", C.c_int, [C.c_int, _P, _vp, _vp, _vp, _vp, _vp]),
     ("bhBatchViewLatency", C.c_int, [C.POINTER(C.c_float), C.c_int, _vp]),
+    ("bhBatchGraphCreate", C.c_int, [C.c_int, _P, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_void_p)]),
+    ("bhBatchGraphLaunch", C.c_int, [_vp, _vp]),
+    ("bhBatchGraphInfo", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    ("bhBatchGraphLatency", C.c_int, [_vp, C.POINTER(C.c_float), C.c_int, _vp]),
+    ("bhBatchGraphDestroy", None, [_vp]),
     ("bhProbeGrid", C.c_int, [_vp, _P, C.c_int, C.c_int, _vp]),
     ("bhProbeDirs", C.c_int, [_vp, _vp, C.c_int, _P, _vp]),
     ("bhTraceRecords", C.c_int, [_vp, _P, _vp]),
@@ -846,8 +851,47 @@ class BatchBuffers:
         check(load().bhBatchViewLatency(out, self.n, stream), "bhBatchViewLatency")
         return list(out)
 
+    def graph(self, autotune: int = 1) -> "BatchGraph":
+        """The persistent launch of this batch captured as a graph."""
+        return BatchGraph(self, autotune)
+
     def readback(self, v):
         P = self.views[v]
         rgba = self.bufs[v][0].download(np.uint8).reshape(P.height, P.width, 4)
         acc = self.bufs[v][1].download(np.float32).reshape(P.height, P.width, 4)
         return rgba, acc
+
+
+class BatchGraph:
+    """bhBatchGraph* over a BatchBuffers' views and buffers."""
+
+    def __init__(self, buffers: "BatchBuffers", autotune: int = 1):
+        self.b = buffers
+        p = C.c_void_p()
+        check(load().bhBatchGraphCreate(buffers.n, buffers.views, buffers.outs, buffers.accs, buffers.bAs, buffers.bBs,
+                                        autotune, C.byref(p)), "bhBatchGraphCreate")
+        self.ptr = p
+
+    def launch(self, stream=None):
+        check(load().bhBatchGraphLaunch(self.ptr, stream), "bhBatchGraphLaunch")
+
+    def info(self):
+        wg, ms = C.c_int(), (C.c_float * 3)()
+        check(load().bhBatchGraphInfo(self.ptr, C.byref(wg), ms), "bhBatchGraphInfo")
+        return wg.value, list(ms)
+
+    def latency_ms(self, stream=None):
+        out = (C.c_float * self.b.n)()
+        check(load().bhBatchGraphLatency(self.ptr, out, self.b.n, stream), "bhBatchGraphLatency")
+        return list(out)
+
+    def close(self):
+        if self.ptr:
+            load().bhBatchGraphDestroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

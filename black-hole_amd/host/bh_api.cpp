@@ -798,22 +798,28 @@ namespace {
 // restructured.)
 constexpr bool kPersistentBatchEnabled = true;
 
-hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const* outs, float4* const* accums,
-                                 float4* const* bloomAs, float4* const* bloomBs, hipStream_t st)
+// The view table of a persistent batch (records not yet assigned): per view
+// the sanitized params, frame constants, buffers and its range of queue
+// items; totals for the launch shapes.
+struct BatchTable
 {
-    if (!kPersistentBatchEnabled) return hipErrorNotSupported;
-
-    if (n < 1 || !views || !outs || !accums) return hipErrorInvalidValue;
-    std::vector<bhk::ViewDesc> D((size_t)n);
+    std::vector<bhk::ViewDesc> D;
     size_t totalPix = 0;
     int tiles = 0, maxW = 1, maxH = 1;
+};
+
+hipError_t buildBatchTable(int n, const RenderParams* views, uchar4* const* outs, float4* const* accums,
+                           float4* const* bloomAs, float4* const* bloomBs, BatchTable& T)
+{
+    if (n < 1 || !views || !outs || !accums) return hipErrorInvalidValue;
+    T.D.assign((size_t)n, bhk::ViewDesc{});
     for (int v = 0; v < n; ++v)
     {
         RenderParams P = views[v];
         bhSanitizeRenderParams(&P);
         if (!outs[v] || !accums[v]) return hipErrorInvalidValue;
         if (P.bloomEnabled && (!bloomAs || !bloomBs || !bloomAs[v] || !bloomBs[v])) return hipErrorInvalidValue;
-        bhk::ViewDesc& d = D[(size_t)v];
+        bhk::ViewDesc& d = T.D[(size_t)v];
         std::memset(&d, 0, sizeof(d));
         d.P = P;
         d.K = bhk::makeFrame(P);
@@ -822,30 +828,73 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
         d.bloomA = bloomAs ? bloomAs[v] : nullptr;
         d.bloomB = bloomBs ? bloomBs[v] : nullptr;
         d.tilesX = (P.width + 7) / 8;
-        d.tileBase = tiles;
+        d.tileBase = T.tiles;
         d.tileCount = d.tilesX * ((P.height + 7) / 8);
         d.jitter = jitterOf(P.accumMode, P.sampleIndex);
-        tiles += d.tileCount;
-        totalPix += (size_t)P.width * P.height;
-        maxW = P.width > maxW ? P.width : maxW;
-        maxH = P.height > maxH ? P.height : maxH;
+        T.tiles += d.tileCount;
+        T.totalPix += (size_t)P.width * P.height;
+        T.maxW = P.width > T.maxW ? P.width : T.maxW;
+        T.maxH = P.height > T.maxH ? P.height : T.maxH;
     }
+    return hipSuccess;
+}
+
+void assignBatchRecords(BatchTable& T, BhRayRecord* rec)
+{
+    size_t off = 0;
+    for (auto& d : T.D)
+    {
+        d.rec = rec + off;
+        off += (size_t)d.P.width * d.P.height;
+    }
+}
+
+// The kernels of a persistent batch on `st`: counters reset, the trace over
+// one queue of (view, tile) items, shade + accumulation, bloom + composite.
+// No memset and no copy: capturable as a graph.
+hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int wgSize, int math, hipStream_t st,
+                               Scratch* timers)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    const int occ = math ? bhk::traceBatchOccupancyFast(wgSize) : bhk::traceBatchOccupancyExact(wgSize);
+    bhk::LaunchShape L;
+    L.wgSize = wgSize;
+    const long long want = (long long)occ * numCUs(dev), useful = ((long long)A.totalTiles * 64 + wgSize - 1) / wgSize;
+    L.blocks = (int)(want < useful ? want : useful);
+    if (L.blocks < 1) L.blocks = 1;
+    if (timers) recordEvent(timers, 0, st);
+    hipError_t e = bhk::launchResetBatch(A, st);
+    if (e == hipSuccess) e = math ? bhk::launchTraceBatchFast(A, L, st) : bhk::launchTraceBatchExact(A, L, st);
+    if (e != hipSuccess) return e;
+    if (timers) recordEvent(timers, 1, st);
+    e = bhk::launchShadeBlendBatch(A.views, A.nViews, maxW, maxH, st);
+    if (e != hipSuccess) return e;
+    if (timers) recordEvent(timers, 2, st);
+    e = bhk::launchBloomBatch(A.views, A.nViews, maxW, maxH, st);
+    if (e != hipSuccess) return e;
+    if (timers) recordEvent(timers, 3, st);
+    return hipSuccess;
+}
+
+hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const* outs, float4* const* accums,
+                                 float4* const* bloomAs, float4* const* bloomBs, hipStream_t st)
+{
+    if (!kPersistentBatchEnabled) return hipErrorNotSupported;
+    BatchTable T;
+    hipError_t e = buildBatchTable(n, views, outs, accums, bloomAs, bloomBs, T);
+    if (e != hipSuccess) return e;
     Scratch* S = nullptr;
-    hipError_t e = getScratch(st, totalPix, &S);
+    e = getScratch(st, T.totalPix, &S);
     if (e != hipSuccess) return e;
     e = getBatchScratch(S, n, st);
     if (e != hipSuccess) return e;
-    size_t off = 0;
-    for (auto& d : D)
-    {
-        d.rec = S->rec + off;
-        off += (size_t)d.P.width * d.P.height;
-    }
+    assignBatchRecords(T, S->rec);
     // stage the table in pinned memory (after the previous upload from it
     // has completed) and copy it asynchronously on the stream
     e = hipEventSynchronize(S->staged);
     if (e != hipSuccess) return e;
-    std::memcpy(S->hostViews, D.data(), sizeof(bhk::ViewDesc) * n);
+    std::memcpy(S->hostViews, T.D.data(), sizeof(bhk::ViewDesc) * n);
     e = hipMemcpyAsync(S->views, S->hostViews, sizeof(bhk::ViewDesc) * n, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return e;
     e = hipEventRecord(S->staged, st);
@@ -853,42 +902,40 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
     bhk::BatchArgs A;
     A.views = S->views;
     A.nViews = n;
-    A.totalTiles = tiles;
+    A.totalTiles = T.tiles;
     A.counter = S->counter;
     A.viewDone = S->viewDone;
     A.stamps = S->stamps;
     S->lastViews = n;
     S->lastPersistent = true;
     const BhSchedule sc = currentSchedule();
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-    const int occ = sc.mathMode ? bhk::traceBatchOccupancyFast(sc.wgSize) : bhk::traceBatchOccupancyExact(sc.wgSize);
-    bhk::LaunchShape L;
-    L.wgSize = sc.wgSize;
-    const long long want = (long long)occ * numCUs(dev), useful = ((long long)tiles * 64 + sc.wgSize - 1) / sc.wgSize;
-    L.blocks = (int)(want < useful ? want : useful);
-    if (L.blocks < 1) L.blocks = 1;
     const bool timed = g_timers.load() != 0;
-    if (timed) recordEvent(S, 0, st);
-    e = bhk::launchResetBatch(A, st);
-    if (e != hipSuccess) return e;
-    e = sc.mathMode ? bhk::launchTraceBatchFast(A, L, st) : bhk::launchTraceBatchExact(A, L, st);
-    if (e != hipSuccess) return e;
-    if (timed) recordEvent(S, 1, st);
-    e = bhk::launchShadeBlendBatch(S->views, n, maxW, maxH, st);
-    if (e != hipSuccess) return e;
-    if (timed) recordEvent(S, 2, st);
-    e = bhk::launchBloomBatch(S->views, n, maxW, maxH, st);
+    e = enqueueBatchKernels(A, T.maxW, T.maxH, sc.wgSize, sc.mathMode, st, timed ? S : nullptr);
     if (e != hipSuccess) return e;
     if (timed)
     {
-        recordEvent(S, 3, st);
         int d2 = 0;
         if (hipGetDevice(&d2) != hipSuccess) d2 = 0;
         std::lock_guard<std::mutex> lk(g_mu);
         g_lastTimed = {d2, st};
     }
     return hipGetLastError();
+}
+
+// Per-view times (ms after the batch start) from device wall-clock stamps:
+// stamps[v] the view's last traced tile, stamps[n] the start.
+hipError_t stampsToMs(const unsigned long long* dStamps, int n, float* msOut, int nOut)
+{
+    std::vector<unsigned long long> h((size_t)n + 1);
+    int dev = 0, khz = 0;
+    hipError_t e = hipMemcpy(h.data(), dStamps, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+    if (e != hipSuccess) return e;
+    if (khz <= 0) return hipErrorNotSupported;
+    for (int v = 0; v < nOut; ++v)
+        msOut[v] = h[(size_t)v] >= h[(size_t)n] ? (float)((double)(h[(size_t)v] - h[(size_t)n]) / khz) : -1.f;
+    return hipSuccess;
 }
 
 
@@ -1104,6 +1151,22 @@ struct BhFramebuffer
     float4* bloomB;
 };
 
+// A steady batch of views as one captured graph of the persistent launch.
+struct BhBatchGraph
+{
+    int n = 0, maxW = 1, maxH = 1, tiles = 0;
+    bhk::ViewDesc* table = nullptr;     // device view table (uploaded once)
+    BhRayRecord* rec = nullptr;         // records of every view
+    unsigned* counter = nullptr;        // queue head
+    unsigned* viewDone = nullptr;
+    unsigned long long* stamps = nullptr;
+    hipStream_t cs = nullptr;           // private capture stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int wgSize = 64;
+    float tunedMs[3] = {0.f, 0.f, 0.f}; // trace+shade+post of one launch at workgroup 64, 128, 256 (0: not timed)
+};
+
 struct BhAccumulator
 {
     BhFramebuffer* fb = nullptr;
@@ -1200,20 +1263,9 @@ bh_error_t bhBatchViewLatency(float* msOut, int nViews, bh_stream_t stream)
     if (!msOut || nViews < 1 || nViews > S->lastViews) return hipErrorInvalidValue;
     if (S->lastPersistent)
     {
-        // device wall-clock stamps: each view's last tile traced, and the
-        // batch start (stamps[lastViews])
-        const int n = S->lastViews;
-        std::vector<unsigned long long> h((size_t)n + 1);
-        int dev = 0, khz = 0;
         e = hipStreamSynchronize(st);
-        if (e == hipSuccess) e = hipMemcpy(h.data(), S->stamps, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
         if (e != hipSuccess) return e;
-        if (khz <= 0) return hipErrorNotSupported;
-        for (int v = 0; v < nViews; ++v)
-            msOut[v] = h[(size_t)v] >= h[(size_t)n] ? (float)((double)(h[(size_t)v] - h[(size_t)n]) / khz) : -1.f;
-        return hipSuccess;
+        return stampsToMs(S->stamps, S->lastViews, msOut, nViews);
     }
     if ((int)S->viewEvents.size() < nViews + 1) return hipErrorInvalidValue;
     e = hipStreamSynchronize(st);
@@ -1224,6 +1276,130 @@ bh_error_t bhBatchViewLatency(float* msOut, int nViews, bh_stream_t stream)
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// ---------------------------------------------------------------- graph-captured persistent batch
+void bhBatchGraphDestroy(BhBatchGraph* g)
+{
+    if (!g) return;
+    if (g->cs) (void)hipStreamSynchronize(g->cs);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->cs) (void)hipStreamDestroy(g->cs);
+    (void)hipFree(g->table);
+    (void)hipFree(g->rec);
+    (void)hipFree(g->counter);
+    (void)hipFree(g->viewDone);
+    (void)hipFree(g->stamps);
+    delete g;
+}
+
+bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* const* outs, bh_float4* const* accums,
+                              bh_float4* const* bloomAs, bh_float4* const* bloomBs, int autotune, BhBatchGraph** out)
+{
+    if (!out) return hipErrorInvalidValue;
+    *out = nullptr;
+    BatchTable T;
+    hipError_t e = buildBatchTable(nViews, views, reinterpret_cast<uchar4* const*>(outs),
+                                   reinterpret_cast<float4* const*>(accums), reinterpret_cast<float4* const*>(bloomAs),
+                                   reinterpret_cast<float4* const*>(bloomBs), T);
+    if (e != hipSuccess) return e;
+    BhBatchGraph* g = new (std::nothrow) BhBatchGraph;
+    if (!g) return hipErrorOutOfMemory;
+    g->n = nViews;
+    g->maxW = T.maxW;
+    g->maxH = T.maxH;
+    g->tiles = T.tiles;
+    e = hipMalloc(&g->rec, std::max<size_t>(T.totalPix, 1) * sizeof(BhRayRecord));
+    if (e == hipSuccess) e = hipMalloc(&g->table, sizeof(bhk::ViewDesc) * nViews);
+    if (e == hipSuccess) e = hipMalloc(&g->counter, kCounterBytes);
+    if (e == hipSuccess) e = hipMalloc(&g->viewDone, sizeof(unsigned) * nViews);
+    if (e == hipSuccess) e = hipMalloc(&g->stamps, sizeof(unsigned long long) * (nViews + 1));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->cs, hipStreamNonBlocking);
+    if (e == hipSuccess)
+    {
+        assignBatchRecords(T, g->rec);
+        e = hipMemcpy(g->table, T.D.data(), sizeof(bhk::ViewDesc) * nViews, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) { bhBatchGraphDestroy(g); return e; }
+    bhk::BatchArgs A;
+    A.views = g->table;
+    A.nViews = nViews;
+    A.totalTiles = T.tiles;
+    A.counter = g->counter;
+    A.viewDone = g->viewDone;
+    A.stamps = g->stamps;
+    const BhSchedule sc = currentSchedule();
+    g->wgSize = sc.wgSize;
+    if (autotune)
+    {
+        // one warm-up and one timed eager launch per workgroup size; the
+        // frames do not depend on it
+        const int wgs[3] = {64, 128, 256};
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        e = hipEventCreate(&t0);
+        if (e == hipSuccess) e = hipEventCreate(&t1);
+        float best = 1e30f;
+        for (int w = 0; w < 3 && e == hipSuccess; ++w)
+        {
+            for (int r = 0; r < 2 && e == hipSuccess; ++r)
+            {
+                e = hipEventRecord(t0, g->cs);
+                if (e == hipSuccess) e = enqueueBatchKernels(A, g->maxW, g->maxH, wgs[w], sc.mathMode, g->cs, nullptr);
+                if (e == hipSuccess) e = hipEventRecord(t1, g->cs);
+                if (e == hipSuccess) e = hipEventSynchronize(t1);
+            }
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+            if (e == hipSuccess)
+            {
+                g->tunedMs[w] = ms;
+                if (ms < best) { best = ms; g->wgSize = wgs[w]; }
+            }
+        }
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        if (e != hipSuccess) { bhBatchGraphDestroy(g); return e; }
+    }
+    e = hipStreamBeginCapture(g->cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) { bhBatchGraphDestroy(g); return e; }
+    hipError_t le = enqueueBatchKernels(A, g->maxW, g->maxH, g->wgSize, sc.mathMode, g->cs, nullptr);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(g->cs, &graph);
+    if (le != hipSuccess || e != hipSuccess)
+    {
+        if (graph) (void)hipGraphDestroy(graph);
+        bhBatchGraphDestroy(g);
+        return le != hipSuccess ? le : e;
+    }
+    g->graph = graph;
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) { g->exec = nullptr; bhBatchGraphDestroy(g); return e; }
+    *out = g;
+    return hipSuccess;
+}
+
+bh_error_t bhBatchGraphLaunch(BhBatchGraph* g, bh_stream_t stream)
+{
+    if (!g || !g->exec) return hipErrorInvalidValue;
+    return hipGraphLaunch(g->exec, (hipStream_t)stream);
+}
+
+bh_error_t bhBatchGraphInfo(BhBatchGraph* g, int* wgSize, float* tunedMs3)
+{
+    if (!g) return hipErrorInvalidValue;
+    if (wgSize) *wgSize = g->wgSize;
+    if (tunedMs3)
+        for (int k = 0; k < 3; ++k) tunedMs3[k] = g->tunedMs[k];
+    return hipSuccess;
+}
+
+bh_error_t bhBatchGraphLatency(BhBatchGraph* g, float* msOut, int nViews, bh_stream_t stream)
+{
+    if (!g || !msOut || nViews < 1 || nViews > g->n) return hipErrorInvalidValue;
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return e;
+    return stampsToMs(g->stamps, g->n, msOut, nViews);
 }
 
 // ---------------------------------------------------------------- framebuffer

@@ -279,6 +279,25 @@ bh_error_t bhRenderBatch(int nViews, const RenderParams* views, bh_uchar4* const
 bh_error_t bhRenderBatchPersistent(int nViews, const RenderParams* views, bh_uchar4* const* outs,
                                    bh_float4* const* accums, bh_float4* const* bloomAs,
                                    bh_float4* const* bloomBs, bh_stream_t stream);
+/* A steady batch captured once and replayed: the persistent launch of
+ * bhRenderBatchPersistent for fixed views and buffers, as a hipGraph of its
+ * kernels (counters reset by a kernel; no memset or copy nodes).  Create
+ * uploads the view table once, gives the batch its own records and
+ * counters, and with `autotune` first times one eager launch per trace
+ * workgroup size (64, 128, 256; the frames do not depend on it) and
+ * captures the fastest.  Launch replays the whole batch on `stream`;
+ * Latency reads the per-view trace completion times of the last replay
+ * (synchronizes `stream`).  Frames equal bhRenderBatchPersistent's. */
+typedef struct BhBatchGraph BhBatchGraph;
+bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* const* outs,
+                              bh_float4* const* accums, bh_float4* const* bloomAs,
+                              bh_float4* const* bloomBs, int autotune, BhBatchGraph** g);
+bh_error_t bhBatchGraphLaunch(BhBatchGraph* g, bh_stream_t stream);
+/* Workgroup size the graph traces with, and the tuning times (ms, 0 when not tuned). */
+bh_error_t bhBatchGraphInfo(BhBatchGraph* g, int* wgSize, float* tunedMs3);
+bh_error_t bhBatchGraphLatency(BhBatchGraph* g, float* msOut, int nViews, bh_stream_t stream);
+void       bhBatchGraphDestroy(BhBatchGraph* g);
+
 /* Per-view completion time (ms after the batch began on the GPU) of the
  * last bhRenderBatch (the view's whole pipeline) or bhRenderBatchPersistent
  * (the view's last traced tile, from device wall-clock stamps) on `stream`;

@@ -107,6 +107,38 @@ def test_persistent_uneven_64_view_batch(gpu):
             assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32)), v
 
 
+def test_persistent_batch_graph_replays(gpu):
+    """The 64-view persistent launch captured as a graph (autotuned trace
+    workgroup): three replays, with host synchronizations and unrelated eager
+    frames between them, each give the eager launch's frames bit for bit."""
+    views = gpu.workload_views("uneven_views", 1, 64)
+    B = gpu.BatchBuffers(views)
+    with _BandOff(gpu):
+        assert B.render_persistent() == 0
+        gpu.synchronize()
+        ref = [B.readback(v) for v in range(len(views))]
+        G = B.graph(autotune=1)
+        wg, tuned = G.info()
+        say("batch graph: workgroup %d, tuning ms %s" % (wg, ["%.2f" % t for t in tuned]))
+        assert wg in (64, 128, 256) and all(t > 0 for t in tuned)
+        try:
+            for rep in range(3):
+                for b in B.bufs:
+                    b[0].fill(0)
+                G.launch()
+                gpu.synchronize()
+                lat = G.latency_ms()
+                assert all(x > 0 for x in lat), lat
+                for v in range(len(views)):
+                    rgba, acc = B.readback(v)
+                    assert np.array_equal(rgba, ref[v][0]), (rep, v)
+                    assert np.array_equal(acc.view(np.uint32), ref[v][1].view(np.uint32)), (rep, v)
+                say("replay %d identical; trace latency max %.2f ms" % (rep, max(lat)))
+                gpu.render_pipeline(views[rep])      # unrelated eager work between replays
+        finally:
+            G.close()
+
+
 def test_uneven_64_view_batch(gpu, oracle):
     """The seed-1 batch of 64 uneven views (320x180 .. 1920x1080, Kerr-Newman,
     disk on/off) through bhRenderBatch: every view equals its own
