@@ -197,6 +197,7 @@ SIGNATURES = [
                                          C.POINTER(C.c_int)]),
     ("bhSurrogateTrain", C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _vp]),
     ("bhSurrogateForward", C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
+    ("bhSurrogateForwardFused", C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
     ("bhSurrogateLossGrad", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp]),
     ("bhSetSurrogate", C.c_int, [_vp, C.c_float, C.c_float]),
     ("bhLastSurrogateStats", C.c_int, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), _vp]),
@@ -722,12 +723,15 @@ class Surrogate:
         check(load().bhSurrogateTrain(self.ptr, steps, batch, lr, loss.ctypes.data_as(C.c_void_p), None), "train")
         return loss
 
-    def forward(self, x: np.ndarray) -> np.ndarray:
+    def forward(self, x: np.ndarray, fused: bool = False) -> np.ndarray:
+        """Head outputs [n x 5] from the training path's GEMMs, or (fused)
+        from the render path's fused inference kernel."""
         a = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
         n = a.shape[0]
         xi, yo = DeviceBuffer(a.nbytes), DeviceBuffer(n * 5 * 4)
         xi.upload(a)
-        check(load().bhSurrogateForward(self.ptr, xi.ptr, n, yo.ptr, None), "forward")
+        fn = load().bhSurrogateForwardFused if fused else load().bhSurrogateForward
+        check(fn(self.ptr, xi.ptr, n, yo.ptr, None), "forward")
         synchronize()
         return yo.download(np.float32).reshape(n, 5)
 

@@ -67,7 +67,9 @@ struct RouteArgs
     const int* dynIndex;
     const float* inputs;    // [cap x 6]
     const unsigned* list;   // [cap]
-    const unsigned* count;
+    const unsigned* count;  // device row count (or null: `rows`)
+    int rows;
+    float* outRows;         // forward only: [rows x 5] head outputs, no routing (or null)
     const __bf16* W;        // network parameters (bf16 weights)
     const float* Pf;        // network parameters (f32, biases)
     int wOff[kMlpLayers], bOff[kMlpLayers];
@@ -75,7 +77,8 @@ struct RouteArgs
     float diskProbMax;      // ... and the predicted disk-hit probability below this
     BhRayRecord* rec;
     unsigned char* mask;
-    unsigned* accepted;
+    unsigned* accepted;     // accepted-ray count (zero before launch)
+    float4* accList;        // [cap] accepted rays: (pixel bits, deflection outputs x2, 0)
 };
 
 struct ProbeArgs

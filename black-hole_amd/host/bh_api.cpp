@@ -186,6 +186,7 @@ struct Scratch
     // counter[48] (accepted)
     float* candInputs = nullptr;
     unsigned* candList = nullptr;
+    float4* candAcc = nullptr;                   // rays the surrogate answered (pixel, deflection outputs)
     size_t candCap = 0;
     hipEvent_t bandEv[2] = {nullptr, nullptr};   // timing: band pass start / end
     hipEvent_t tileEv[2] = {nullptr, nullptr};   // timing: FP32 tile kernel start / end (its own stream)
@@ -334,9 +335,11 @@ hipError_t getCandScratch(Scratch* S, size_t n, hipStream_t st)
     if (e != hipSuccess) return e;
     (void)hipFree(S->candInputs);
     (void)hipFree(S->candList);
-    S->candInputs = nullptr; S->candList = nullptr; S->candCap = 0;
+    (void)hipFree(S->candAcc);
+    S->candInputs = nullptr; S->candList = nullptr; S->candAcc = nullptr; S->candCap = 0;
     e = hipMalloc(&S->candInputs, n * 6 * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&S->candList, n * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&S->candAcc, n * sizeof(float4));
     if (e != hipSuccess) return e;
     S->candCap = n;
     return hipSuccess;
@@ -558,6 +561,8 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             R.inputs = Sc->candInputs;
             R.list = Sc->candList;
             R.count = counter + kCandCountSlot;
+            R.rows = 0;
+            R.outRows = nullptr;
             R.W = net.W;
             R.Pf = net.P;
             for (int l = 0; l < bhk::kMlpLayers; ++l) { R.wOff[l] = net.wOff[l]; R.bOff[l] = net.bOff[l]; }
@@ -568,6 +573,7 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             R.rec = rec;
             R.mask = bandMask;
             R.accepted = counter + kAcceptSlot;
+            R.accList = Sc->candAcc;
             const bool rt = allowTimers && g_timers.load() != 0;
             if (rt && !Sc->routeEv[0])
             {
@@ -1084,6 +1090,7 @@ void freeScratch(Scratch& S)
     (void)hipFree(S.tileHist);
     (void)hipFree(S.candInputs);
     (void)hipFree(S.candList);
+    (void)hipFree(S.candAcc);
     for (auto& ev : S.bandEv)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : S.tileEv)

@@ -446,6 +446,28 @@ bh_error_t bhSurrogateForward(BhSurrogate* s, const float* devInputs, int n, flo
     return e;
 }
 
+bh_error_t bhSurrogateForwardFused(BhSurrogate* s, const float* devInputs, int n, float* devOut, bh_stream_t stream)
+{
+    if (!s || n < 0 || (n > 0 && (!devInputs || !devOut))) return hipErrorInvalidValue;
+    if (n == 0) return hipSuccess;
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(s->mu);
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const bhk::SurrogateNet net = bhk::surrogateNet(s);
+    bhk::RouteArgs R{};
+    R.inputs = devInputs;
+    R.count = nullptr;
+    R.rows = n;
+    R.outRows = devOut;
+    R.W = net.W;
+    R.Pf = net.P;
+    for (int l = 0; l < kMlpLayers; ++l) { R.wOff[l] = net.wOff[l]; R.bOff[l] = net.bOff[l]; }
+    return bhk::launchSurrogateRoute(R, n, cus, st);
+}
+
 bh_error_t bhSurrogateLossGrad(BhSurrogate* s, const float* devInputs, const void* devTargets, int n,
                                float* lossHost, float* gradHost, bh_stream_t stream)
 {

@@ -483,6 +483,10 @@ bh_error_t bhSurrogateTrain(BhSurrogate* s, int steps, int batch, float lr, floa
  * columns, milliradians), disk-hit logit, hit radius / 10, and s, the log of
  * the expected squared exit error in mrad^2. */
 bh_error_t bhSurrogateForward(BhSurrogate* s, const float* devInputs, int n, float* devOut, bh_stream_t stream);
+/* The same outputs from the render path's fused inference kernel (the
+ * network held in LDS, activations in registers, as the router runs it). */
+bh_error_t bhSurrogateForwardFused(BhSurrogate* s, const float* devInputs, int n, float* devOut,
+                                   bh_stream_t stream);
 /* Loss terms (sums over rows) and the gradient of the mean loss with respect
  * to every parameter for n rows (host outputs; synchronizes). */
 bh_error_t bhSurrogateLossGrad(BhSurrogate* s, const float* devInputs, const void* devTargets, int n,

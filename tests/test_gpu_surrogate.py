@@ -97,6 +97,25 @@ def test_forward_matches_cpu(gpu, net):
     assert err < 5e-3
 
 
+@pytest.mark.parametrize("n", [1, 63, 777, 70001])
+def test_fused_inference_matches_cpu(gpu, net, n):
+    """The render path's fused inference kernel (network in LDS, activations
+    in registers, SiLU through exp2/rcp) against the numpy oracle with the
+    same BF16 rounding points, and against the training path's GEMM forward,
+    on ragged row counts (partial 16-ray tiles, several passes per wavefront)."""
+    rng = np.random.default_rng(30 + n)
+    x = rng.uniform(-1, 1, (n, 6)).astype(np.float32)
+    yf = net.forward(x, fused=True)
+    yg = net.forward(x)
+    yr, _ = mo.forward(net.params(), x, bf=True)
+    scale = np.abs(yr[:, :5]).max()
+    err_cpu = np.abs(yf - yr[:, :5]).max() / scale
+    err_gemm = np.abs(yf - yg).max() / scale
+    say("fused forward n=%d: max rel err vs cpu %.2e, vs GEMM path %.2e" % (n, err_cpu, err_gemm))
+    assert np.isfinite(yf).all()
+    assert err_cpu < 5e-3 and err_gemm < 5e-3
+
+
 def test_gradients_match_cpu(gpu, net):
     rng = np.random.default_rng(4)
     n = 512
