@@ -72,10 +72,19 @@ struct GemmArgs
     __bf16* Aout;           // kEpiBiasAct: [M x N] activations
     const float* Zin;       // kEpiSiluBack: [M x N] pre-activations of the layer
     __bf16* dZout;          // kEpiSiluBack: [M x N]
-    float* db;              // kEpiSiluBack: [N] bias gradient (accumulated)
+    float* db;              // kEpiSiluBack: [N] bias gradient (accumulated; unused when dbPart is set)
+    float* part;            // kEpiStore: [splitK x M x N] per-split partial products, stored (not added)
+                            // instead of atomics into C; reduce them with launchReduceParts (or null)
+    float* dbPart;          // kEpiSiluBack: [2 * ceil(M / 64) x N] column sums of 32-row slabs, stored
+                            // instead of atomics into db; reduce with launchReduceParts (or null)
 };
 
 hipError_t launchGemm(const GemmArgs& g, int splitK, hipStream_t st);
+// dst[i] += sum over p < P of part[p * L + i], i < L (fixed summation order).
+// tmp (or null): [ceil(P / 64) x L] floats, used when P > 256 for a first
+// pass over segments of 64 partials.
+hipError_t launchReduceParts(const float* part, int P, int L, float* dst, float* tmp, hipStream_t st);
+inline int gemmDbParts(int M) { return 2 * ((M + 63) / 64); }
 hipError_t launchFeatures(const float* x, int M, __bf16* X0, hipStream_t st);
 hipError_t launchLossHead(const float* Y, const float* bias, const MlpTarget* T, int M, __bf16* dOut,
                           __bf16* dOutTrunk, float* lossOut, float* dbHead, hipStream_t st);

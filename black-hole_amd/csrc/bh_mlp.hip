@@ -15,6 +15,8 @@
 //   D (16 x 16): col l % 16, row = 4 (l / 16) + v, v = 0..3 (4 f32 per lane)
 // ---------------------------------------------------------------------------
 #include <hip/hip_runtime.h>
+#include <algorithm>
+
 #include "bh_mlp.h"
 
 namespace bhk {
@@ -28,8 +30,9 @@ BHK_MLP_INL __bf16 tobf(float x) { return (__bf16)x; }
 // C[M x N] (+)= op(A)[M x K] . op(B)[K x N].  op(A) = A (row-major, lda) or
 // A^T (A stored K x M); op(B) = B (row-major K x N) or B^T (B stored N x K).
 //
-// Workgroups of 4 wavefronts own a 64 x 64 output tile (each wavefront a
-// 32 x 32 quarter: 2 x 2 MFMA tiles) and walk K in chunks of 64: both
+// Workgroups of 4 wavefronts own a 64 x kBN output tile (each wavefront a
+// 32 x kBN/2 quarter: 2 x kBN/32 MFMA tiles; kBN = 64) and walk K in chunks
+// of 64: both
 // operand tiles are staged in LDS with k contiguous ([row][k], rows padded
 // to 72 elements), so every MFMA fragment is one 16-byte LDS read whatever
 // the operand's storage order.  An operand stored k-contiguous is copied
@@ -37,7 +40,8 @@ BHK_MLP_INL __bf16 tobf(float x) { return (__bf16)x; }
 // row at a time (16-byte loads) and transposed on its way into LDS.  The
 // vector path needs 8-element-aligned rows and extents (all the network's
 // shapes); other shapes load element by element with bounds checks.
-// gridDim.z splits K (partial tiles accumulated with atomics).
+// gridDim.z splits K: each split's partial tile is stored to `part` for a
+// reduction, or accumulated into C with atomics.
 //
 // Fused epilogues (kEpiBiasAct, kEpiSiluBack) replace the separate
 // bias/SiLU and SiLU-backward passes: same expressions, same rounding
@@ -45,13 +49,13 @@ BHK_MLP_INL __bf16 tobf(float x) { return (__bf16)x; }
 constexpr int kGBM = 64, kGBN = 64, kGBK = 64;
 constexpr int kGLds = kGBK + 8;    // LDS row stride (bf16): 144 bytes, 16-byte aligned
 
-template <bool kVec>
+template <bool kVec, int kRows = 64>
 BHK_MLP_INL void stageTile(const __bf16* __restrict__ X, int ld, int trans, int rows0, int nRows, int k0, int K,
                            __bf16* __restrict__ T, int tid)
 {
-    // T[r][k] = op(X)[rows0 + r][k0 + k], r < 64, k < 64: 512 groups of 8
+    // T[r][k] = op(X)[rows0 + r][k0 + k], r < kRows, k < 64: 8 kRows groups of 8
 #pragma unroll
-    for (int it = 0; it < 2; ++it)
+    for (int it = 0; it < kRows / 32; ++it)
     {
         const int q = tid + 256 * it;
         if (!trans)
@@ -79,7 +83,8 @@ BHK_MLP_INL void stageTile(const __bf16* __restrict__ X, int ld, int trans, int 
         else
         {
             // X[k][row]: 8 consecutive rows of one k, transposed into T
-            const int kr = q >> 3, rg = (q & 7) * 8;
+            constexpr int kGroups = kRows / 8;
+            const int kr = q / kGroups, rg = (q % kGroups) * 8;
             const int k = k0 + kr, row = rows0 + rg;
             bf16x8 v;
             if (kVec)
@@ -102,43 +107,44 @@ BHK_MLP_INL void stageTile(const __bf16* __restrict__ X, int ld, int trans, int 
     }
 }
 
-template <bool kVec>
+template <bool kVec, int kBN>
 __global__ __launch_bounds__(256) void gemmKernel(GemmArgs g)
 {
+    constexpr int kNT = kBN / 32;   // 16-column MFMA tiles per wavefront
     __shared__ __attribute__((aligned(16))) __bf16 As[kGBM * kGLds];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[kGBN * kGLds];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[kBN * kGLds];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.y * kGBM, n0 = blockIdx.x * kGBN;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int m0 = blockIdx.y * kGBM, n0 = blockIdx.x * kBN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * (kBN / 2);
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
     const int chunks = (g.K + kGBK - 1) / kGBK;
     const int per = (chunks + (int)gridDim.z - 1) / (int)gridDim.z;
     const int c0 = (int)blockIdx.z * per, c1 = min(chunks, c0 + per);
-    f32x4 acc[2][2];
+    f32x4 acc[2][kNT];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < kNT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int c = c0; c < c1; ++c)
     {
         const int k0 = c * kGBK;
         stageTile<kVec>(g.A, g.lda, g.transA, m0, g.M, k0, g.K, As, tid);
-        stageTile<kVec>(g.B, g.ldb, !g.transB, n0, g.N, k0, g.K, Bs, tid);
+        stageTile<kVec, kBN>(g.B, g.ldb, !g.transB, n0, g.N, k0, g.K, Bs, tid);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < kGBK; ks += 32)
         {
-            bf16x8 a[2], b[2];
+            bf16x8 a[2], b[kNT];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
                 a[i] = *reinterpret_cast<const bf16x8*>(As + (wm + i * 16 + r16) * kGLds + ks + kq);
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < kNT; ++j)
                 b[j] = *reinterpret_cast<const bf16x8*>(Bs + (wn + j * 16 + r16) * kGLds + ks + kq);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < kNT; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
@@ -146,7 +152,7 @@ __global__ __launch_bounds__(256) void gemmKernel(GemmArgs g)
     // D layout: column r16 of each 16 x 16 tile, rows 4 (lane / 16) + v
     const bool atom = g.accumulate || gridDim.z > 1;
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < kNT; ++j)
     {
         const int col = n0 + wn + j * 16 + r16;
         const bool cok = col < g.N;
@@ -163,7 +169,8 @@ __global__ __launch_bounds__(256) void gemmKernel(GemmArgs g)
                 const float x = acc[i][j][v];
                 if (g.epi == kEpiStore)
                 {
-                    if (atom) atomicAdd(g.C + o, x);
+                    if (g.part) g.part[(size_t)blockIdx.z * g.M * g.N + (size_t)row * g.N + col] = x;
+                    else if (atom) atomicAdd(g.C + o, x);
                     else g.C[o] = x;
                 }
                 else if (g.epi == kEpiBiasAct)
@@ -187,12 +194,61 @@ __global__ __launch_bounds__(256) void gemmKernel(GemmArgs g)
             // the four lanes of a column hold its 32 rows of this wavefront
             colSum += __shfl_xor(colSum, 16);
             colSum += __shfl_xor(colSum, 32);
-            if (lane < 16 && cok && colSum != 0.f) atomicAdd(g.db + col, colSum);
+            if (g.dbPart)
+            {
+                if (lane < 16 && cok) g.dbPart[(size_t)(blockIdx.y * 2 + (wave >> 1)) * g.N + col] = colSum;
+            }
+            else if (lane < 16 && cok && colSum != 0.f)
+                atomicAdd(g.db + col, colSum);
         }
     }
 }
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// out[y * L + i] (+)= sum over p in segment y of part[p * L + i]: a workgroup
+// owns 64 consecutive i and one segment of p (blockIdx.y); its 16 wavefronts
+// sum interleaved slices of the segment (coalesced 64-lane rows), then the
+// slices are added in LDS in a fixed order.  `add`: accumulate into out.
+__global__ __launch_bounds__(1024) void reducePartsKernel(const float* part, int P, int L, float* out, int add)
+{
+    __shared__ float sl[16][64];
+    const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + c;
+    const int seg = (P + (int)gridDim.y - 1) / (int)gridDim.y;
+    const int p0 = (int)blockIdx.y * seg, p1 = min(P, p0 + seg);
+    float s = 0.f;
+    if (i < L)
+        for (int p = p0 + w; p < p1; p += 16) s += part[(size_t)p * L + i];
+    sl[w][c] = s;
+    __syncthreads();
+    if (w == 0 && i < L)
+    {
+        float t = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) t += sl[k][c];
+        float* o = out + (size_t)blockIdx.y * L + i;
+        *o = add ? *o + t : t;
+    }
+}
+
+hipError_t launchReduceParts(const float* part, int P, int L, float* dst, float* tmp, hipStream_t st)
+{
+    if (P <= 0 || L <= 0) return hipSuccess;
+    const unsigned bx = (unsigned)((L + 63) / 64);
+    if (P > 256 && tmp)
+    {
+        // few long columns: segments of 64 partials in parallel, then their sums
+        const int S = (P + 63) / 64;
+        hipLaunchKernelGGL(reducePartsKernel, dim3(bx, S), dim3(1024), 0, st, part, P, L, tmp, 0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        part = tmp;
+        P = S;
+    }
+    hipLaunchKernelGGL(reducePartsKernel, dim3(bx, 1), dim3(1024), 0, st, part, P, L, dst, 1);
+    return hipGetLastError();
+}
 
 hipError_t launchGemm(const GemmArgs& g, int splitK, hipStream_t st)
 {
@@ -204,9 +260,12 @@ hipError_t launchGemm(const GemmArgs& g, int splitK, hipStream_t st)
     const int contA = g.transA ? g.M : g.K, contB = g.transB ? g.K : g.N;
     const bool vec = aligned16(g.A) && aligned16(g.B) && g.lda % 8 == 0 && g.ldb % 8 == 0 && contA % 8 == 0 &&
                      contB % 8 == 0;
+    // 64 x 64 tiles: 64 x 128 tiles (each A row block staged once for the
+    // full width of a layer) halve the workgroup count and were slower on
+    // every training shape (step 436 -> 513 us at batch 65536)
     dim3 grid((g.N + kGBN - 1) / kGBN, (g.M + kGBM - 1) / kGBM, splitK);
-    if (vec) hipLaunchKernelGGL(gemmKernel<true>, grid, dim3(256), 0, st, g);
-    else hipLaunchKernelGGL(gemmKernel<false>, grid, dim3(256), 0, st, g);
+    if (vec) hipLaunchKernelGGL((gemmKernel<true, kGBN>), grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL((gemmKernel<false, kGBN>), grid, dim3(256), 0, st, g);
     return hipGetLastError();
 }
 
@@ -261,11 +320,16 @@ constexpr float kMlpClip = 20.f;      // mrad
 constexpr float kMlpRegW = 1e-3f;   // (mrad^2: the balance against the disk BCE of a 1e3 weight on rad^2)
 
 __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const float* bias, const MlpTarget* T, int M,
-                                                      __bf16* dOut, __bf16* dOutTrunk, float* lossOut)
+                                                      __bf16* dOut, __bf16* dOutTrunk, float* lossOut, float* dbHead)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    float lReg = 0.f, lBce = 0.f, lHit = 0.f;
-    if (i < M)
+    // grid-stride over rows; per-thread sums of the three loss terms and of
+    // the head-bias gradient db[c] = sum_i dOut[i][c] (the bf16 values the
+    // backward GEMMs read), reduced per workgroup before one atomic each
+    float part[3 + kMlpOut];
+#pragma unroll
+    for (int c = 0; c < 3 + kMlpOut; ++c) part[c] = 0.f;
+    const float invM = 1.f / (float)M;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
     {
         float y[kMlpOutPad];
 #pragma unroll
@@ -273,10 +337,10 @@ __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const floa
         float g[kMlpOutPad];
 #pragma unroll
         for (int c = 0; c < kMlpOutPad; ++c) g[c] = 0.f;
+        float lReg = 0.f, lHit = 0.f;
         const MlpTarget t = T[i];
-        const float invM = 1.f / (float)M;
         const float p = 1.f / (1.f + __expf(-y[2]));
-        lBce = fmaxf(y[2], 0.f) - y[2] * t.disk + __logf(1.f + __expf(-fabsf(y[2])));
+        const float lBce = fmaxf(y[2], 0.f) - y[2] * t.disk + __logf(1.f + __expf(-fabsf(y[2])));
         g[2] = (p - t.disk) * invM;
         if (t.escaped > 0.5f && t.band < 0.5f)
         {
@@ -311,47 +375,41 @@ __global__ __launch_bounds__(256) void lossHeadKernel(const float* Y, const floa
 #pragma unroll
         for (int c = 0; c < kMlpOutPad; ++c)
         {
-            dOut[(size_t)i * kMlpOutPad + c] = tobf(g[c]);
+            const __bf16 gb = tobf(g[c]);
+            dOut[(size_t)i * kMlpOutPad + c] = gb;
             dOutTrunk[(size_t)i * kMlpOutPad + c] = tobf(c == 4 ? 0.f : g[c]);
+            if (c < kMlpOut) part[3 + c] += (float)gb;
         }
+        part[0] += lReg;
+        part[1] += lBce;
+        part[2] += lHit;
     }
-    // wavefront reduction of the three loss terms
-    for (int off = 32; off > 0; off >>= 1)
+    __shared__ float red[4][3 + kMlpOut];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 3 + kMlpOut; ++c)
     {
-        lReg += __shfl_down(lReg, off);
-        lBce += __shfl_down(lBce, off);
-        lHit += __shfl_down(lHit, off);
+        float v = part[c];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if (lane == 0) red[wave][c] = v;
     }
-    if ((threadIdx.x & 63) == 0)
+    __syncthreads();
+    if (threadIdx.x < 3 + kMlpOut)
     {
-        atomicAdd(&lossOut[0], lReg);
-        atomicAdd(&lossOut[1], lBce);
-        atomicAdd(&lossOut[2], lHit);
+        const int c = threadIdx.x;
+        const float v = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+        atomicAdd(c < 3 ? &lossOut[c] : &dbHead[c - 3], v);
     }
-}
-
-// head bias gradient: db[c] = sum_i dOut[i][c] (dOut already scaled by 1/M)
-__global__ __launch_bounds__(256) void headBiasGradKernel(const __bf16* dOut, int M, float* db)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    float v[kMlpOut];
-#pragma unroll
-    for (int c = 0; c < kMlpOut; ++c) v[c] = i < M ? (float)dOut[(size_t)i * kMlpOutPad + c] : 0.f;
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int c = 0; c < kMlpOut; ++c) v[c] += __shfl_down(v[c], off);
-    if ((threadIdx.x & 63) == 0)
-#pragma unroll
-        for (int c = 0; c < kMlpOut; ++c) atomicAdd(&db[c], v[c]);
 }
 
 hipError_t launchLossHead(const float* Y, const float* bias, const MlpTarget* T, int M, __bf16* dOut,
                           __bf16* dOutTrunk, float* lossOut, float* dbHead, hipStream_t st)
 {
     if (M <= 0) return hipSuccess;
-    hipLaunchKernelGGL(lossHeadKernel, dim3((M + 255) / 256), dim3(256), 0, st, Y, bias, T, M, dOut, dOutTrunk,
-                       lossOut);
-    hipLaunchKernelGGL(headBiasGradKernel, dim3((M + 255) / 256), dim3(256), 0, st, dOut, M, dbHead);
+    // at most 128 workgroups: 1024 atomics in all, on 8 addresses
+    const int blocks = std::max(1, std::min((M + 255) / 256, 128));
+    hipLaunchKernelGGL(lossHeadKernel, dim3(blocks), dim3(256), 0, st, Y, bias, T, M, dOut, dOutTrunk, lossOut,
+                       dbHead);
     return hipGetLastError();
 }
 

@@ -57,6 +57,7 @@ struct Layout
     }
 };
 const Layout kL;
+constexpr int kMaxSplit = 128;   // batch chunks of the weight-gradient GEMMs
 
 template <class T> hipError_t dalloc(T** p, size_t n)
 {
@@ -83,6 +84,8 @@ struct BhSurrogate
     __bf16* dOut = nullptr;      // [cap x 16]
     __bf16* dOutT = nullptr;     // [cap x 16], log-variance column zero
     __bf16* dZ[2] = {};          // [cap x 128] each: gradients at the pre-activations (ping-pong)
+    float* part = nullptr;       // split-K partials of the weight gradients [kMaxSplit x 128 x 224]
+    float* dbPart = nullptr;     // column-sum partials of the bias gradients [gemmDbParts(cap) x 128]
     float* loss = nullptr;       // per-step loss terms [lossCap x 3]
     int lossCap = 0;
     // training set
@@ -104,6 +107,9 @@ void freeWorkspace(BhSurrogate* s)
     (void)hipFree(s->dOutT);
     s->dOutT = nullptr;
     for (int k = 0; k < 2; ++k) { (void)hipFree(s->dZ[k]); s->dZ[k] = nullptr; }
+    (void)hipFree(s->part);
+    (void)hipFree(s->dbPart);
+    s->part = nullptr; s->dbPart = nullptr;
     s->X0 = nullptr; s->Y5 = nullptr; s->dOut = nullptr;
     s->cap = 0;
 }
@@ -126,6 +132,8 @@ hipError_t ensureWorkspace(BhSurrogate* s, int rows, hipStream_t st)
     if (e == hipSuccess) e = dalloc(&s->dOutT, n * kMlpOutPad);
     if (e == hipSuccess) e = dalloc(&s->dZ[0], n * kMlpHidden);
     if (e == hipSuccess) e = dalloc(&s->dZ[1], n * kMlpHidden);
+    if (e == hipSuccess) e = dalloc(&s->part, (size_t)kMaxSplit * kMlpHidden * kMlpIn);
+    if (e == hipSuccess) e = dalloc(&s->dbPart, (size_t)bhk::gemmDbParts(rows) * kMlpHidden);
     if (e != hipSuccess) { freeWorkspace(s); return e; }
     s->cap = rows;
     return hipSuccess;
@@ -143,6 +151,7 @@ bhk::GemmArgs gemm(const __bf16* A, int lda, int transA, const __bf16* B, int ld
     g.epi = bhk::kEpiStore;
     g.act = 0;
     g.bias = nullptr; g.Zout = nullptr; g.Aout = nullptr; g.Zin = nullptr; g.dZout = nullptr; g.db = nullptr;
+    g.part = nullptr; g.dbPart = nullptr;
     return g;
 }
 
@@ -171,16 +180,41 @@ hipError_t forward(BhSurrogate* s, const float* x, int rows, hipStream_t st)
     return e;
 }
 
-// dZ_{l-1} = (dZ_l . W_l) * silu'(Z_{l-1}) and db_{l-1} in one GEMM
+// dZ_{l-1} = (dZ_l . W_l) * silu'(Z_{l-1}) in one GEMM; its epilogue stores
+// the column sums of db_{l-1} per 32-row slab in dbPart (reduced after)
 bhk::GemmArgs backData(const __bf16* dY, int ldy, const __bf16* W, int in, int outN, const float* Zprev,
-                       __bf16* dZout, float* db, int rows)
+                       __bf16* dZout, float* dbPart, int rows)
 {
     bhk::GemmArgs g = gemm(dY, ldy, 0, W, in, 0, nullptr, in, rows, in, outN, 0);
     g.epi = bhk::kEpiSiluBack;
     g.Zin = Zprev;
     g.dZout = dZout;
-    g.db = db;
+    g.dbPart = dbPart;
     return g;
+}
+
+// dW = dY^T X over the batch (K = rows), split over the batch into partial
+// products that a reduction adds into the gradient (no atomics; the
+// summation order is fixed)
+hipError_t weightGrad(BhSurrogate* s, const __bf16* dY, int ldy, int outM, const __bf16* X, int in, float* gW,
+                      int rows, hipStream_t st)
+{
+    const int split = std::max(1, std::min(kMaxSplit, rows / 512));
+    bhk::GemmArgs g = gemm(dY, ldy, 1, X, in, 0, gW, in, outM, in, rows, 0);
+    g.part = s->part;
+    hipError_t e = bhk::launchGemm(g, split, st);
+    if (e == hipSuccess) e = bhk::launchReduceParts(s->part, split, outM * in, gW, nullptr, st);
+    return e;
+}
+
+hipError_t dataGrad(BhSurrogate* s, const __bf16* dY, int ldy, const __bf16* W, int in, int outN, const float* Zprev,
+                    __bf16* dZout, float* gb, int rows, hipStream_t st)
+{
+    hipError_t e = bhk::launchGemm(backData(dY, ldy, W, in, outN, Zprev, dZout, s->dbPart, rows), 1, st);
+    // the weight-gradient partials are consumed by now: their buffer holds
+    // the first pass of this reduction
+    if (e == hipSuccess) e = bhk::launchReduceParts(s->dbPart, bhk::gemmDbParts(rows), in, gb, s->part, st);
+    return e;
 }
 
 // Backward pass after forward(): gradients accumulate into s->g (zeroed by
@@ -189,24 +223,19 @@ hipError_t backward(BhSurrogate* s, const bhk::MlpTarget* T, int rows, float* lo
 {
     hipError_t e =
         bhk::launchLossHead(s->Y5, s->p + kL.b[4], T, rows, s->dOut, s->dOutT, lossSlot, s->g + kL.b[4], st);
-    // weight gradients reduce over the batch: split it into chunks of 512 rows
-    const int split = std::max(1, std::min(128, rows / 512));
     // head: dW4 = dOut^T A3; dZ3 = (dOut W4) * silu'(Z3), db3
+    if (e == hipSuccess) e = weightGrad(s, s->dOut, kMlpOutPad, kMlpOutPad, s->A[3], kMlpHidden, s->g + kL.w[4], rows, st);
     if (e == hipSuccess)
-        e = bhk::launchGemm(gemm(s->dOut, kMlpOutPad, 1, s->A[3], kMlpHidden, 0, s->g + kL.w[4], kMlpHidden,
-                                 kMlpOutPad, kMlpHidden, rows, 1), split, st);
-    if (e == hipSuccess)
-        e = bhk::launchGemm(backData(s->dOutT, kMlpOutPad, s->pb + kL.w[4], kMlpHidden, kMlpOutPad, s->Z[3],
-                                     s->dZ[1], s->g + kL.b[3], rows), 1, st);
+        e = dataGrad(s, s->dOutT, kMlpOutPad, s->pb + kL.w[4], kMlpHidden, kMlpOutPad, s->Z[3], s->dZ[1],
+                     s->g + kL.b[3], rows, st);
     for (int l = 3, cur = 1; l >= 0 && e == hipSuccess; --l, cur ^= 1)
     {
         const int in = bhk::mlpLayerIn(l);
         const __bf16* X = (l == 0) ? s->X0 : s->A[l - 1];
-        e = bhk::launchGemm(gemm(s->dZ[cur], kMlpHidden, 1, X, in, 0, s->g + kL.w[l], in, kMlpHidden, in, rows, 1),
-                            split, st);
+        e = weightGrad(s, s->dZ[cur], kMlpHidden, kMlpHidden, X, in, s->g + kL.w[l], rows, st);
         if (e == hipSuccess && l > 0)
-            e = bhk::launchGemm(backData(s->dZ[cur], kMlpHidden, s->pb + kL.w[l], in, kMlpHidden, s->Z[l - 1],
-                                         s->dZ[cur ^ 1], s->g + kL.b[l - 1], rows), 1, st);
+            e = dataGrad(s, s->dZ[cur], kMlpHidden, s->pb + kL.w[l], in, kMlpHidden, s->Z[l - 1], s->dZ[cur ^ 1],
+                         s->g + kL.b[l - 1], rows, st);
     }
     return e;
 }
@@ -494,7 +523,20 @@ bh_error_t bhMlpGemmTest(const void* A, int lda, int transA, const void* B, int 
     if (!A || !B || !C) return hipErrorInvalidValue;
     bhk::GemmArgs g = gemm(static_cast<const __bf16*>(A), lda, transA, static_cast<const __bf16*>(B), ldb, transB, C,
                            ldc, M, N, K, splitK > 1 ? 1 : 0);
-    return bhk::launchGemm(g, splitK, (hipStream_t)stream);
+    hipStream_t st = (hipStream_t)stream;
+    if (splitK <= 1 || splitK > kMaxSplit || ldc != N) return bhk::launchGemm(g, splitK, st);
+    // split K as the weight gradients run it: partial products stored per
+    // split, then reduced into C (C += sum)
+    float* part = nullptr;
+    hipError_t e = dalloc(&part, (size_t)splitK * M * N);
+    if (e != hipSuccess) return e;
+    g.accumulate = 0;
+    g.part = part;
+    e = bhk::launchGemm(g, splitK, st);
+    if (e == hipSuccess) e = bhk::launchReduceParts(part, splitK, M * N, C, nullptr, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipFree(part);
+    return e != hipSuccess ? e : es;
 }
 
 }  // extern "C"

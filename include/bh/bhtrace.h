@@ -509,7 +509,9 @@ bh_error_t bhLastSurrogateStats(unsigned* candidates, unsigned* accepted, bh_str
  * consulted; hipErrorNotReady otherwise.  Synchronizes the stream. */
 bh_error_t bhLastSurrogateKernelMs(float* ms, bh_stream_t stream);
 /* The MFMA GEMM used by the network, exposed for tests: C (+)= op(A) op(B)
- * with bf16 operands (device) and f32 C (device; zero it when splitK > 1). */
+ * with bf16 operands (device) and f32 C (device; zero it when splitK > 1).
+ * With splitK > 1 and ldc == N the K splits go through the weight-gradient
+ * path (per-split partials reduced into C; synchronizes the stream). */
 bh_error_t bhMlpGemmTest(const void* A, int lda, int transA, const void* B, int ldb, int transB, float* C, int ldc,
                          int M, int N, int K, int splitK, bh_stream_t stream);
 

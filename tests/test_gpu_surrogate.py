@@ -33,7 +33,8 @@ def bits_to_f64(u):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-@pytest.mark.parametrize("M,N,K", [(37, 45, 70), (16, 16, 32), (130, 16, 224), (64, 128, 128)])
+@pytest.mark.parametrize("M,N,K", [(37, 45, 70), (16, 16, 32), (130, 16, 224), (64, 128, 128), (37, 130, 70),
+                                   (130, 224, 640)])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 @pytest.mark.parametrize("split", [1, 3])
 def test_gemm_layouts(gpu, M, N, K, ta, tb, split):
