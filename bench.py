@@ -212,7 +212,8 @@ def surrogate_leg(bt, lib, frames, steps, sigma, plain_ms):
                 rms = float(np.median(route_ms))
                 r["route_kernel_ms"] = round(rms, 4)
                 tf = cand * MLP_FLOPS_PER_ROW / (rms * 1e-3) / 1e12
-                infer_roof = {"bound": "mfma_bf16", "kernel": "surrogateRouteKernel", "rows": int(cand),
+                infer_roof = {"bound": "mfma_bf16", "kernel": "surrogateRouteKernel<8, 4>",
+                              "kernels_timed": ["surrogateRouteKernel<8, 4>", "surrogateFinishKernel"], "rows": int(cand),
                               "flops_per_launch": int(cand * MLP_FLOPS_PER_ROW), "kernel_ms": round(rms, 4),
                               "achieved": round(tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                               "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)}
@@ -221,8 +222,8 @@ def surrogate_leg(bt, lib, frames, steps, sigma, plain_ms):
         out["headline_camera_disk_off"] = disk_off
         tf = MLP_TRAIN_FLOPS_PER_STEP / (step_ms * 1e-3) / 1e12
         out["mlp_roofline"] = {
-            "train": {"bound": "mfma_bf16", "kernels": ["gemmKernel<true>", "featuresKernel", "lossHeadKernel",
-                                                        "headBiasGradKernel", "adamKernel", "zeroKernel"],
+            "train": {"bound": "mfma_bf16", "kernels": ["gemmKernel<true, 64>", "reducePartsKernel", "featuresKernel",
+                                                        "lossHeadKernel", "adamKernel", "zeroKernel"],
                       "flops_per_step": MLP_TRAIN_FLOPS_PER_STEP, "step_ms": round(step_ms, 4),
                       "achieved": round(tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                       "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)},
