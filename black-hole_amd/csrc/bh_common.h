@@ -199,6 +199,39 @@ BHK_HD float captureRadius(const RenderParams& P)
 // ------------------------------------------------------------------ blackbody (trace.cuh:101-120)
 template <class M> BHK_INL v3 blackbody(float kelvin)
 {
+    if constexpr (M::fast)
+    {
+        // The same fit with its 2.2 power taken in the log2 domain: above
+        // 6600 K red and green are c (t - 60)^k clamped to 255, so
+        // (min(c (t-60)^k, 255) / 255)^2.2 = exp2(2.2 min(log2(c/255) + k log2(t-60), 0))
+        // (log2(329.698727446 / 255) and log2(288.1221695283 / 255) below)
+        // with one log2 shared by both channels; the constant channels
+        // (255 -> 1) need no power at all.  Equal to the direct form up to
+        // the rounding of the hardware exp2/log2.
+        const float t = clampf(kelvin, 1200.f, 40000.f) * 0.01f;
+        float r, g, b;
+        if (t <= 66.f)
+        {
+            r = 1.f;
+            const float gl = clampf(99.4708025861f * M::log_(t) - 161.1195681661f, 0.f, 255.f) * (1.f / 255.f);
+            g = __builtin_amdgcn_exp2f(2.2f * __builtin_amdgcn_logf(gl));
+        }
+        else
+        {
+            const float L = __builtin_amdgcn_logf(t - 60.f);
+            r = __builtin_amdgcn_exp2f(2.2f * fminf(fmaf(-0.1332047592f, L, 0.370651066f), 0.f));
+            g = __builtin_amdgcn_exp2f(2.2f * fminf(fmaf(-0.0755148492f, L, 0.176183432f), 0.f));
+        }
+        if (t >= 66.f) b = 1.f;
+        else if (t <= 19.f) b = 0.f;
+        else
+        {
+            const float bl = clampf(138.5177312231f * M::log_(t - 10.f) - 305.0447927307f, 0.f, 255.f) *
+                             (1.f / 255.f);
+            b = __builtin_amdgcn_exp2f(2.2f * __builtin_amdgcn_logf(bl));
+        }
+        return mk3(r, g, b);
+    }
     float t = M::divc(clampf(kelvin, 1200.f, 40000.f), 100.f, 0.01f);
     float r, g, b;
     if (t <= 66.f) r = 255.f;
@@ -248,6 +281,18 @@ BHK_INL float wrapDeltaPhi(float d)
     d = fmodf(d + kPi, 2.f * kPi);
     if (d < 0.f) d += 2.f * kPi;
     return d - kPi;
+}
+
+// The same wrap by d - 2 pi floor((d + pi) / (2 pi)) with a fused
+// multiply-add: off by a few ulp of d from the exact fmod for the |d| of
+// tens of radians the hot spots see (they enter a Gaussian of width 0.18).
+BHK_INL float wrapDeltaPhiFast(float d)
+{
+    const float e = d + kPi;
+    const float q = floorf(e * 0.159154943091895f);
+    float w = fmaf(-q, 2.f * kPi, e);
+    w = w < 0.f ? w + 2.f * kPi : (w >= 2.f * kPi ? w - 2.f * kPi : w);
+    return w - kPi;
 }
 
 // ------------------------------------------------------------------ disk (trace.cuh:269-403)
@@ -376,8 +421,17 @@ BHK_INL bool diskSubSample(const DiskParams& D, float E, float Lz, v3 p, float t
         const float om1 = D.om1 <= 0.f ? omega : D.om1;
         const float inv2sPhi = 1.f / (2.f * 0.18f * 0.18f);
         const float inv2sR0 = D.inv2sR0, inv2sR1 = D.inv2sR1;
-        float dph0 = wrapDeltaPhi(azim - (om0 * tEmit + 0.4f));
-        float dph1 = wrapDeltaPhi(azim - (om1 * tEmit + 0.4f + kPi));
+        float dph0, dph1;
+        if constexpr (M::fast)
+        {
+            dph0 = wrapDeltaPhiFast(azim - (om0 * tEmit + 0.4f));
+            dph1 = wrapDeltaPhiFast(azim - (om1 * tEmit + 0.4f + kPi));
+        }
+        else
+        {
+            dph0 = wrapDeltaPhi(azim - (om0 * tEmit + 0.4f));
+            dph1 = wrapDeltaPhi(azim - (om1 * tEmit + 0.4f + kPi));
+        }
         float dr0 = rc - rHs0;
         float dr1 = rc - rHs1;
         float flick0 = 0.72f + 0.28f * M::sin_(1.7f * tEmit + 0.9f);
@@ -434,7 +488,11 @@ struct Chord
     int n;
 };
 
-template <class M> BHK_INL int chordPrepare(const DiskParams& D, v3 a, v3 b, float tEmit, Chord& c)
+// Chord [a, b] with the cylindrical radii of its ends given (rcA, rcB):
+// they enter only the culls below, which skip chords whose sub-samples all
+// fail the gates of diskSubSample.
+template <class M>
+BHK_INL int chordPrepareRc(const DiskParams& D, v3 a, v3 b, float rcA, float rcB, float tEmit, Chord& c)
 {
     c.n = 0;
     v3 seg = b - a;
@@ -442,8 +500,6 @@ template <class M> BHK_INL int chordPrepare(const DiskParams& D, v3 a, v3 b, flo
     if (segLen < 1e-7f) return 0;
     if (a.y > D.maxH && b.y > D.maxH) return 0;
     if (a.y < -D.maxH && b.y < -D.maxH) return 0;
-    float rcA = M::sqrt_(a.x * a.x + a.z * a.z);
-    float rcB = M::sqrt_(b.x * b.x + b.z * b.z);
     float rcMin = fminf(rcA, rcB), rcMax = fmaxf(rcA, rcB);
     if (rcMin - segLen > D.rout) return 0;
     if (rcMax + segLen < D.rin) return 0;
@@ -465,6 +521,12 @@ template <class M> BHK_INL int chordPrepare(const DiskParams& D, v3 a, v3 b, flo
     c.ds = M::div(segLen, (float)nSub);
     c.n = nSub;
     return nSub;
+}
+template <class M> BHK_INL int chordPrepare(const DiskParams& D, v3 a, v3 b, float tEmit, Chord& c)
+{
+    if (a.y > D.maxH && b.y > D.maxH) { c.n = 0; return 0; }
+    if (a.y < -D.maxH && b.y < -D.maxH) { c.n = 0; return 0; }
+    return chordPrepareRc<M>(D, a, b, M::sqrt_(a.x * a.x + a.z * a.z), M::sqrt_(b.x * b.x + b.z * b.z), tEmit, c);
 }
 
 // Sub-sample j of a chord: position at t = (j + 1/2) / n along [a, b].

@@ -280,6 +280,8 @@ template <class M, bool kWant> struct KerrRay
     int step, status, haveDeriv;
     KDeriv nx;                    // fast policy: right-hand side at the current state
     float nxK;
+    float prevRc, phPrev;         // fast policy: cylindrical radius and azimuth of prevPos
+    bool prevXZ;                  // fast policy: prevPos.x, .z are current
 
     BHK_INL void init(const FrameK& K, v3 origin, v3 dir)
     {
@@ -323,6 +325,9 @@ template <class M, bool kWant> struct KerrRay
         sth = s; cth = c;
         tFlight = 0.f;
         prevPos = origin;
+        prevRc = M::sqrt_(origin.x * origin.x + origin.z * origin.z);
+        phPrev = ph0;
+        prevXZ = true;
         dir3 = dir;
         // fast policy: the first step's first stage (each step then leaves
         // the next step's first stage in nx)
@@ -456,10 +461,74 @@ template <class M, bool kWant> struct KerrRay
             // (trace.cuh:859-862), but its only reader, the disk segment
             // sampler, ignores it (trace.cuh:270), and the exit direction
             // comes from the coordinate velocity: it is not formed
-            float sp = M::sin_(ph), cp = M::cos_(ph);
-            v3 pos = mk3(r * sth * cp, r * cth, -r * sth * sp);
-            if (trans > 0.01f) chordPrepare<M>(K.D, prevPos, pos, K.diskTime - tFlight, ch);
-            prevPos = pos;
+            if constexpr (M::fast)
+            {
+                // The new point's cylindrical radius r sin(th) and height
+                // r cos(th) need no azimuth, and the chord culls need no more
+                // (the previous point's are carried).  The culls are first
+                // applied from them alone: the slab and scale-height tests
+                // exactly as in chordPrepareRc, the radial test against an
+                // upper bound of the chord length, |b - a|^2 <= (rcB - rcA)^2
+                // + rcA rcB dph^2 + (yB - yA)^2 (from 2 (1 - cos x) <= x^2),
+                // widened by 1% + 1e-4 rcMax^2 for the rounding of the
+                // hardware sin/cos and by 0.01 in the gap.  Only chords that
+                // may survive form their Cartesian ends: the same products
+                // as r sin(th) cos(ph) etc., so the ends, the chord and the
+                // exact culls of chordPrepareRc are unchanged.
+                const float rcB = r * sth, yB = r * cth;
+                const float rcA = prevRc, yA = prevPos.y;
+                const DiskParams& D = K.D;
+                bool may = trans > 0.01f;
+                if (may)
+                {
+                    const float rcMax = fmaxf(rcA, rcB);
+                    if ((yA > D.maxH && yB > D.maxH) || (yA < -D.maxH && yB < -D.maxH)) may = false;
+                    else if ((yA > 0.f) == (yB > 0.f) &&
+                             fminf(fabsf(yA), fabsf(yB)) > 1.02f * 2.6f * fmaxf(0.055f * rcMax, 1e-4f))
+                        may = false;
+                    else
+                    {
+                        const float dph = ph - phPrev, drc = rcB - rcA, dy = yB - yA;
+                        const float seg2 = fmaf(1.01f * rcA * rcB, dph * dph, 1.01f * fmaf(drc, drc, dy * dy)) +
+                                           1e-4f * rcMax * rcMax;
+                        const float outGap = fminf(rcA, rcB) - D.rout - 0.01f;
+                        const float inGap = D.rin - rcMax - 0.01f;
+                        if ((outGap > 0.f && outGap * outGap > seg2) || (inGap > 0.f && inGap * inGap > seg2))
+                            may = false;
+                    }
+                }
+                if (may)
+                {
+                    const float sp = M::sin_(ph), cp = M::cos_(ph);
+                    const v3 pos = mk3(rcB * cp, yB, -rcB * sp);
+                    if (!prevXZ)
+                    {
+                        // rounded products, as when the end is carried from
+                        // the step before (not fused into the chord's sums)
+#pragma clang fp contract(off)
+                        const float spa = M::sin_(phPrev), cpa = M::cos_(phPrev);
+                        prevPos.x = rcA * cpa;
+                        prevPos.z = -rcA * spa;
+                    }
+                    chordPrepareRc<M>(D, prevPos, pos, rcA, rcB, K.diskTime - tFlight, ch);
+                    prevPos = pos;
+                    prevXZ = true;
+                }
+                else
+                {
+                    prevPos.y = yB;
+                    prevXZ = false;
+                }
+                prevRc = rcB;
+                phPrev = ph;
+            }
+            else
+            {
+                float sp = M::sin_(ph), cp = M::cos_(ph);
+                v3 pos = mk3(r * sth * cp, r * cth, -r * sth * sp);
+                if (trans > 0.01f) chordPrepare<M>(K.D, prevPos, pos, K.diskTime - tFlight, ch);
+                prevPos = pos;
+            }
         }
         ++step;
         return false;
