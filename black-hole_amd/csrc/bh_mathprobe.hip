@@ -36,6 +36,15 @@ __global__ __launch_bounds__(256) void mathProbeKernel(int fn, const double* __r
         break;
     }
     case BH_MATH_D64_RCP: r0 = D64Fast::rcp(a); break;
+    case BH_MATH_BLACKBODY:
+    {
+        const v3 f = blackbody<MathFast>((float)a), e = blackbody<MathExact>((float)a);
+        const int c = (int)b;
+        r0 = c == 0 ? f.x : (c == 1 ? f.y : f.z);
+        r1 = c == 0 ? e.x : (c == 1 ? e.y : e.z);
+        break;
+    }
+    case BH_MATH_WRAP_PHI: r0 = wrapDeltaPhiFast((float)a); r1 = wrapDeltaPhi((float)a); break;
     default: r0 = r1 = __builtin_nan("");
     }
     out[2 * i] = r0;
@@ -46,7 +55,7 @@ __global__ __launch_bounds__(256) void mathProbeKernel(int fn, const double* __r
 
 extern "C" bh_error_t bhMathProbe(int fn, const double* devIn, int n, double* devOut, bh_stream_t stream)
 {
-    if (n < 0 || (n > 0 && (!devIn || !devOut)) || fn < 0 || fn > BH_MATH_D64_RCP) return hipErrorInvalidValue;
+    if (n < 0 || (n > 0 && (!devIn || !devOut)) || fn < 0 || fn > BH_MATH_WRAP_PHI) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(bhk::mathProbeKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        fn, devIn, n, devOut);

@@ -226,7 +226,10 @@ enum
     BH_MATH_FAST_POW = 7,         /* a^b via hardware exp2/log2 */
     BH_MATH_D64_SINCOS = 8,       /* sin(a), cos(a) in double: Cody-Waite reduction and Taylor polynomials */
     BH_MATH_D64_SINCOS_NEAR = 9,  /* sin, cos of a + b from those of a by angle addition (|b| < 1/16) */
-    BH_MATH_D64_RCP = 10          /* 1/a in double: hardware seed and two Newton steps */
+    BH_MATH_D64_RCP = 10,         /* 1/a in double: hardware seed and two Newton steps */
+    BH_MATH_BLACKBODY = 11,       /* channel (int)b of the disk's linear blackbody colour at a kelvin:
+                                     [0] fast policy (2.2 power in the log2 domain), [1] exact policy */
+    BH_MATH_WRAP_PHI = 12         /* a wrapped to (-pi, pi]: [0] fast policy (floor), [1] exact (fmodf) */
 };
 bh_error_t bhMathProbe(int fn, const double* devIn, int n, double* devOut, bh_stream_t stream);
 

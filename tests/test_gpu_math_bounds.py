@@ -7,6 +7,10 @@ evaluated on the GPU exactly as the fast trace kernels evaluate them
   magnitudes;
 * the fast policy's hardware sin/cos (of theta and of the winding phi), rcp,
   rsq, sqrt, and exp/log/pow from hardware exp2/log2;
+* the fast policy's disk blackbody (the 2.2 power of Tanner Helland's fit
+  taken in the log2 domain) and hot-spot azimuth wrap (floor instead of
+  fmodf), against the published fit / the exact wrap in float64 and against
+  the exact policy's own evaluation on the GPU;
 * the FP64 band integrator's sin/cos (Cody-Waite reduction modulo pi/2 with
   a two-term constant, Taylor polynomials to degree 15/16) over |x| <= 300,
   its angle-addition step for |dl| < 1/16, and its reciprocal (hardware
@@ -20,7 +24,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 LD = np.longdouble
-ATAN2, SINCOS, RCP, RSQ, SQRT, EXP, LOG, POW, D64SC, D64NEAR, D64RCP = range(11)
+ATAN2, SINCOS, RCP, RSQ, SQRT, EXP, LOG, POW, D64SC, D64NEAR, D64RCP, BLACKBODY, WRAPPHI = range(13)
 
 
 def say(*a):
@@ -124,3 +128,48 @@ def test_d64_rcp_bound(gpu):
     rel = np.abs((got.astype(LD) - ref) / ref).astype(np.float64).max()
     say("D64 rcp: max relative error %.3e" % rel)
     assert rel <= 2.3e-16
+
+
+def helland_linear(kelvin):
+    """(Tanner Helland's fit / 255)^2.2 in float64: the disk's linear colour."""
+    t = np.clip(kelvin, 1200.0, 40000.0) / 100.0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        lo = t <= 66.0
+        r = np.where(lo, 255.0, 329.698727446 * np.abs(t - 60.0) ** -0.1332047592)
+        g = np.where(lo, 99.4708025861 * np.log(t) - 161.1195681661, 288.1221695283 * np.abs(t - 60.0) ** -0.0755148492)
+        b = np.where(t >= 66.0, 255.0, np.where(t <= 19.0, 0.0, 138.5177312231 * np.log(np.maximum(t - 10.0, 1e-9)) - 305.0447927307))
+    return (np.clip(np.stack([r, g, b]), 0.0, 255.0) / 255.0) ** 2.2
+
+
+def test_fast_blackbody_bound(gpu):
+    k = f32(np.concatenate([np.linspace(1000.0, 41000.0, 120001), [1900, 2000, 6599.99, 6600, 6600.01, 6700]]))
+    ref = helland_linear(k)
+    worst = []
+    for c in range(3):
+        r = gpu.math_probe(BLACKBODY, k, float(c))
+        fast, exact = r[:, 0], r[:, 1]
+        # absolute on the linear [0, 1] scale; the exact policy's float32
+        # form of 138.52 ln(t - 10) - 305.04 cancels to ~1e-4 on the 0..255
+        # scale near t = 20, which the 2.2 power then shrinks
+        ef = np.abs(fast - ref[c]).max()
+        ee = np.abs(exact - ref[c]).max()
+        dfe = np.abs(fast - exact).max()
+        worst.append((ef, ee, dfe))
+        say("blackbody channel %d: fast %.3e, exact %.3e vs the fit; fast vs exact %.3e" % (c, ef, ee, dfe))
+        assert ef <= 4e-6 and ee <= 4e-6 and dfe <= 4e-6
+
+
+def test_fast_hot_spot_wrap_bound(gpu):
+    d = f32(np.concatenate([np.linspace(-200.0, 200.0, 400001), [-np.pi, np.pi, 0.0, 2 * np.pi, -2 * np.pi]]))
+    r = gpu.math_probe(WRAPPHI, d)
+    # both forms wrap e = d + pi as rounded in float (d + kPi in the kernel)
+    e = (d.astype(np.float32) + np.float32(np.pi)).astype(LD)
+    ref = np.mod(e, 2 * LD(np.float32(np.pi))) - LD(np.float32(np.pi))
+    e_fast = np.abs(r[:, 0].astype(LD) - ref).astype(np.float64)
+    e_ex = np.abs(r[:, 1].astype(LD) - ref).astype(np.float64)
+    # a value at the period's edge may land on either end: compare on the circle
+    e_fast = np.minimum(e_fast, np.abs(e_fast - 2 * np.pi))
+    e_ex = np.minimum(e_ex, np.abs(e_ex - 2 * np.pi))
+    say("hot-spot wrap |d| <= 200: fast %.3e, fmodf %.3e rad" % (e_fast.max(), e_ex.max()))
+    assert np.all((r[:, 0] >= -np.pi - 1e-6) & (r[:, 0] <= np.pi + 1e-6))
+    assert e_fast.max() <= 1e-6 and e_ex.max() <= 1e-6
