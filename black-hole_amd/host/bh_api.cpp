@@ -71,8 +71,15 @@ BandK makeBandK(const RenderParams& P, float critBand, float polarBand)
     const double rp = bhHorizonOuterFP64(B.M, B.a, B.Q);
     B.rScanLo = rp > 0.0 ? rp * 1.001 : (double)kSingCutOverM * B.M;
     B.rScanHi = std::max(5.0 * B.M, B.rScanLo * 1.5);
-    B.critBand = critBand;
-    B.polarBand = polarBand;
+    // The widths are those that hold every ray within 1e-3 rad of FP64 at
+    // steps dPhi <= 0.007 (the 'high' preset).  Coarser steps move FP32
+    // traces further from FP64 (more of their adaptive-step decisions
+    // differ), so the bands widen with the step: the critical band by
+    // s = dPhi / 0.007, the polar band by s^2 (64 uneven 'balanced' views,
+    // dPhi 0.012: the unscaled bands leave 7 views over 1e-3 rad).
+    const double s = std::max(1.0, (double)P.dPhi / 0.007);
+    B.critBand = (double)critBand * s;
+    B.polarBand = (double)polarBand * s * s;
     // camera terms of the ray invariants (trace.cuh:213-247, 704-739), in
     // double from the float camera position, with contraction off (this
     // file is compiled with -ffp-contract=off)

@@ -398,6 +398,8 @@ void bhAutotuneClear(void);
  * (1 - b^2/b_c^2 for the spherical models), and Kerr rays whose polar
  * turning point has sin^2(theta) < polarBand, are traced in double
  * precision on a second stream while the FP32 kernel traces the rest.
+ * The widths apply to steps dPhi <= 0.007; a frame with a coarser step uses
+ * critBand * s and polarBand * s^2, s = dPhi / 0.007.
  * No counterpart in the reference (which is FP32 throughout). */
 typedef struct BhPrecision
 {

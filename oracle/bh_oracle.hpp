@@ -880,6 +880,10 @@ inline V3<Shade<T>> renderSampleHDR(const RenderParams& P, int px, int py, T jx,
 // the closed form 1 - b^2/b_c^2, b_c^2 = r_ph^2 / f(r_ph).
 struct BandCfg { bool on; double crit, polar; };
 
+// The widths cfg.crit / cfg.polar apply to steps dPhi <= 0.007; coarser
+// steps widen the critical band by s = dPhi / 0.007 and the polar band by s^2.
+inline double bandWidthScale(const RenderParams& Pr) { return std::max(1.0, (double)Pr.dPhi / 0.007); }
+
 struct BandInv { double E, Lz, carter; bool ok; };
 
 inline BandInv bandInvariants(const RenderParams& Pr, V3<double> o, V3<double> dir)
@@ -976,7 +980,8 @@ inline bool rayInBand(const RenderParams& Pr, V3<double> o, V3<double> dir, cons
         const double d = bandCriticalDelta(Pr, xi, eta), pol = bandPolarSin2(Pr.aSpin, xi, eta);
         if (deltaOut) *deltaOut = d;
         if (polarOut) *polarOut = pol;
-        return std::fabs(d) < cfg.crit || pol < cfg.polar;
+        const double sc = bandWidthScale(Pr);
+        return std::fabs(d) < cfg.crit * sc || pol < cfg.polar * sc * sc;
     }
     const double M = Pr.M, Q2 = (double)Pr.Qc * (double)Pr.Qc;
     const double disc = 9.0 * M * M - 8.0 * Q2;
@@ -996,7 +1001,7 @@ inline bool rayInBand(const RenderParams& Pr, V3<double> o, V3<double> dir, cons
     const double b2 = (r0 * pl) * (r0 * pl) / f0;
     const double d = 1.0 - b2 / bc2;
     if (deltaOut) *deltaOut = d;
-    return std::fabs(d) < cfg.crit;
+    return std::fabs(d) < cfg.crit * bandWidthScale(Pr);
 }
 
 // Camera ray of a pixel in double (trace.cuh:910-912).

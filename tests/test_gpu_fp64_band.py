@@ -200,3 +200,29 @@ def test_tile_order_from_previous_steps(gpu, oracle, band_on, name, frame, w, h)
     bo[~np.isfinite(gk[order])] = 255
     assert (np.diff(bo) >= -1).all()
     say(name, "tiles", tx * ty, "ordered by previous steps; first buckets", bo[:4], "last", bo[-4:])
+
+
+# Views of the 64-view uneven batch (seed 1, 'balanced' step dPhi = 0.012) on
+# which the band widths of the 'high' step left rays over 1e-3 rad; the bands
+# now widen with the step (critical by dPhi / 0.007, polar by its square).
+# Full size, against the library's full-frame FP64 trace (itself checked
+# against the oracle's FP64 trace above).
+@pytest.mark.parametrize("view", [21, 29, 42, 60])
+def test_balanced_step_views_within_1e3_of_fp64(gpu, band_on, view):
+    V = gpu.workload_views("uneven_views", 1, 64)[view]
+    P, _ = gpu.sanitized(V)
+    prod = gpu.trace_records(P)
+    nb, _ = gpu.last_band_stats()
+    p0, s0 = gpu.get_precision(), gpu.get_schedule()
+    try:
+        gpu.set_precision(band=1, crit=1e30, polar=1e30)
+        gpu.set_schedule(math=0)
+        ref = gpu.trace_records(P)
+    finally:
+        gpu.restore_precision(p0)
+        gpu.restore_schedule(s0)
+    _, st = gpu.exit_angle_diff(prod, ref)
+    say(f"view {view} {P.width}x{P.height} dPhi {P.dPhi:.3f}: {nb} band rays, max {st.max:.3e} rad, "
+        f"p99.9 {st.p999:.3e}, over 1e-3: {st.over1mrad}, outcome mismatches {st.outcomeMismatch}")
+    assert st.outcomeMismatch == 0
+    assert st.max <= 1e-3
