@@ -302,7 +302,9 @@ def other_configs(bt, lib, fp64):
     """The other workloads at their full sizes (not the headline metric):
     trace time and frame rate of one-frame launches, their exit error against
     a full-frame FP64 trace (when `fp64`), and the 64-view uneven batch
-    (views rendered as single-view pipelines on the library's worker streams)."""
+    (views rendered as single-view pipelines on the library's worker streams,
+    and in one persistent launch with the FP64 band pass, with every view's
+    exit error against its FP64 trace when `fp64`)."""
     import ctypes as C
 
     import numpy as np
@@ -357,41 +359,74 @@ def other_configs(bt, lib, fp64):
                               "view_latency_ms_p50": round(float(np.median(lat)), 2),
                               "view_latency_ms_max": round(float(max(lat)), 2),
                               "launch": "bhRenderBatch: single-view pipelines (FP64 band on) on 6 worker streams"}
-    # the same views in one persistent launch (every ray FP32: the
-    # reference's arithmetic), eager and as a replayed graph
-    p0 = bt.get_precision()
-    try:
-        bt.set_precision(band=0)
-        bt.check(B.render_persistent(), "bhRenderBatchPersistent")
-        bt.synchronize()
-        pe = None
-        for _ in range(3):
-            t0 = time.perf_counter()
+    # the same views in one persistent launch, eager and as a replayed
+    # graph: with the FP64 band pass (the library default: every view
+    # classified, the band rays of all views traced in one launch beside the
+    # FP32 trace) and, for reference, every ray in FP32 (band off)
+    def persistent(band):
+        p0 = bt.get_precision()
+        try:
+            bt.set_precision(band=band)
             bt.check(B.render_persistent(), "bhRenderBatchPersistent")
             bt.synchronize()
-            t = time.perf_counter() - t0
-            pe = t if pe is None or t < pe else pe
-        G = B.graph(autotune=1)
-        wg, tuned = G.info()
-        G.launch()
-        bt.synchronize()
-        reps = 5
-        t0 = time.perf_counter()
-        for _ in range(reps):
+            pe = None
+            for _ in range(3):
+                t0 = time.perf_counter()
+                bt.check(B.render_persistent(), "bhRenderBatchPersistent")
+                bt.synchronize()
+                t = time.perf_counter() - t0
+                pe = t if pe is None or t < pe else pe
+            G = B.graph(autotune=1)
+            wg, tuned = G.info()
             G.launch()
-        bt.synchronize()
-        tg = (time.perf_counter() - t0) / reps
-        glat = G.latency_ms()
-        G.close()
-    finally:
-        bt.restore_precision(p0)
-    out["uneven_views_64_persistent"] = {
-        "rays": rays, "eager_batch_ms": round(pe * 1e3, 2), "eager_mrays_s": round(rays / pe / 1e6, 2),
-        "graph_batch_ms": round(tg * 1e3, 2), "graph_mrays_s": round(rays / tg / 1e6, 2),
-        "trace_latency_ms_p50": round(float(np.median(glat)), 2), "trace_latency_ms_max": round(float(max(glat)), 2),
-        "trace_wg_size": wg, "autotune_ms_wg64_128_256": [round(x, 2) for x in tuned],
-        "launch": "bhRenderBatchPersistent / bhBatchGraph: one persistent trace over a device queue of (view, tile) "
-                  "items + batched shade/bloom/composite; every ray FP32 (band off)"}
+            bt.synchronize()
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                G.launch()
+            bt.synchronize()
+            tg = (time.perf_counter() - t0) / reps
+            glat = G.latency_ms()
+            G.close()
+        finally:
+            bt.restore_precision(p0)
+        return {"eager_batch_ms": round(pe * 1e3, 2), "eager_mrays_s": round(rays / pe / 1e6, 2),
+                "graph_batch_ms": round(tg * 1e3, 2), "graph_mrays_s": round(rays / tg / 1e6, 2),
+                "trace_latency_ms_p50": round(float(np.median(glat)), 2),
+                "trace_latency_ms_max": round(float(max(glat)), 2),
+                "trace_wg_size": wg, "autotune_ms_wg64_128_256": [round(x, 2) for x in tuned]}
+
+    pb = persistent(1)
+    progress(f"uneven_views_64 persistent (FP64 band): {pb['graph_mrays_s']} Mrays/s")
+    pf = persistent(0)
+    out["uneven_views_64_persistent"] = dict(
+        rays=rays, **pb,
+        launch="bhRenderBatchPersistent / bhBatchGraph: every view classified, then one persistent FP32 trace over "
+               "a device queue of (view, tile) items beside one FP64 trace of all views' band rays over a queue of "
+               "(view, 64-ray chunk) items; batched shade/bloom/composite",
+        fp32_only={k: pf[k] for k in ("eager_batch_ms", "eager_mrays_s", "graph_batch_ms", "graph_mrays_s")})
+    if fp64:
+        # exit directions of every view against the library's FP64 trace of
+        # the view (the batch's records equal the view's own band-on trace)
+        p0, s0 = bt.get_precision(), bt.get_schedule()
+        worst, p999, over, mism = 0.0, 0.0, 0, 0
+        for V in views:
+            Q, _ = bt.sanitized(V)
+            prod = bt.trace_records(Q)
+            try:
+                bt.set_precision(band=1, crit=1e30, polar=1e30)
+                bt.set_schedule(math=0)
+                ref = bt.trace_records(Q)
+            finally:
+                bt.restore_precision(p0)
+                bt.restore_schedule(s0)
+            _, st = bt.exit_angle_diff(prod, ref)
+            worst, p999 = max(worst, st.max), max(p999, st.p999)
+            over += st.over1mrad
+            mism += st.outcomeMismatch
+        out["uneven_views_64_persistent"]["fp64_error"] = {
+            "views": len(views), "max_rad": float("%.3e" % worst), "max_view_p999_rad": float("%.3e" % p999),
+            "over_1e-3": int(over), "outcome_mismatch": int(mism)}
     return out
 
 

@@ -314,6 +314,44 @@ hipError_t launchTileOrder(const unsigned* tileKey, int nTiles, unsigned* hist, 
     return hipGetLastError();
 }
 
+// Band pass of a persistent batch.  Before the views' classifications:
+// every list length to zero.  After them: each view's first 64-ray chunk
+// (an exclusive scan of the chunk counts, one thread: batches have tens to
+// hundreds of views), the number of chunks, per-view chunk completions and
+// the queue head to zero.
+__global__ void batchBandResetKernel(BatchBandArgs A)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A.nViews) A.counts[i] = 0u;
+}
+
+__global__ void batchBandScanKernel(BatchBandArgs A)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    unsigned base = 0u;
+    for (int v = 0; v < A.nViews; ++v)
+    {
+        A.chunkBase[v] = base;
+        A.bandDone[v] = 0u;
+        base += (A.counts[v] + (unsigned)(kBandChunkRays - 1)) / (unsigned)kBandChunkRays;
+    }
+    A.chunkBase[A.nViews] = base;
+    *A.queue = 0u;
+}
+
+hipError_t launchBatchBandReset(const BatchBandArgs& A, hipStream_t st)
+{
+    if (A.nViews <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batchBandResetKernel, dim3((A.nViews + 63) / 64), dim3(64), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t launchBatchBandScan(const BatchBandArgs& A, hipStream_t st)
+{
+    hipLaunchKernelGGL(batchBandScanKernel, dim3(1), dim3(64), 0, st, A);
+    return hipGetLastError();
+}
+
 hipError_t launchBandClassify(const BandArgs& A, hipStream_t st)
 {
     const int n = A.P.width * A.P.height;

@@ -100,6 +100,7 @@ struct ViewDesc
     float4* accum;
     float4* bloomA;
     float4* bloomB;
+    const unsigned char* bandMask;  // [W*H] FP64 band pixels of this view (skipped by the FP32 trace), or null
     int tilesX;             // ceil(W/8)
     int tileBase;           // first queue index of this view
     int tileCount;          // ceil(W/8) * ceil(H/8)
@@ -110,7 +111,7 @@ struct ViewDesc
 // The host stages the table and the kernels read it: one layout in both
 // compilation passes (the offsets the batch kernels' scalar loads use).
 static_assert(sizeof(RenderParams) == 160, "RenderParams layout");
-static_assert(offsetof(ViewDesc, rec) == 160 + sizeof(FrameK) && offsetof(ViewDesc, tilesX) == offsetof(ViewDesc, rec) + 40,
+static_assert(offsetof(ViewDesc, rec) == 160 + sizeof(FrameK) && offsetof(ViewDesc, tilesX) == offsetof(ViewDesc, rec) + 48,
               "ViewDesc layout");
 static_assert(offsetof(ViewDesc, tileBase) == offsetof(ViewDesc, tilesX) + 4 &&
                   offsetof(ViewDesc, tileCount) == offsetof(ViewDesc, tilesX) + 8 &&
@@ -124,8 +125,25 @@ struct BatchArgs
     int totalTiles;
     unsigned* counter;      // tile queue head
     unsigned* viewDone;     // [nViews] finished tiles per view
-    unsigned long long* stamps;   // [nViews + 1]: completion per view, [nViews] = start
+    unsigned long long* stamps;   // [nViews + 1]: completion per view (the later of its FP32 tiles and its
+                                  // FP64 band rays), [nViews] = start
 };
+
+// FP64 band pass of a persistent batch: every view is classified by its own
+// launch (BandArgs by value, as for a single view), then one launch traces
+// the band rays of all views from a queue of 64-ray chunks of the views'
+// lists, beside the FP32 batch trace.
+struct BatchBandArgs
+{
+    const BandArgs* views;      // [nViews] device table: each view's FP64 constants, records, mask, list, count
+    int nViews;
+    unsigned* counts;           // [nViews] band list lengths (each view's BandArgs::count points here)
+    unsigned* chunkBase;        // [nViews + 1] first chunk of each view's list; [nViews]: number of chunks
+    unsigned* bandDone;         // [nViews] chunks finished per view
+    unsigned* queue;            // chunk queue head
+    unsigned long long* stamps; // [nViews] completion per view (shared with the FP32 batch trace)
+};
+constexpr int kBandChunkRays = 64;
 
 // Surrogate training set: rays of many configurations traced by the plain
 // FP32 Kerr integrator (fast policy), labelled for the network.
@@ -172,6 +190,12 @@ hipError_t launchBandTraceFast(const BandArgs& A, int blocks, hipStream_t st);
 hipError_t launchProbeExact(const ProbeArgs& A, hipStream_t st);
 hipError_t launchProbeFast(const ProbeArgs& A, hipStream_t st);
 hipError_t launchTraceBatchExact(const BatchArgs& A, const LaunchShape& S, hipStream_t st);
+// band pass of a persistent batch: list lengths zeroed (before the views'
+// classifications), chunk offsets and queue set (after them), the traces
+hipError_t launchBatchBandReset(const BatchBandArgs& A, hipStream_t st);
+hipError_t launchBatchBandScan(const BatchBandArgs& A, hipStream_t st);
+hipError_t launchBandTraceBatchExact(const BatchBandArgs& A, int blocks, hipStream_t st);
+hipError_t launchBandTraceBatchFast(const BatchBandArgs& A, int blocks, hipStream_t st);
 hipError_t launchTraceBatchFast(const BatchArgs& A, const LaunchShape& S, hipStream_t st);
 int traceBatchOccupancyExact(int wgSize);
 int traceBatchOccupancyFast(int wgSize);

@@ -214,7 +214,8 @@ __global__ __launch_bounds__(256) void traceBatchKernel(const ViewDesc* __restri
                 // all of this wavefront's record stores are issued before the count
                 __threadfence();
                 const unsigned d = atomicAdd(&A.viewDone[prev], 1u) + 1u;
-                if (d == (unsigned)views[prev].tileCount) A.stamps[prev] = (unsigned long long)wall_clock64();
+                // the view's FP64 band rays may finish later: the later stamp stays
+                if (d == (unsigned)views[prev].tileCount) atomicMax(&A.stamps[prev], (unsigned long long)wall_clock64());
             }
             t = atomicAdd(A.counter, 1u);
         }
@@ -231,8 +232,8 @@ __global__ __launch_bounds__(256) void traceBatchKernel(const ViewDesc* __restri
         const ViewDesc& V = views[v];
         const unsigned tile = t - (unsigned)V.tileBase;
         if (V.P.model == BH_KERR || V.P.model == BH_KERR_NEWMAN)
-            traceTile<KRay, false>(V.P, V.K, V.rec, V.tilesX, V.jitter, tile, lane);
-        else traceTile<SRay, false>(V.P, V.K, V.rec, V.tilesX, V.jitter, tile, lane);
+            traceTile<KRay, false>(V.P, V.K, V.rec, V.tilesX, V.jitter, tile, lane, V.bandMask);
+        else traceTile<SRay, false>(V.P, V.K, V.rec, V.tilesX, V.jitter, tile, lane, V.bandMask);
         prev = v;
     }
 }
@@ -324,6 +325,83 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void tr
             const unsigned long long idle = __ballot(!active && !drained);
             if (__popcll(idle) >= kBandRefill || __ballot(active) == 0ull) break;
         }
+    }
+}
+
+// ---------------------------------------------------------------- FP64 band rays of a batch
+// One ray of a view's band list (entry j, when the list has one), traced to
+// completion: the same integration and record as traceBandKernel's.
+template <class Ray> BHK_INL void traceBandListRay(const BandArgs& V, unsigned j)
+{
+    if (j >= *V.count) return;
+    const unsigned pix = V.list[j];
+    const int px = (int)(pix % (unsigned)V.P.width), py = (int)(pix / (unsigned)V.P.width);
+    const d3 o = dk3(V.P.camPos.x, V.P.camPos.y, V.P.camPos.z);
+    const float2 jit = launchJitter(V.P, V.jitter, V.dynIndex);
+    Ray R;
+    R.init(V.B, o, cameraDir64(V.P, px, py, jit));
+    Chord ch;
+    ch.n = 0;
+    bool done = (R.status != RAY_RUNNING);
+    while (!done)
+    {
+        done = R.advance(V.B, V.D, ch);
+        if (ch.n > 0) diskChordSerial<Pol>(V.D, R.diskE(), R.diskLz(), ch, R.acc, R.trans);
+    }
+    float4* out = reinterpret_cast<float4*>(V.rec + pix);
+    const int flags = (R.status == RAY_ESCAPED ? 1 : 2) | (R.step << 8);
+    out[0] = make_float4((float)R.escDir.x, (float)R.escDir.y, (float)R.escDir.z, __int_as_float(flags));
+    out[1] = make_float4(R.acc.x, R.acc.y, R.acc.z, R.trans);
+}
+
+// Wavefronts take 64-ray chunks of the views' band lists from one queue; a
+// chunk's view is wavefront-uniform, so its constants come through scalar
+// loads, and each lane traces one ray of the chunk.  The loop has the shape
+// of traceBatchKernel's: one lane-0 region per iteration, at its top and
+// before the cross-lane read of the queue item (the completion count of the
+// chunk finished last, then the next chunk), and every wavefront leaves
+// after chunks + 1 items at most, a bound each lane counts itself.  Band
+// wavefronts run beside the FP32 batch trace at raised issue priority, as
+// traceBandKernel's do, and at most 256 VGPRs.
+template <class SRay, class KRay>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void traceBandBatchKernel(BatchBandArgs A)
+{
+    __builtin_amdgcn_s_setprio(3);
+    const unsigned lane = __lane_id();
+    const unsigned total = A.chunkBase[A.nViews];
+    int prev = -1;      // view of the chunk this wavefront finished last (wavefront-uniform)
+    for (unsigned it = 0; it <= total; ++it)
+    {
+        unsigned c = 0;
+        if (lane == 0)
+        {
+            if (prev >= 0)
+            {
+                // all of this wavefront's record stores are issued before the count
+                __threadfence();
+                const unsigned d = atomicAdd(&A.bandDone[prev], 1u) + 1u;
+                if (d == A.chunkBase[prev + 1] - A.chunkBase[prev])
+                    atomicMax(&A.stamps[prev], (unsigned long long)wall_clock64());
+            }
+            c = atomicAdd(A.queue, 1u);
+        }
+        c = (unsigned)__builtin_amdgcn_readfirstlane(__shfl((int)c, 0));
+        if (c >= total) break;
+        // the chunk's view: the last with chunkBase <= c (views without band
+        // rays share the next view's base and come before it)
+        int lo = 0, hi = A.nViews - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (A.chunkBase[mid] <= c) lo = mid;
+            else hi = mid - 1;
+        }
+        const int v = __builtin_amdgcn_readfirstlane(lo);
+        const BandArgs& V = A.views[v];
+        const unsigned j = (c - A.chunkBase[v]) * (unsigned)kBandChunkRays + lane;
+        if (V.B.kerr) traceBandListRay<KRay>(V, j);
+        else traceBandListRay<SRay>(V, j);
+        prev = v;
     }
 }
 
@@ -529,6 +607,13 @@ hipError_t BHK_SUFFIX(launchBandTrace)(const BandArgs& A, int blocks, hipStream_
     using SR = SphRay64<Pol, D>;
     if (A.B.kerr) hipLaunchKernelGGL(traceBandKernel<KR>, dim3(blocks), dim3(64), 0, st, A);
     else hipLaunchKernelGGL(traceBandKernel<SR>, dim3(blocks), dim3(64), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t BHK_SUFFIX(launchBandTraceBatch)(const BatchBandArgs& A, int blocks, hipStream_t st)
+{
+    using D = typename std::conditional<Pol::fast, D64Fast, D64Exact>::type;
+    hipLaunchKernelGGL((traceBandBatchKernel<SphRay64<Pol, D>, KerrRay64<Pol, D>>), dim3(blocks), dim3(64), 0, st, A);
     return hipGetLastError();
 }
 

@@ -151,6 +151,53 @@ struct BatchSlot
     ViewMemo memo;
 };
 
+// FP64 band pass of a persistent batch: the per-view band descriptors
+// (device table, and its pinned host staging for the eager launch), every
+// view's band mask and list (slices of one buffer each, in view order), and
+// the pass's counters: list lengths [n], chunk offsets [n + 1], per-view
+// chunk completions [n], queue head [1].
+struct BatchBandBufs
+{
+    bhk::BandArgs* table = nullptr;
+    bhk::BandArgs* hostTable = nullptr;
+    unsigned char* mask = nullptr;
+    unsigned* list = nullptr;
+    unsigned* words = nullptr;
+    size_t pixCap = 0;
+    int viewCap = 0;
+};
+
+void freeBatchBand(BatchBandBufs& b)
+{
+    (void)hipFree(b.table);
+    (void)hipHostFree(b.hostTable);
+    (void)hipFree(b.mask);
+    (void)hipFree(b.list);
+    (void)hipFree(b.words);
+    b = BatchBandBufs{};
+}
+
+// Grows the buffers to n views and `pix` pixels (after `st` has finished
+// with the old ones); `pinned`: with a host staging table.
+hipError_t growBatchBand(BatchBandBufs& b, int n, size_t pix, bool pinned, hipStream_t st)
+{
+    if (n <= b.viewCap && pix <= b.pixCap && (!pinned || b.hostTable)) return hipSuccess;
+    hipError_t e = st ? hipStreamSynchronize(st) : hipSuccess;
+    if (e != hipSuccess) return e;
+    const int nv = std::max(n, b.viewCap);
+    const size_t np = std::max(std::max(pix, b.pixCap), (size_t)1);
+    freeBatchBand(b);
+    e = hipMalloc(&b.table, sizeof(bhk::BandArgs) * nv);
+    if (e == hipSuccess && pinned) e = hipHostMalloc(&b.hostTable, sizeof(bhk::BandArgs) * nv, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&b.mask, np);
+    if (e == hipSuccess) e = hipMalloc(&b.list, np * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&b.words, sizeof(unsigned) * (3 * (size_t)nv + 2));
+    if (e != hipSuccess) { freeBatchBand(b); return e; }
+    b.viewCap = nv;
+    b.pixCap = np;
+    return hipSuccess;
+}
+
 struct Scratch
 {
     BhRayRecord* rec = nullptr;
@@ -205,6 +252,7 @@ struct Scratch
     // shading
     hipStream_t aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    BatchBandBufs bband;         // band pass of persistent batches on this stream
 };
 
 std::mutex g_mu;
@@ -803,9 +851,10 @@ namespace {
 
 // The persistent multi-view launch: one trace kernel over a device queue of
 // (view, tile) items of every view (traceBatchKernel), then batched shade,
-// bloom and composite kernels over a device table of the views.  Every ray is
-// traced in FP32 (the reference's arithmetic; no FP64 band pass), so frames
-// equal launchRenderPipeline's with the band pass off.  (Its first GPU runs
+// bloom and composite kernels over a device table of the views.  With the
+// FP64 band pass on, every view is classified first and one kernel traces
+// the band rays of all views (traceBandBatchKernel) beside the FP32 trace;
+// frames equal launchRenderPipeline's under the same precision.  (Its first GPU runs
 // hung: the queue loop ended with a lane-0 block that the compiler merged
 // with the next iteration's queue atomic, see traceBatchKernel; the loop was
 // restructured.)
@@ -862,11 +911,61 @@ void assignBatchRecords(BatchTable& T, BhRayRecord* rec)
     }
 }
 
+// The FP64 band pass of a persistent batch, ready to enqueue: each view's
+// classification arguments (by value, as for a single view), the batched
+// band trace's arguments, and the auxiliary stream the FP32 batch trace
+// forks onto.
+struct BatchBandPlan
+{
+    std::vector<bhk::BandArgs> views;
+    bhk::BatchBandArgs A{};
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    int blocks = 1;
+};
+
+// Band descriptors of every view of T (records assigned) in the buffers b,
+// with the band widths of `pr` (makeBandK, as for a single view); sets each
+// view's band mask in T.  The device table b.table is not uploaded here.
+void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b, BatchBandPlan& plan)
+{
+    const int n = (int)T.D.size();
+    plan.views.assign((size_t)n, bhk::BandArgs{});
+    unsigned* w = b.words;
+    size_t off = 0;
+    for (int v = 0; v < n; ++v)
+    {
+        bhk::ViewDesc& d = T.D[(size_t)v];
+        bhk::BandArgs& a = plan.views[(size_t)v];
+        std::memset(&a, 0, sizeof(a));
+        a.P = d.P;
+        a.B = bhk::makeBandK(d.P, pr.critBand, pr.polarBand);
+        a.D = bhk::makeDisk(d.P, a.B.kerr ? d.P.aSpin : 0.f, d.P.Qc);
+        a.rec = d.rec;
+        a.mask = b.mask + off;
+        a.list = b.list + off;
+        a.count = w + v;
+        a.tilesX = d.tilesX;
+        a.jitter = d.jitter;
+        d.bandMask = a.mask;
+        off += (size_t)d.P.width * d.P.height;
+    }
+    plan.A.views = b.table;
+    plan.A.nViews = n;
+    plan.A.counts = w;
+    plan.A.chunkBase = w + n;
+    plan.A.bandDone = w + 2 * n + 1;
+    plan.A.queue = w + 3 * n + 1;
+}
+
 // The kernels of a persistent batch on `st`: counters reset, the trace over
 // one queue of (view, tile) items, shade + accumulation, bloom + composite.
-// No memset and no copy: capturable as a graph.
+// With `band`: every view classified, then the FP64 band rays of all views
+// traced on `st` beside the FP32 batch trace on the auxiliary stream (forked
+// after the classifications, joined before shading), as the single-view
+// pipeline does.  No memset and no copy: capturable as a graph.
 hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int wgSize, int math, hipStream_t st,
-                               Scratch* timers)
+                               Scratch* timers, const BatchBandPlan* band = nullptr)
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
@@ -878,7 +977,24 @@ hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int 
     if (L.blocks < 1) L.blocks = 1;
     if (timers) recordEvent(timers, 0, st);
     hipError_t e = bhk::launchResetBatch(A, st);
-    if (e == hipSuccess) e = math ? bhk::launchTraceBatchFast(A, L, st) : bhk::launchTraceBatchExact(A, L, st);
+    if (e != hipSuccess) return e;
+    if (band)
+    {
+        bhk::BatchBandArgs B = band->A;
+        B.stamps = A.stamps;
+        e = bhk::launchBatchBandReset(B, st);
+        for (size_t v = 0; v < band->views.size() && e == hipSuccess; ++v) e = bhk::launchBandClassify(band->views[v], st);
+        if (e == hipSuccess) e = bhk::launchBatchBandScan(B, st);
+        if (e == hipSuccess) e = hipEventRecord(band->fork, st);
+        if (e == hipSuccess)
+            e = math ? bhk::launchBandTraceBatchFast(B, band->blocks, st) : bhk::launchBandTraceBatchExact(B, band->blocks, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(band->aux, band->fork, 0);
+        if (e == hipSuccess)
+            e = math ? bhk::launchTraceBatchFast(A, L, band->aux) : bhk::launchTraceBatchExact(A, L, band->aux);
+        if (e == hipSuccess) e = hipEventRecord(band->join, band->aux);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, band->join, 0);
+    }
+    else e = math ? bhk::launchTraceBatchFast(A, L, st) : bhk::launchTraceBatchExact(A, L, st);
     if (e != hipSuccess) return e;
     if (timers) recordEvent(timers, 1, st);
     e = bhk::launchShadeBlendBatch(A.views, A.nViews, maxW, maxH, st);
@@ -903,13 +1019,40 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
     e = getBatchScratch(S, n, st);
     if (e != hipSuccess) return e;
     assignBatchRecords(T, S->rec);
-    // stage the table in pinned memory (after the previous upload from it
-    // has completed) and copy it asynchronously on the stream
+    const BhPrecision pr = currentPrecision();
+    const bool band = pr.fp64Band != 0;
+    BatchBandPlan plan;
+    if (band)
+    {
+        e = growBatchBand(S->bband, n, T.totalPix, true, st);
+        if (e == hipSuccess && !S->aux)
+        {
+            e = hipStreamCreateWithFlags(&S->aux, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&S->fork, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&S->join, hipEventDisableTiming);
+        }
+        if (e != hipSuccess) return e;
+        planBatchBand(T, pr, S->bband, plan);
+        plan.aux = S->aux;
+        plan.fork = S->fork;
+        plan.join = S->join;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        plan.blocks = 4 * numCUs(dev);
+    }
+    // stage the tables in pinned memory (after the previous uploads from
+    // them have completed) and copy them asynchronously on the stream
     e = hipEventSynchronize(S->staged);
     if (e != hipSuccess) return e;
     std::memcpy(S->hostViews, T.D.data(), sizeof(bhk::ViewDesc) * n);
     e = hipMemcpyAsync(S->views, S->hostViews, sizeof(bhk::ViewDesc) * n, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return e;
+    if (band)
+    {
+        std::memcpy(S->bband.hostTable, plan.views.data(), sizeof(bhk::BandArgs) * n);
+        e = hipMemcpyAsync(S->bband.table, S->bband.hostTable, sizeof(bhk::BandArgs) * n, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+    }
     e = hipEventRecord(S->staged, st);
     if (e != hipSuccess) return e;
     bhk::BatchArgs A;
@@ -923,7 +1066,7 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
     S->lastPersistent = true;
     const BhSchedule sc = currentSchedule();
     const bool timed = g_timers.load() != 0;
-    e = enqueueBatchKernels(A, T.maxW, T.maxH, sc.wgSize, sc.mathMode, st, timed ? S : nullptr);
+    e = enqueueBatchKernels(A, T.maxW, T.maxH, sc.wgSize, sc.mathMode, st, timed ? S : nullptr, band ? &plan : nullptr);
     if (e != hipSuccess) return e;
     if (timed)
     {
@@ -1112,6 +1255,7 @@ void freeScratch(Scratch& S)
     }
     if (S.fork) (void)hipEventDestroy(S.fork);
     if (S.join) (void)hipEventDestroy(S.join);
+    freeBatchBand(S.bband);
 }
 
 // Releases what the library holds for `st` on the current device: its
@@ -1179,6 +1323,9 @@ struct BhBatchGraph
     hipGraphExec_t exec = nullptr;
     int wgSize = 64;
     float tunedMs[3] = {0.f, 0.f, 0.f}; // trace+shade+post of one launch at workgroup 64, 128, 256 (0: not timed)
+    bool band = false;                  // FP64 band pass (the precision at creation)
+    BatchBandBufs bband;
+    BatchBandPlan plan;                 // its aux stream and fork/join events are the graph's own
 };
 
 struct BhAccumulator
@@ -1300,6 +1447,14 @@ void bhBatchGraphDestroy(BhBatchGraph* g)
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->cs) (void)hipStreamDestroy(g->cs);
+    if (g->plan.aux)
+    {
+        (void)hipStreamSynchronize(g->plan.aux);
+        (void)hipStreamDestroy(g->plan.aux);
+    }
+    if (g->plan.fork) (void)hipEventDestroy(g->plan.fork);
+    if (g->plan.join) (void)hipEventDestroy(g->plan.join);
+    freeBatchBand(g->bband);
     (void)hipFree(g->table);
     (void)hipFree(g->rec);
     (void)hipFree(g->counter);
@@ -1330,10 +1485,28 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
     if (e == hipSuccess) e = hipMalloc(&g->viewDone, sizeof(unsigned) * nViews);
     if (e == hipSuccess) e = hipMalloc(&g->stamps, sizeof(unsigned long long) * (nViews + 1));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->cs, hipStreamNonBlocking);
+    const BhPrecision pr = currentPrecision();
+    g->band = pr.fp64Band != 0;
     if (e == hipSuccess)
     {
         assignBatchRecords(T, g->rec);
-        e = hipMemcpy(g->table, T.D.data(), sizeof(bhk::ViewDesc) * nViews, hipMemcpyHostToDevice);
+        if (g->band)
+        {
+            e = growBatchBand(g->bband, nViews, T.totalPix, false, nullptr);
+            if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->plan.aux, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&g->plan.fork, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&g->plan.join, hipEventDisableTiming);
+            if (e == hipSuccess)
+            {
+                planBatchBand(T, pr, g->bband, g->plan);
+                int dev = 0;
+                if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+                g->plan.blocks = 4 * numCUs(dev);
+                e = hipMemcpy(g->bband.table, g->plan.views.data(), sizeof(bhk::BandArgs) * nViews,
+                              hipMemcpyHostToDevice);
+            }
+        }
+        if (e == hipSuccess) e = hipMemcpy(g->table, T.D.data(), sizeof(bhk::ViewDesc) * nViews, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) { bhBatchGraphDestroy(g); return e; }
     bhk::BatchArgs A;
@@ -1359,7 +1532,9 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
             for (int r = 0; r < 2 && e == hipSuccess; ++r)
             {
                 e = hipEventRecord(t0, g->cs);
-                if (e == hipSuccess) e = enqueueBatchKernels(A, g->maxW, g->maxH, wgs[w], sc.mathMode, g->cs, nullptr);
+                if (e == hipSuccess)
+                    e = enqueueBatchKernels(A, g->maxW, g->maxH, wgs[w], sc.mathMode, g->cs, nullptr,
+                                            g->band ? &g->plan : nullptr);
                 if (e == hipSuccess) e = hipEventRecord(t1, g->cs);
                 if (e == hipSuccess) e = hipEventSynchronize(t1);
             }
@@ -1377,7 +1552,8 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
     }
     e = hipStreamBeginCapture(g->cs, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) { bhBatchGraphDestroy(g); return e; }
-    hipError_t le = enqueueBatchKernels(A, g->maxW, g->maxH, g->wgSize, sc.mathMode, g->cs, nullptr);
+    hipError_t le = enqueueBatchKernels(A, g->maxW, g->maxH, g->wgSize, sc.mathMode, g->cs, nullptr,
+                                        g->band ? &g->plan : nullptr);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(g->cs, &graph);
     if (le != hipSuccess || e != hipSuccess)

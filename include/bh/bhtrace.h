@@ -273,10 +273,15 @@ bh_error_t bhRenderBatch(int nViews, const RenderParams* views, bh_uchar4* const
  * items over all views, traced by one kernel (wavefronts take the next item
  * as they finish; the view index is uniform per wavefront), then batched
  * shade, bloom and composite kernels over a device table of the views, all
- * on `stream`.  Every ray is traced in FP32 (the reference's arithmetic; the
- * FP64 band pass does not apply), so each view's frame equals
- * launchRenderPipeline's with the band pass off, bit for bit.  Per-view
- * completion times are read with bhBatchViewLatency.  No reference
+ * on `stream`.  With the FP64 band pass on (bhSetPrecision), every view is
+ * first classified and the band rays of all views are traced in double by
+ * one launch over a queue of (view, 64-ray chunk) items, beside the FP32
+ * trace (which skips them) on a library stream forked from `stream` and
+ * joined back; with it off every ray is traced in FP32 (the reference's
+ * arithmetic).  Either way each view's frame equals launchRenderPipeline's
+ * under the same precision, bit for bit.  Per-view completion times (the
+ * later of a view's FP32 tiles and its band rays) are read with
+ * bhBatchViewLatency.  No reference
  * counterpart (the reference renders one view per launch,
  * black_hole_kernel.cu:108-134). */
 bh_error_t bhRenderBatchPersistent(int nViews, const RenderParams* views, bh_uchar4* const* outs,
@@ -284,9 +289,10 @@ bh_error_t bhRenderBatchPersistent(int nViews, const RenderParams* views, bh_uch
                                    bh_float4* const* bloomBs, bh_stream_t stream);
 /* A steady batch captured once and replayed: the persistent launch of
  * bhRenderBatchPersistent for fixed views and buffers, as a hipGraph of its
- * kernels (counters reset by a kernel; no memset or copy nodes).  Create
- * uploads the view table once, gives the batch its own records and
- * counters, and with `autotune` first times one eager launch per trace
+ * kernels (counters reset by a kernel; no memset or copy nodes), with the
+ * band pass when the precision at creation has it on.  Create uploads the
+ * view table once, gives the batch its own records, band buffers, auxiliary
+ * stream and counters, and with `autotune` first times one eager launch per trace
  * workgroup size (64, 128, 256; the frames do not depend on it) and
  * captures the fastest.  Launch replays the whole batch on `stream`;
  * Latency reads the per-view trace completion times of the last replay

@@ -5,8 +5,10 @@ bit for bit, and match the oracle.
 bhRenderBatch runs the views as single-view pipelines on worker streams
 forked from the caller's stream (views finish in any order).
 bhRenderBatchPersistent traces all views in one persistent launch over a
-device queue of (view, tile) items, in FP32; it is compared with
-launchRenderPipeline under the band-off precision.
+device queue of (view, tile) items; with the FP64 band pass on it first
+classifies every view and traces the band rays of all views in one launch
+over a queue of (view, 64-ray chunk) items beside it.  It is compared with
+launchRenderPipeline under the same precision, band on and band off.
 """
 import ctypes as C
 
@@ -45,26 +47,26 @@ def small_views(bt, n=12):
 HIP_ERROR_INVALID_VALUE = 1
 
 
-class _BandOff:
-    """The persistent launch traces every ray in FP32: compare it with
-    launchRenderPipeline under the same (band-off) precision."""
+class _Band:
+    """The FP64 band pass on or off (default widths) for the block."""
 
-    def __init__(self, gpu):
-        self.gpu = gpu
+    def __init__(self, gpu, on):
+        self.gpu, self.on = gpu, on
 
     def __enter__(self):
         self.p0 = self.gpu.get_precision()
-        self.gpu.set_precision(band=0)
+        self.gpu.set_precision(band=self.on)
 
     def __exit__(self, *exc):
         self.gpu.restore_precision(self.p0)
 
 
-def test_persistent_batch_equals_single_view_launches(gpu, oracle):
+@pytest.mark.parametrize("band", [0, 1])
+def test_persistent_batch_equals_single_view_launches(gpu, oracle, band):
     views = small_views(gpu)
-    say("persistent batch of", len(views), "small views")
+    say("persistent batch of", len(views), "small views, FP64 band", band)
     B = gpu.BatchBuffers(views)
-    with _BandOff(gpu):
+    with _Band(gpu, band):
         assert B.render_persistent() == 0
         gpu.synchronize()
         lat = B.latency_ms()
@@ -75,7 +77,8 @@ def test_persistent_batch_equals_single_view_launches(gpu, oracle):
             ref_rgba, ref_acc = gpu.render_pipeline(P)
             assert np.array_equal(rgba, ref_rgba), v
             assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32)), v
-    oracle.set_band(0)
+    p = gpu.get_precision()
+    oracle.set_band(band, p.critBand, p.polarBand)
     try:
         for v, P in enumerate(views):
             o_rgba, _ = oracle.render_pipeline(P)
@@ -86,13 +89,14 @@ def test_persistent_batch_equals_single_view_launches(gpu, oracle):
         oracle.set_band(p.fp64Band, p.critBand, p.polarBand)
 
 
-def test_persistent_uneven_64_view_batch(gpu):
+@pytest.mark.parametrize("band", [0, 1])
+def test_persistent_uneven_64_view_batch(gpu, band):
     """The seed-1 64-view batch in one persistent launch: every view equals
-    its own (band-off) launchRenderPipeline bit for bit, again on a second
-    launch into the same buffers."""
+    its own launchRenderPipeline under the same precision bit for bit, again
+    on a second launch into the same buffers."""
     views = gpu.workload_views("uneven_views", 1, 64)
     B = gpu.BatchBuffers(views)
-    with _BandOff(gpu):
+    with _Band(gpu, band):
         for rep in range(2):
             assert B.render_persistent() == 0
             gpu.synchronize()
@@ -107,13 +111,16 @@ def test_persistent_uneven_64_view_batch(gpu):
             assert np.array_equal(acc.view(np.uint32), ref_acc.view(np.uint32)), v
 
 
-def test_persistent_batch_graph_replays(gpu):
+@pytest.mark.parametrize("band", [1, 0])
+def test_persistent_batch_graph_replays(gpu, band):
     """The 64-view persistent launch captured as a graph (autotuned trace
     workgroup): three replays, with host synchronizations and unrelated eager
-    frames between them, each give the eager launch's frames bit for bit."""
+    frames between them, each give the eager launch's frames bit for bit.
+    With the FP64 band pass on (the default), whose classification and band
+    trace the graph holds beside the FP32 trace, and with it off."""
     views = gpu.workload_views("uneven_views", 1, 64)
     B = gpu.BatchBuffers(views)
-    with _BandOff(gpu):
+    with _Band(gpu, band):
         assert B.render_persistent() == 0
         gpu.synchronize()
         ref = [B.readback(v) for v in range(len(views))]
