@@ -220,8 +220,7 @@ BHK_INL int bandBucket(unsigned keyBits)
 // it.
 constexpr unsigned kBandSortBlocks = 64;
 
-__global__ __launch_bounds__(kBandBuckets) void bandHistKernel(const unsigned* count, const unsigned* key,
-                                                              unsigned* hist)
+__device__ void bandHist(const unsigned* count, const unsigned* key, unsigned* hist)
 {
     __shared__ unsigned h[kBandBuckets];
     const int t = threadIdx.x;
@@ -234,9 +233,14 @@ __global__ __launch_bounds__(kBandBuckets) void bandHistKernel(const unsigned* c
     if (h[t]) atomicAdd(hist + t, h[t]);
 }
 
-__global__ __launch_bounds__(kBandBuckets) void bandScatterKernel(const unsigned* count, const unsigned* list,
-                                                                 const unsigned* key, unsigned* hist,
-                                                                 unsigned* sorted)
+__global__ __launch_bounds__(kBandBuckets) void bandHistKernel(const unsigned* count, const unsigned* key,
+                                                              unsigned* hist)
+{
+    bandHist(count, key, hist);
+}
+
+__device__ void bandScatter(const unsigned* count, const unsigned* list, const unsigned* key, unsigned* hist,
+                            unsigned* sorted)
 {
     __shared__ unsigned h[kBandBuckets], base[kBandBuckets];
     const int t = threadIdx.x;
@@ -261,6 +265,43 @@ __global__ __launch_bounds__(kBandBuckets) void bandScatterKernel(const unsigned
         if (i < n) sorted[base[b] + rank] = list[i];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(kBandBuckets) void bandScatterKernel(const unsigned* count, const unsigned* list,
+                                                                 const unsigned* key, unsigned* hist,
+                                                                 unsigned* sorted)
+{
+    bandScatter(count, list, key, hist, sorted);
+}
+
+// The same sort for every view of a batch at once: blockIdx.y is the view,
+// whose list, keys, histogram and sorted list are its descriptor's.
+__global__ __launch_bounds__(kBandBuckets) void bandHistBatchKernel(const BandArgs* views)
+{
+    const BandArgs& A = views[blockIdx.y];
+    bandHist(A.count, A.listKey, A.listHist);
+}
+__global__ __launch_bounds__(kBandBuckets) void bandScanBatchKernel(const BandArgs* views)
+{
+    __shared__ unsigned sh[kBandBuckets];
+    unsigned* hist = views[blockIdx.y].listHist;
+    const int t = threadIdx.x;
+    const unsigned v = hist[t];
+    sh[t] = v;
+    __syncthreads();
+    for (int d = 1; d < kBandBuckets; d <<= 1)
+    {
+        const unsigned add = t >= d ? sh[t - d] : 0u;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    hist[kBandBuckets + t] = sh[t] - v;
+}
+__global__ __launch_bounds__(kBandBuckets) void bandScatterBatchKernel(const BandArgs* views)
+{
+    const BandArgs& A = views[blockIdx.y];
+    bandScatter(A.count, A.list, A.listKey, A.listHist, A.sorted);
 }
 
 hipError_t launchBandSort(const BandArgs& A, hipStream_t st)
@@ -323,6 +364,7 @@ __global__ void batchBandResetKernel(BatchBandArgs A)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < A.nViews) A.counts[i] = 0u;
+    if (A.hist && i < A.nViews * 2 * kBandBuckets) A.hist[i] = 0u;
 }
 
 __global__ void batchBandScanKernel(BatchBandArgs A)
@@ -342,7 +384,18 @@ __global__ void batchBandScanKernel(BatchBandArgs A)
 hipError_t launchBatchBandReset(const BatchBandArgs& A, hipStream_t st)
 {
     if (A.nViews <= 0) return hipSuccess;
-    hipLaunchKernelGGL(batchBandResetKernel, dim3((A.nViews + 63) / 64), dim3(64), 0, st, A);
+    const int n = A.hist ? A.nViews * 2 * kBandBuckets : A.nViews;
+    hipLaunchKernelGGL(batchBandResetKernel, dim3((n + 255) / 256), dim3(256), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t launchBatchBandSort(const BatchBandArgs& A, hipStream_t st)
+{
+    if (A.nViews <= 0 || !A.hist) return hipSuccess;
+    const dim3 g(kBandSortBlocks, (unsigned)A.nViews), g1(1, (unsigned)A.nViews), b(kBandBuckets);
+    hipLaunchKernelGGL(bandHistBatchKernel, g, b, 0, st, A.views);
+    hipLaunchKernelGGL(bandScanBatchKernel, g1, b, 0, st, A.views);
+    hipLaunchKernelGGL(bandScatterBatchKernel, g, b, 0, st, A.views);
     return hipGetLastError();
 }
 

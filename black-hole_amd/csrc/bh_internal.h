@@ -142,6 +142,8 @@ struct BatchBandArgs
     unsigned* bandDone;         // [nViews] chunks finished per view
     unsigned* queue;            // chunk queue head
     unsigned long long* stamps; // [nViews] completion per view (shared with the FP32 batch trace)
+    unsigned* hist;             // [nViews x 512] each view's band sort histogram (BandArgs::listHist), or null:
+                                // lists traced unsorted
 };
 constexpr int kBandChunkRays = 64;
 
@@ -194,6 +196,8 @@ hipError_t launchTraceBatchExact(const BatchArgs& A, const LaunchShape& S, hipSt
 // classifications), chunk offsets and queue set (after them), the traces
 hipError_t launchBatchBandReset(const BatchBandArgs& A, hipStream_t st);
 hipError_t launchBatchBandScan(const BatchBandArgs& A, hipStream_t st);
+// every view's band list sorted by its cost keys, longest rays first (with hist)
+hipError_t launchBatchBandSort(const BatchBandArgs& A, hipStream_t st);
 hipError_t launchBandTraceBatchExact(const BatchBandArgs& A, int blocks, hipStream_t st);
 hipError_t launchBandTraceBatchFast(const BatchBandArgs& A, int blocks, hipStream_t st);
 hipError_t launchTraceBatchFast(const BatchArgs& A, const LaunchShape& S, hipStream_t st);

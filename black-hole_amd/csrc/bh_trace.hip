@@ -329,12 +329,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void tr
 }
 
 // ---------------------------------------------------------------- FP64 band rays of a batch
-// One ray of a view's band list (entry j, when the list has one), traced to
-// completion: the same integration and record as traceBandKernel's.
+// One ray of a view's band list (entry j of the sorted list when there is
+// one, when the list has that many), traced to completion: the same
+// integration and record as traceBandKernel's.
 template <class Ray> BHK_INL void traceBandListRay(const BandArgs& V, unsigned j)
 {
     if (j >= *V.count) return;
-    const unsigned pix = V.list[j];
+    const unsigned pix = (V.sorted ? V.sorted : V.list)[j];
     const int px = (int)(pix % (unsigned)V.P.width), py = (int)(pix / (unsigned)V.P.width);
     const d3 o = dk3(V.P.camPos.x, V.P.camPos.y, V.P.camPos.z);
     const float2 jit = launchJitter(V.P, V.jitter, V.dynIndex);

@@ -133,6 +133,15 @@ namespace {
 // and 165 ms again on 12, whose 24 queues the command processor time-slices.
 constexpr int kBatchStreams = 6;
 
+// FP64 band wavefronts per CU of a persistent batch's band trace.  Each
+// holds two FP32 trace wavefronts' registers while it runs, so fewer, busier
+// band wavefronts cost the FP32 trace less, as long as the band kernel still
+// ends before it.  64 uneven views with sorted band lists: 91.5 ms at 3 per
+// CU, 89.9 ms at 2, 88.0 ms at 1, 151 ms at 1/2 (the band kernel then sets
+// the time).  Two leave the band kernel twice the throughput of one for
+// batches with more band rays.
+constexpr int kBatchBandWavesPerCU = 2;
+
 // The last view traced into a record buffer (RenderParams less the
 // per-frame sample index and disk time) and that buffer: a frame of the same
 // view orders its work by those records' step counts.
@@ -162,6 +171,9 @@ struct BatchBandBufs
     bhk::BandArgs* hostTable = nullptr;
     unsigned char* mask = nullptr;
     unsigned* list = nullptr;
+    unsigned* key = nullptr;      // cost key of each listed ray
+    unsigned* sorted = nullptr;   // each view's list, longest rays first
+    unsigned* hist = nullptr;     // [viewCap x 512] sort histograms
     unsigned* words = nullptr;
     size_t pixCap = 0;
     int viewCap = 0;
@@ -173,6 +185,9 @@ void freeBatchBand(BatchBandBufs& b)
     (void)hipHostFree(b.hostTable);
     (void)hipFree(b.mask);
     (void)hipFree(b.list);
+    (void)hipFree(b.key);
+    (void)hipFree(b.sorted);
+    (void)hipFree(b.hist);
     (void)hipFree(b.words);
     b = BatchBandBufs{};
 }
@@ -191,6 +206,9 @@ hipError_t growBatchBand(BatchBandBufs& b, int n, size_t pix, bool pinned, hipSt
     if (e == hipSuccess && pinned) e = hipHostMalloc(&b.hostTable, sizeof(bhk::BandArgs) * nv, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&b.mask, np);
     if (e == hipSuccess) e = hipMalloc(&b.list, np * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&b.key, np * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&b.sorted, np * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&b.hist, sizeof(unsigned) * 512 * (size_t)nv);
     if (e == hipSuccess) e = hipMalloc(&b.words, sizeof(unsigned) * (3 * (size_t)nv + 2));
     if (e != hipSuccess) { freeBatchBand(b); return e; }
     b.viewCap = nv;
@@ -253,6 +271,8 @@ struct Scratch
     hipStream_t aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     BatchBandBufs bband;         // band pass of persistent batches on this stream
+    std::vector<RenderParams> lastBatchViews;   // views of the last persistent batch (sample index, disk time 0)
+    const BhRayRecord* lastBatchRec = nullptr;  // and the records it traced into
 };
 
 std::mutex g_mu;
@@ -926,8 +946,11 @@ struct BatchBandPlan
 
 // Band descriptors of every view of T (records assigned) in the buffers b,
 // with the band widths of `pr` (makeBandK, as for a single view); sets each
-// view's band mask in T.  The device table b.table is not uploaded here.
-void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b, BatchBandPlan& plan)
+// view's band mask in T.  Each view's band list is sorted longest rays
+// first, by keys from the step counts its records hold when the batch
+// repeats (`repeat`), else from the band statistics.  The device table
+// b.table is not uploaded here.
+void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b, bool repeat, BatchBandPlan& plan)
 {
     const int n = (int)T.D.size();
     plan.views.assign((size_t)n, bhk::BandArgs{});
@@ -944,6 +967,10 @@ void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b,
         a.rec = d.rec;
         a.mask = b.mask + off;
         a.list = b.list + off;
+        a.listKey = b.key + off;
+        a.sorted = b.sorted + off;
+        a.listHist = b.hist + (size_t)512 * v;
+        a.prevRec = repeat ? d.rec : nullptr;
         a.count = w + v;
         a.tilesX = d.tilesX;
         a.jitter = d.jitter;
@@ -956,6 +983,7 @@ void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b,
     plan.A.chunkBase = w + n;
     plan.A.bandDone = w + 2 * n + 1;
     plan.A.queue = w + 3 * n + 1;
+    plan.A.hist = b.hist;
 }
 
 // The kernels of a persistent batch on `st`: counters reset, the trace over
@@ -984,6 +1012,7 @@ hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int 
         B.stamps = A.stamps;
         e = bhk::launchBatchBandReset(B, st);
         for (size_t v = 0; v < band->views.size() && e == hipSuccess; ++v) e = bhk::launchBandClassify(band->views[v], st);
+        if (e == hipSuccess) e = bhk::launchBatchBandSort(B, st);
         if (e == hipSuccess) e = bhk::launchBatchBandScan(B, st);
         if (e == hipSuccess) e = hipEventRecord(band->fork, st);
         if (e == hipSuccess)
@@ -1032,13 +1061,27 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
             if (e == hipSuccess) e = hipEventCreateWithFlags(&S->join, hipEventDisableTiming);
         }
         if (e != hipSuccess) return e;
-        planBatchBand(T, pr, S->bband, plan);
+        // the same views as the last persistent batch on this stream (apart
+        // from sample index and disk time), into the same records: their
+        // step counts order the band rays
+        std::vector<RenderParams> key(T.D.size());
+        for (size_t v = 0; v < T.D.size(); ++v)
+        {
+            key[v] = T.D[v].P;
+            key[v].sampleIndex = 0;
+            key[v].diskTime = 0.f;
+        }
+        const bool repeat = S->lastBatchRec == S->rec && S->lastBatchViews.size() == key.size() &&
+                            std::memcmp(S->lastBatchViews.data(), key.data(), sizeof(RenderParams) * key.size()) == 0;
+        S->lastBatchViews = key;
+        S->lastBatchRec = S->rec;
+        planBatchBand(T, pr, S->bband, repeat, plan);
         plan.aux = S->aux;
         plan.fork = S->fork;
         plan.join = S->join;
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-        plan.blocks = 4 * numCUs(dev);
+        plan.blocks = kBatchBandWavesPerCU * numCUs(dev);
     }
     // stage the tables in pinned memory (after the previous uploads from
     // them have completed) and copy them asynchronously on the stream
@@ -1498,10 +1541,12 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
             if (e == hipSuccess) e = hipEventCreateWithFlags(&g->plan.join, hipEventDisableTiming);
             if (e == hipSuccess)
             {
-                planBatchBand(T, pr, g->bband, g->plan);
+                // replays repeat the batch: band rays ordered by the step
+                // counts of the previous launch's records
+                planBatchBand(T, pr, g->bband, true, g->plan);
                 int dev = 0;
                 if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-                g->plan.blocks = 4 * numCUs(dev);
+                g->plan.blocks = kBatchBandWavesPerCU * numCUs(dev);
                 e = hipMemcpy(g->bband.table, g->plan.views.data(), sizeof(bhk::BandArgs) * nViews,
                               hipMemcpyHostToDevice);
             }
