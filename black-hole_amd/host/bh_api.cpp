@@ -728,11 +728,11 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     if (e != hipSuccess) return e;
     if (timed) (void)hipEventRecord(Sc->tileEv[0], Sc->aux);
     e = sc.mathMode ? bhk::launchTraceFast(A, S, Sc->aux) : bhk::launchTraceExact(A, S, Sc->aux);
-    if (e != hipSuccess) return e;
-    if (timed) (void)hipEventRecord(Sc->tileEv[1], Sc->aux);
-    e = hipEventRecord(Sc->join, Sc->aux);
-    if (e != hipSuccess) return e;
-    return hipStreamWaitEvent(st, Sc->join, 0);
+    if (timed && e == hipSuccess) (void)hipEventRecord(Sc->tileEv[1], Sc->aux);
+    // the forked auxiliary stream is joined back also when the launch failed
+    hipError_t j = hipEventRecord(Sc->join, Sc->aux);
+    if (j == hipSuccess) j = hipStreamWaitEvent(st, Sc->join, 0);
+    return e != hipSuccess ? e : j;
 }
 
 void recordEvent(Scratch* S, int i, hipStream_t st)
@@ -1019,9 +1019,15 @@ hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int 
             e = math ? bhk::launchBandTraceBatchFast(B, band->blocks, st) : bhk::launchBandTraceBatchExact(B, band->blocks, st);
         if (e == hipSuccess) e = hipStreamWaitEvent(band->aux, band->fork, 0);
         if (e == hipSuccess)
+        {
+            // once the auxiliary stream is forked it is joined back, also when
+            // the trace launch fails, so no work of this batch outlives the
+            // caller's stream order
             e = math ? bhk::launchTraceBatchFast(A, L, band->aux) : bhk::launchTraceBatchExact(A, L, band->aux);
-        if (e == hipSuccess) e = hipEventRecord(band->join, band->aux);
-        if (e == hipSuccess) e = hipStreamWaitEvent(st, band->join, 0);
+            hipError_t j = hipEventRecord(band->join, band->aux);
+            if (j == hipSuccess) j = hipStreamWaitEvent(st, band->join, 0);
+            if (e == hipSuccess) e = j;
+        }
     }
     else e = math ? bhk::launchTraceBatchFast(A, L, st) : bhk::launchTraceBatchExact(A, L, st);
     if (e != hipSuccess) return e;
