@@ -37,17 +37,17 @@ HBM_PEAK_GBS = 8000.0
 
 # Exact algorithmic flops per frame of WORKLOAD (ray integration and disk
 # radiative transfer of every pixel), and the share of the FP64 band pixels
-# (|delta| < 0.04 or sin^2 theta_min < 1e-3: 29,830 pixels, traced by the
-# FP64 band kernel, not the FP32 tile kernel), from
+# (the default band rule: 15,816 pixels, traced by the FP64 band kernel, not
+# the FP32 tile kernel), from
 #   python oracle/count_flops.py kerr09_1080p_high 0 1 2
 # (counted-float instantiation of the oracle; 1 flop per +,-,*,/,sqrt,
 # floor, transcendental).  Keyed by frame index (disk time 0, 0.5, 1).
 FLOPS = {0: 506225668732, 1: 506225457204, 2: 506228100901}
-BAND_FLOPS = {0: 14268228423, 1: 14270951097, 2: 14272393917}
+BAND_FLOPS = {0: 10000991730, 1: 10003259008, 2: 10004442809}
 # HBM bytes per launch of the FP32 tile kernel on WORKLOAD (FP64 band on):
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by a
 # plain-copy calibration (FETCH x2.0, WRITE x1.0); profiles/pmc_trace_kerr09_1080p_high.md
-TRAFFIC_BYTES = (7717 * 2 + 65010) * 1024
+TRAFFIC_BYTES = (7621 * 2 + 65446) * 1024
 FP32_NONPACKED_TFLOPS = 78.6
 # The surrogate network: 224 -> 128 -> 128 -> 128 -> 128 -> 16 (padded)
 # multiply-adds per row, 2 flops each; a training step is the forward pass
@@ -59,7 +59,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # The same counts for frame 0 of the other workloads: (every pixel, FP64 band
 # pixels' share), from `python oracle/count_flops.py <workload> 0`.
 OTHER_FLOPS = {"schw_256_balanced": (2202725945, 34857266),
-               "kn_m07_q04_2160p_high": (1845561825607, 76329881318)}
+               "kn_m07_q04_2160p_high": (1845561825607, 58876316351)}
 
 
 _T0 = time.perf_counter()

@@ -17,7 +17,9 @@
 //              is the closed form 1 - b^2/b_c^2.  |delta| < critBand.
 //   polar:     sin^2(theta_min) = 1 - u+, u+ the larger root of
 //              a^2 u^2 - (a^2 - eta - xi^2) u - eta = 0 (theta turning
-//              point, u = cos^2 theta).  Kerr models only.  < polarBand.
+//              point, u = cos^2 theta).  Kerr models only.  < polarBand
+//              for |delta| < 0.2 s, then < 0.3, 0.1, 0.03 polarBand from
+//              |delta| = 0.2 s, 0.5 s, s (s: the step scale of the widths).
 //
 // The FP64 integrators follow the reference's traceSpherical
 // (trace.cuh:466-611) and traceKerr (trace.cuh:684-890) operation for
@@ -127,6 +129,7 @@ struct BandK
     double bCrit2;           // spherical models: critical impact parameter squared
     double rScanLo, rScanHi; // Kerr models: radial range of the critical-curve scan
     double critBand, polarBand;
+    double polarTier[3];     // |delta| beyond which the polar band narrows to 0.3, 0.1, 0.03 of polarBand
     // the camera's Boyer-Lindquist position, basis and static-observer
     // metric terms (trace.cuh:213-247, 704-739): the same for every ray of
     // the frame, evaluated once on the host in double
@@ -327,13 +330,32 @@ BHK_INL bool bandStatsF32(const BandK& B, d3 dir, float& delta, float& polar)
     return isfinite(delta) && isfinite(polar);
 }
 
+// Width of the polar band for a ray at |delta| from the critical curve.
+// Near-pole rays lose accuracy in FP32 mostly when they also wind: away
+// from the critical curve only rays much closer to the axis need FP64.
+BHK_INL double polarBandAt(const BandK& B, double absDelta)
+{
+    if (absDelta < B.polarTier[0]) return B.polarBand;
+    if (absDelta < B.polarTier[1]) return 0.3 * B.polarBand;
+    if (absDelta < B.polarTier[2]) return 0.1 * B.polarBand;
+    return 0.03 * B.polarBand;
+}
+
+// The band decision from a ray's statistics (`ok`: bandStats' result).
+BHK_INL bool bandDecision(const BandK& B, bool ok, double delta, double polar)
+{
+    if (!ok) return true;
+    if (!B.kerr && delta == 1.0) return false;
+    const double ad = fabs(delta);
+    return ad < B.critBand || polar < polarBandAt(B, ad);
+}
+
 // true: trace this ray in FP64.
 BHK_INL bool inCriticalBand(const BandK& B, d3 o, d3 dir)
 {
     double delta, polar;
-    if (!bandStats(B, o, dir, delta, polar)) return true;
-    if (!B.kerr && delta == 1.0) return false;
-    return fabs(delta) < B.critBand || polar < B.polarBand;
+    const bool ok = bandStats(B, o, dir, delta, polar);
+    return bandDecision(B, ok, delta, polar);
 }
 
 // ------------------------------------------------------------------ spherical, FP64 (trace.cuh:466-611)

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
             }
         }
         const bool ok = exact ? bandStats(A.B, o, dir, delta, polar) : true;
-        band = !ok || (!(A.B.kerr == 0 && delta == 1.0) && (fabs(delta) < A.B.critBand || polar < A.B.polarBand));
+        band = bandDecision(A.B, ok, delta, polar);
         A.mask[i] = band ? 1 : 0;
         tile = (py >> 3) * A.tilesX + (px >> 3);
         // cost key of the ray (smaller: longer), reduced to the tile's

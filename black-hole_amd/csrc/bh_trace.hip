@@ -567,7 +567,7 @@ __global__ __launch_bounds__(256) void surrogateDatasetKernel(DatasetArgs A)
         double delta, polar;
         const BandK& Bk = A.bands[cfg];
         const bool ok = bandStats(Bk, dk3(o.x, o.y, o.z), dk3(dir.x, dir.y, dir.z), delta, polar);
-        t.band = (!ok || fabs(delta) < Bk.critBand || polar < Bk.polarBand) ? 1.f : 0.f;
+        t.band = bandDecision(Bk, ok, delta, polar) ? 1.f : 0.f;
     }
     t.escaped = (R.status == RAY_ESCAPED) ? 1.f : 0.f;
     t.disk = disk;

@@ -402,10 +402,11 @@ void bhAutotuneClear(void);
  * FP64 invariants: rays near the critical (photon-ring) curve, |delta| <
  * critBand with delta = min_r R(r)/(r^2+a^2)^2 of the radial potential
  * (1 - b^2/b_c^2 for the spherical models), and Kerr rays whose polar
- * turning point has sin^2(theta) < polarBand, are traced in double
- * precision on a second stream while the FP32 kernel traces the rest.
- * The widths apply to steps dPhi <= 0.007; a frame with a coarser step uses
- * critBand * s and polarBand * s^2, s = dPhi / 0.007.
+ * turning point has sin^2(theta) < polarBand (near the critical curve;
+ * 0.3, 0.1 and 0.03 of it from |delta| = 0.2 s, 0.5 s and s), are traced
+ * in double precision on a second stream while the FP32 kernel traces the
+ * rest.  The widths apply to steps dPhi <= 0.007; a frame with a coarser
+ * step uses critBand * s and polarBand * s^2, s = dPhi / 0.007.
  * No counterpart in the reference (which is FP32 throughout). */
 typedef struct BhPrecision
 {
@@ -427,8 +428,8 @@ bh_error_t bhLastTraceKernelMs(float* tileMs, float* bandMs, bh_stream_t stream)
  * band pass on, where the queue hands out the costliest tiles first):
  * per-tile cost keys, the smallest over the tile's FP32 rays of the float
  * bits of 64 / (steps + 1) from the previous frame's records when the frame
- * repeats the previous view into the same records, else of
- * min(|delta|, 10 polar) (all ones when the tile has no FP32 ray), and the
+ * repeats the previous view into the same records, else of |delta|
+ * (all ones when the tile has no FP32 ray), and the
  * queue order, `capacity` entries each (either may be NULL).  *nTiles
  * receives the tile count; hipErrorNotReady when the frame was not ordered.
  * Synchronizes the stream. */

@@ -980,8 +980,13 @@ inline bool rayInBand(const RenderParams& Pr, V3<double> o, V3<double> dir, cons
         const double d = bandCriticalDelta(Pr, xi, eta), pol = bandPolarSin2(Pr.aSpin, xi, eta);
         if (deltaOut) *deltaOut = d;
         if (polarOut) *polarOut = pol;
-        const double sc = bandWidthScale(Pr);
-        return std::fabs(d) < cfg.crit * sc || pol < cfg.polar * sc * sc;
+        const double sc = bandWidthScale(Pr), ad = std::fabs(d);
+        // the polar band narrows away from the critical curve: the full
+        // width below |delta| = 0.2 s, then 0.3, 0.1 and 0.03 of it from
+        // 0.2 s, 0.5 s and s
+        const double pw = cfg.polar * sc * sc;
+        const double polW = ad < 0.2 * sc ? pw : ad < 0.5 * sc ? 0.3 * pw : ad < 1.0 * sc ? 0.1 * pw : 0.03 * pw;
+        return ad < cfg.crit * sc || pol < polW;
     }
     const double M = Pr.M, Q2 = (double)Pr.Qc * (double)Pr.Qc;
     const double disc = 9.0 * M * M - 8.0 * Q2;

@@ -127,7 +127,7 @@ def test_full_frame_fp64_trace_matches_fp64_oracle(gpu, oracle, name, frame, w, 
 @pytest.mark.parametrize("name,frame,w,h", [("kerr09_1080p_high", 0, 480, 270), ("kn_m07_q04_2160p_high", 0, 333, 187)])
 def test_tile_order_keys_and_permutation(gpu, oracle, band_on, name, frame, w, h):
     """The tile queue of a band-on frame: each tile's key is the smallest
-    min(|delta|, 10 polar) over its FP32 rays (the oracle's FP64 band
+    |delta| over its FP32 rays (the oracle's FP64 band
     statistics, up to the FP32 estimate's error, below 2e-5 absolute, for
     rays far from the band), the order is a permutation of the tiles, and it
     visits the 256 log2-key buckets in ascending order.  (A frame of a view
@@ -145,7 +145,7 @@ def test_tile_order_keys_and_permutation(gpu, oracle, band_on, name, frame, w, h
     assert keys.size == tx * ty
     assert np.array_equal(np.sort(order), np.arange(tx * ty))
     m, d, pol = oracle.band_mask(P, band_on.critBand, band_on.polarBand)
-    k = np.where(m > 0, np.inf, np.minimum(np.abs(d), 10.0 * pol)).astype(np.float32)
+    k = np.where(m > 0, np.inf, np.abs(d)).astype(np.float32)
     pad = np.full((ty * 8, tx * 8), np.inf, np.float32)
     pad[:h, :w] = k
     want = pad.reshape(ty, 8, tx, 8).min(axis=(1, 3)).ravel()
