@@ -14,7 +14,9 @@
 //              R(r) = (r^2 + a^2 - a xi)^2 - Delta(r) (eta + (xi - a)^2),
 //              xi = Lz/E, eta = Carter/E^2 (Kerr-Newman radial potential;
 //              Bardeen 1973, de Vries 2000).  For the spherical models this
-//              is the closed form 1 - b^2/b_c^2.  |delta| < critBand.
+//              is the closed form 1 - b^2/b_c^2.  |delta| < critBand for
+//              delta < 0 (the ray turns and escapes), delta < critBand / 4
+//              for delta > 0 (it falls in).
 //   polar:     sin^2(theta_min) = 1 - u+, u+ the larger root of
 //              a^2 u^2 - (a^2 - eta - xi^2) u - eta = 0 (theta turning
 //              point, u = cos^2 theta).  Kerr models only.  < polarBand
@@ -129,6 +131,7 @@ struct BandK
     double bCrit2;           // spherical models: critical impact parameter squared
     double rScanLo, rScanHi; // Kerr models: radial range of the critical-curve scan
     double critBand, polarBand;
+    double critBandIn;       // critical band width for delta > 0 (rays that fall in)
     double polarTier[3];     // |delta| beyond which the polar band narrows to 0.3, 0.1, 0.03 of polarBand
     // the camera's Boyer-Lindquist position, basis and static-observer
     // metric terms (trace.cuh:213-247, 704-739): the same for every ray of
@@ -342,12 +345,15 @@ BHK_INL double polarBandAt(const BandK& B, double absDelta)
 }
 
 // The band decision from a ray's statistics (`ok`: bandStats' result).
+// A ray with delta > 0 falls in: it has no exit direction, and FP32 decides
+// its fate like FP64 unless it is very close to the critical curve, so its
+// critical band is narrower.
 BHK_INL bool bandDecision(const BandK& B, bool ok, double delta, double polar)
 {
     if (!ok) return true;
     if (!B.kerr && delta == 1.0) return false;
     const double ad = fabs(delta);
-    return ad < B.critBand || polar < polarBandAt(B, ad);
+    return ad < (delta > 0.0 ? B.critBandIn : B.critBand) || polar < polarBandAt(B, ad);
 }
 
 // true: trace this ray in FP64.

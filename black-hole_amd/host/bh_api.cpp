@@ -85,6 +85,10 @@ BandK makeBandK(const RenderParams& P, float critBand, float polarBand)
     // |delta| >= 0.2 s and sin^2 theta_min >= 0.3 polarBand, >= 0.5 s and
     // >= 0.1 polarBand, >= s and >= 0.03 polarBand trace in FP32 within
     // 3.5e-4 rad of FP64).
+    // Rays that fall in (delta > 0) carry no exit direction; in FP32 they
+    // fall in like in FP64 from delta = 0.005 s on (no outcome differs on
+    // those frames), so their critical band is a quarter as wide.
+    B.critBandIn = 0.25 * B.critBand;
     B.polarTier[0] = 0.2 * s;
     B.polarTier[1] = 0.5 * s;
     B.polarTier[2] = 1.0 * s;

@@ -399,9 +399,10 @@ void bhAutotuneClear(void);
 /* Precision: FP64 critical-band rays                                        */
 /* ------------------------------------------------------------------------ */
 /* With fp64Band set, every frame first classifies each pixel's ray from its
- * FP64 invariants: rays near the critical (photon-ring) curve, |delta| <
- * critBand with delta = min_r R(r)/(r^2+a^2)^2 of the radial potential
- * (1 - b^2/b_c^2 for the spherical models), and Kerr rays whose polar
+ * FP64 invariants: rays near the critical (photon-ring) curve, -critBand <
+ * delta < critBand / 4 with delta = min_r R(r)/(r^2+a^2)^2 of the radial
+ * potential (1 - b^2/b_c^2 for the spherical models; delta > 0: the ray
+ * falls in and has no exit direction), and Kerr rays whose polar
  * turning point has sin^2(theta) < polarBand (near the critical curve;
  * 0.3, 0.1 and 0.03 of it from |delta| = 0.2 s, 0.5 s and s), are traced
  * in double precision on a second stream while the FP32 kernel traces the

@@ -986,7 +986,8 @@ inline bool rayInBand(const RenderParams& Pr, V3<double> o, V3<double> dir, cons
         // 0.2 s, 0.5 s and s
         const double pw = cfg.polar * sc * sc;
         const double polW = ad < 0.2 * sc ? pw : ad < 0.5 * sc ? 0.3 * pw : ad < 1.0 * sc ? 0.1 * pw : 0.03 * pw;
-        return ad < cfg.crit * sc || pol < polW;
+        // rays that fall in (d > 0) have a critical band a quarter as wide
+        return ad < (d > 0.0 ? 0.25 : 1.0) * cfg.crit * sc || pol < polW;
     }
     const double M = Pr.M, Q2 = (double)Pr.Qc * (double)Pr.Qc;
     const double disc = 9.0 * M * M - 8.0 * Q2;
@@ -1006,7 +1007,7 @@ inline bool rayInBand(const RenderParams& Pr, V3<double> o, V3<double> dir, cons
     const double b2 = (r0 * pl) * (r0 * pl) / f0;
     const double d = 1.0 - b2 / bc2;
     if (deltaOut) *deltaOut = d;
-    return std::fabs(d) < cfg.crit * bandWidthScale(Pr);
+    return std::fabs(d) < (d > 0.0 ? 0.25 : 1.0) * cfg.crit * bandWidthScale(Pr);
 }
 
 // Camera ray of a pixel in double (trace.cuh:910-912).
