@@ -148,11 +148,12 @@ constexpr int kBatchStreams = 6;
 // FP64 band wavefronts per CU of a persistent batch's band trace.  Each
 // holds two FP32 trace wavefronts' registers while it runs, so fewer, busier
 // band wavefronts cost the FP32 trace less, as long as the band kernel still
-// ends before it.  64 uneven views with sorted band lists: 91.5 ms at 3 per
-// CU, 89.9 ms at 2, 88.0 ms at 1, 151 ms at 1/2 (the band kernel then sets
-// the time).  Two leave the band kernel twice the throughput of one for
-// batches with more band rays.
-constexpr int kBatchBandWavesPerCU = 2;
+// ends before it.  64 uneven views with sorted band lists and the uniform
+// band widths: 91.5 ms at 3 per CU, 89.9 ms at 2, 88.0 ms at 1, 151 ms at
+// 1/2 (the band kernel then sets the time).  With the narrowed polar and
+// infall bands (a third of the band rays): 87.2 ms at 2, 85.7 ms at 1, the
+// band kernel ending at about half of the FP32 trace.
+constexpr int kBatchBandWavesPerCU = 1;
 
 // The last view traced into a record buffer (RenderParams less the
 // per-frame sample index and disk time) and that buffer: a frame of the same
