@@ -37,13 +37,13 @@ HBM_PEAK_GBS = 8000.0
 
 # Exact algorithmic flops per frame of WORKLOAD (ray integration and disk
 # radiative transfer of every pixel), and the share of the FP64 band pixels
-# (the default band rule: 11,580 pixels, traced by the FP64 band kernel, not
+# (the default band rule: 8,336 pixels, traced by the FP64 band kernel, not
 # the FP32 tile kernel), from
 #   python oracle/count_flops.py kerr09_1080p_high 0 1 2
 # (counted-float instantiation of the oracle; 1 flop per +,-,*,/,sqrt,
 # floor, transcendental).  Keyed by frame index (disk time 0, 0.5, 1).
 FLOPS = {0: 506225668732, 1: 506225457204, 2: 506228100901}
-BAND_FLOPS = {0: 6742865591, 1: 6745196474, 2: 6746511972}
+BAND_FLOPS = {0: 4791139978, 1: 4793639269, 2: 4794055461}
 # HBM bytes per launch of the FP32 tile kernel on WORKLOAD (FP64 band on):
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by a
 # plain-copy calibration (FETCH x2.0, WRITE x1.0); profiles/pmc_trace_kerr09_1080p_high.md
@@ -59,7 +59,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # The same counts for frame 0 of the other workloads: (every pixel, FP64 band
 # pixels' share), from `python oracle/count_flops.py <workload> 0`.
 OTHER_FLOPS = {"schw_256_balanced": (2202725945, 23447943),
-               "kn_m07_q04_2160p_high": (1845561825607, 41849064827)}
+               "kn_m07_q04_2160p_high": (1845561825607, 31190190725)}
 
 
 _T0 = time.perf_counter()

@@ -15,8 +15,9 @@
 //              xi = Lz/E, eta = Carter/E^2 (Kerr-Newman radial potential;
 //              Bardeen 1973, de Vries 2000).  For the spherical models this
 //              is the closed form 1 - b^2/b_c^2.  |delta| < critBand for
-//              delta < 0 (the ray turns and escapes), delta < critBand / 4
-//              for delta > 0 (it falls in).
+//              delta < 0 (the ray turns and escapes; for Kerr rays half of
+//              it from sin^2 theta_min = 0.1 s^2 on, a quarter from 0.3 s^2),
+//              delta < critBand / 4 for delta > 0 (it falls in).
 //   polar:     sin^2(theta_min) = 1 - u+, u+ the larger root of
 //              a^2 u^2 - (a^2 - eta - xi^2) u - eta = 0 (theta turning
 //              point, u = cos^2 theta).  Kerr models only.  < polarBand
@@ -132,6 +133,7 @@ struct BandK
     double rScanLo, rScanHi; // Kerr models: radial range of the critical-curve scan
     double critBand, polarBand;
     double critBandIn;       // critical band width for delta > 0 (rays that fall in)
+    double critPolarTier[2]; // Kerr rays that escape: sin^2 theta_min beyond which the critical band halves, quarters
     double polarTier[3];     // |delta| beyond which the polar band narrows to 0.3, 0.1, 0.03 of polarBand
     // the camera's Boyer-Lindquist position, basis and static-observer
     // metric terms (trace.cuh:213-247, 704-739): the same for every ray of
@@ -344,16 +346,27 @@ BHK_INL double polarBandAt(const BandK& B, double absDelta)
     return 0.03 * B.polarBand;
 }
 
+// Width of the critical band for a ray with these statistics.  A ray with
+// delta > 0 falls in: it has no exit direction, and FP32 decides its fate
+// like FP64 unless it is very close to the critical curve.  Of the Kerr rays
+// that escape, those whose polar turning point keeps well away from the axis
+// lose less in FP32 while they wind.
+BHK_INL double critBandAt(const BandK& B, double delta, double polar)
+{
+    if (delta > 0.0) return B.critBandIn;
+    if (!B.kerr) return B.critBand;
+    if (polar >= B.critPolarTier[1]) return 0.25 * B.critBand;
+    if (polar >= B.critPolarTier[0]) return 0.5 * B.critBand;
+    return B.critBand;
+}
+
 // The band decision from a ray's statistics (`ok`: bandStats' result).
-// A ray with delta > 0 falls in: it has no exit direction, and FP32 decides
-// its fate like FP64 unless it is very close to the critical curve, so its
-// critical band is narrower.
 BHK_INL bool bandDecision(const BandK& B, bool ok, double delta, double polar)
 {
     if (!ok) return true;
     if (!B.kerr && delta == 1.0) return false;
     const double ad = fabs(delta);
-    return ad < (delta > 0.0 ? B.critBandIn : B.critBand) || polar < polarBandAt(B, ad);
+    return ad < critBandAt(B, delta, polar) || polar < polarBandAt(B, ad);
 }
 
 // true: trace this ray in FP64.

@@ -89,6 +89,11 @@ BandK makeBandK(const RenderParams& P, float critBand, float polarBand)
     // fall in like in FP64 from delta = 0.005 s on (no outcome differs on
     // those frames), so their critical band is a quarter as wide.
     B.critBandIn = 0.25 * B.critBand;
+    // Kerr rays that escape with their polar turning point far from the axis
+    // (sin^2 theta_min >= 0.1 s^2, >= 0.3 s^2) trace in FP32 within 2.9e-4
+    // rad of FP64 from |delta| = critBand / 2, critBand / 4 on.
+    B.critPolarTier[0] = 0.1 * s * s;
+    B.critPolarTier[1] = 0.3 * s * s;
     B.polarTier[0] = 0.2 * s;
     B.polarTier[1] = 0.5 * s;
     B.polarTier[2] = 1.0 * s;

@@ -402,7 +402,9 @@ void bhAutotuneClear(void);
  * FP64 invariants: rays near the critical (photon-ring) curve, -critBand <
  * delta < critBand / 4 with delta = min_r R(r)/(r^2+a^2)^2 of the radial
  * potential (1 - b^2/b_c^2 for the spherical models; delta > 0: the ray
- * falls in and has no exit direction), and Kerr rays whose polar
+ * falls in and has no exit direction; escaping Kerr rays with sin^2 of the
+ * polar turning angle >= 0.1 s^2 or 0.3 s^2 use critBand / 2 or / 4 on the
+ * escaping side), and Kerr rays whose polar
  * turning point has sin^2(theta) < polarBand (near the critical curve;
  * 0.3, 0.1 and 0.03 of it from |delta| = 0.2 s, 0.5 s and s), are traced
  * in double precision on a second stream while the FP32 kernel traces the

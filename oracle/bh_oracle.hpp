@@ -986,8 +986,11 @@ inline bool rayInBand(const RenderParams& Pr, V3<double> o, V3<double> dir, cons
         // 0.2 s, 0.5 s and s
         const double pw = cfg.polar * sc * sc;
         const double polW = ad < 0.2 * sc ? pw : ad < 0.5 * sc ? 0.3 * pw : ad < 1.0 * sc ? 0.1 * pw : 0.03 * pw;
-        // rays that fall in (d > 0) have a critical band a quarter as wide
-        return ad < (d > 0.0 ? 0.25 : 1.0) * cfg.crit * sc || pol < polW;
+        // rays that fall in (d > 0) have a critical band a quarter as wide;
+        // escaping rays whose polar turning point is far from the axis
+        // (sin^2 theta_min >= 0.1 s^2, >= 0.3 s^2) a half, a quarter
+        const double cw = d > 0.0 ? 0.25 : pol >= 0.3 * sc * sc ? 0.25 : pol >= 0.1 * sc * sc ? 0.5 : 1.0;
+        return ad < cw * cfg.crit * sc || pol < polW;
     }
     const double M = Pr.M, Q2 = (double)Pr.Qc * (double)Pr.Qc;
     const double disc = 9.0 * M * M - 8.0 * Q2;
