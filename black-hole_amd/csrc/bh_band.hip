@@ -12,10 +12,11 @@
 
 namespace bhk {
 
-__global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
+// Pixel i of A's view (every lane of the wavefront comes in; lanes past the
+// image take part in the wavefront's list appends only).
+BHK_INL void classifyPixel(const BandArgs& A, int i)
 {
     const int W = A.P.width, H = A.P.height;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool band = false, cand = false;
     float x[kMlpInputs];
     int tile = -1;               // tile of this pixel (-1: none)
@@ -123,6 +124,30 @@ __global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
         }
     }
 }
+
+__global__ __launch_bounds__(256) void bandClassifyKernel(BandArgs A)
+{
+    classifyPixel(A, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// All views of a batch in one launch: block b belongs to the last view whose
+// classifyBase <= b (views without pixels share the next view's base and
+// come before it), and covers 256 of its pixels.
+__global__ __launch_bounds__(256) void bandClassifyBatchKernel(const BandArgs* views, int nViews)
+{
+    const unsigned b = blockIdx.x;
+    int lo = 0, hi = nViews - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (views[mid].classifyBase <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const int v = __builtin_amdgcn_readfirstlane(lo);
+    const BandArgs& A = views[v];
+    classifyPixel(A, (int)(b - A.classifyBase) * 256 + (int)threadIdx.x);
+}
+
 
 // ---------------------------------------------------------------- tile order
 // The FP32 tile tracer takes tiles from a queue; a tile costs as many RK4
@@ -405,6 +430,13 @@ hipError_t launchBatchBandSort(const BatchBandArgs& A, hipStream_t st)
 hipError_t launchBatchBandScan(const BatchBandArgs& A, hipStream_t st)
 {
     hipLaunchKernelGGL(batchBandScanKernel, dim3(1), dim3(64), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t launchBandClassifyBatch(const BatchBandArgs& A, hipStream_t st)
+{
+    if (A.nViews <= 0 || A.classifyBlocks == 0u) return hipSuccess;
+    hipLaunchKernelGGL(bandClassifyBatchKernel, dim3(A.classifyBlocks), dim3(256), 0, st, A.views, A.nViews);
     return hipGetLastError();
 }
 

@@ -56,6 +56,7 @@ struct BandArgs
     unsigned* candList;     // [W*H]
     unsigned* candCount;
     float psiMax;
+    unsigned classifyBase;  // batch: first classification block of this view (blocks of 256 pixels)
 };
 
 // Surrogate inference and routing over the candidate list: accepted rays get
@@ -144,6 +145,7 @@ struct BatchBandArgs
     unsigned long long* stamps; // [nViews] completion per view (shared with the FP32 batch trace)
     unsigned* hist;             // [nViews x 512] each view's band sort histogram (BandArgs::listHist), or null:
                                 // lists traced unsorted
+    unsigned classifyBlocks;    // classification blocks of all views (BandArgs::classifyBase)
 };
 constexpr int kBandChunkRays = 64;
 
@@ -173,6 +175,8 @@ hipError_t launchTraceFast(const TraceArgs& A, const LaunchShape& S, hipStream_t
 int traceOccupancyExact(int kerr, int wgSize, int coop, int tiles);
 int traceOccupancyFast(int kerr, int wgSize, int coop, int tiles);
 hipError_t launchBandClassify(const BandArgs& A, hipStream_t st);
+// Every view of a batch classified in one launch (A.views: the device table).
+hipError_t launchBandClassifyBatch(const BatchBandArgs& A, hipStream_t st);
 hipError_t launchSurrogateDataset(const DatasetArgs& A, hipStream_t st);
 hipError_t launchSurrogateRoute(const RouteArgs& A, int maxRows, int numCU, hipStream_t st);
 

@@ -23,35 +23,10 @@ namespace bhk {
 
 __global__ void resetCounterKernel(unsigned* c) { c[0] = 0u; }
 
-BHK_INL v3 sampleFromRecord(const BhRayRecord* rec, int idx, float gCam)
-{
-    const float4* r = reinterpret_cast<const float4*>(rec + idx);
-    float4 a = r[0], b = r[1];
-    int flags = __float_as_int(a.w);
-    v3 bg = (flags & 1) ? backgroundColor(mk3(a.x, a.y, a.z), gCam) : mk3(0.f, 0.f, 0.f);
-    v3 hdr;
-    {
-#pragma clang fp contract(off)
-        hdr = mk3(b.x, b.y, b.z) + bg * b.w;
-    }
-    if (!finite3(hdr)) hdr = mk3(0.f, 0.f, 0.f);
-    return hdr;
-}
-
-BHK_INL void shadeBlendAt(const RenderParams& P, const BhRayRecord* rec, float4* accum, int idx, float gCam,
-                          int sampleIndex)
-{
-    v3 cur = sampleFromRecord(rec, idx, gCam);
-    float4 pv = accum[idx];
-    v3 prev = mk3(pv.x, pv.y, pv.z);
-    if (!finite3(prev)) prev = cur;
-    v3 c = accumBlend(prev, cur, P.accumMode, sampleIndex);
-    accum[idx] = make_float4(c.x, c.y, c.z, 1.f);
-}
-
 // HDR sample of a record for a whole wavefront (every lane calls it; lanes
 // past the image get an unused value): the sky through
-// backgroundColorWave, otherwise the same operations as sampleFromRecord.
+// backgroundColorWave (bit-identical to backgroundColor per lane), the
+// record's disk light plus the sky weighted by its transmittance.
 BHK_INL v3 sampleFromRecordWave(const BhRayRecord* rec, int idx, bool valid, float gCam)
 {
     float4 a = make_float4(0.f, 0.f, 1.f, 0.f), b = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -91,12 +66,23 @@ __global__ __launch_bounds__(256) void shadeBlendKernel(RenderParams P, const Bh
 __global__ void bumpIndexKernel(int* i) { i[0] += 1; }
 __global__ void setIndexKernel(int* i, int v) { i[0] = v; }
 
+// The batched views' shading, as shadeBlendKernel shades one view (the
+// wavefront's star hits shaded together): blocks past a view's pixels leave
+// whole, so every wavefront that enters has all its lanes.
 __global__ __launch_bounds__(256) void shadeBlendBatchKernel(const ViewDesc* views)
 {
     const ViewDesc& V = views[blockIdx.z];
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= V.P.width * V.P.height) return;
-    shadeBlendAt(V.P, V.rec, V.accum, idx, cameraEnergyFactor(V.P), V.P.sampleIndex);
+    const int n = V.P.width * V.P.height;
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = idx < n;
+    const v3 cur = sampleFromRecordWave(V.rec, idx, valid, cameraEnergyFactor(V.P));
+    if (!valid) return;
+    float4 pv = V.accum[idx];
+    v3 prev = mk3(pv.x, pv.y, pv.z);
+    if (!finite3(prev)) prev = cur;
+    v3 c = accumBlend(prev, cur, V.P.accumMode, V.P.sampleIndex);
+    V.accum[idx] = make_float4(c.x, c.y, c.z, 1.f);
 }
 
 __global__ __launch_bounds__(256) void shadeFinalizeKernel(RenderParams P, const BhRayRecord* rec, uchar4* out)

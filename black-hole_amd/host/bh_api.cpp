@@ -974,6 +974,7 @@ void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b,
     plan.views.assign((size_t)n, bhk::BandArgs{});
     unsigned* w = b.words;
     size_t off = 0;
+    unsigned blocks = 0;
     for (int v = 0; v < n; ++v)
     {
         bhk::ViewDesc& d = T.D[(size_t)v];
@@ -992,9 +993,13 @@ void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b,
         a.count = w + v;
         a.tilesX = d.tilesX;
         a.jitter = d.jitter;
+        a.classifyBase = blocks;
         d.bandMask = a.mask;
-        off += (size_t)d.P.width * d.P.height;
+        const size_t px = (size_t)d.P.width * d.P.height;
+        blocks += (unsigned)((px + 255) / 256);
+        off += px;
     }
+    plan.A.classifyBlocks = blocks;
     plan.A.views = b.table;
     plan.A.nViews = n;
     plan.A.counts = w;
@@ -1029,7 +1034,7 @@ hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int 
         bhk::BatchBandArgs B = band->A;
         B.stamps = A.stamps;
         e = bhk::launchBatchBandReset(B, st);
-        for (size_t v = 0; v < band->views.size() && e == hipSuccess; ++v) e = bhk::launchBandClassify(band->views[v], st);
+        if (e == hipSuccess) e = bhk::launchBandClassifyBatch(B, st);
         if (e == hipSuccess) e = bhk::launchBatchBandSort(B, st);
         if (e == hipSuccess) e = bhk::launchBatchBandScan(B, st);
         if (e == hipSuccess) e = hipEventRecord(band->fork, st);
