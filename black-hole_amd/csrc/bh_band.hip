@@ -393,6 +393,8 @@ __global__ void batchBandResetKernel(BatchBandArgs A)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < A.nViews) A.counts[i] = 0u;
     if (A.hist && i < A.nViews * 2 * kBandBuckets) A.hist[i] = 0u;
+    if (A.tileKey && i < A.nTiles) A.tileKey[i] = 0xFFFFFFFFu;
+    if (A.tileKey && i < 2 * kTileBuckets) A.tileHist[i] = 0u;
 }
 
 __global__ void batchBandScanKernel(BatchBandArgs A)
@@ -412,7 +414,12 @@ __global__ void batchBandScanKernel(BatchBandArgs A)
 hipError_t launchBatchBandReset(const BatchBandArgs& A, hipStream_t st)
 {
     if (A.nViews <= 0) return hipSuccess;
-    const int n = A.hist ? A.nViews * 2 * kBandBuckets : A.nViews;
+    int n = A.hist ? A.nViews * 2 * kBandBuckets : A.nViews;
+    if (A.tileKey)
+    {
+        n = A.nTiles > n ? A.nTiles : n;
+        n = 2 * kTileBuckets > n ? 2 * kTileBuckets : n;
+    }
     hipLaunchKernelGGL(batchBandResetKernel, dim3((n + 255) / 256), dim3(256), 0, st, A);
     return hipGetLastError();
 }

@@ -128,12 +128,13 @@ struct BatchArgs
     unsigned* viewDone;     // [nViews] finished tiles per view
     unsigned long long* stamps;   // [nViews + 1]: completion per view (the later of its FP32 tiles and its
                                   // FP64 band rays), [nViews] = start
+    const unsigned* order;  // [totalTiles] queue position -> batch tile, costliest first (or null: view order)
 };
 
-// FP64 band pass of a persistent batch: every view is classified by its own
-// launch (BandArgs by value, as for a single view), then one launch traces
-// the band rays of all views from a queue of 64-ray chunks of the views'
-// lists, beside the FP32 batch trace.
+// FP64 band pass of a persistent batch: every view is classified (one
+// launch over all views), then one launch traces the band rays of all views
+// from a queue of 64-ray chunks of the views' lists, beside the FP32 batch
+// trace, which takes the batch's tiles costliest first.
 struct BatchBandArgs
 {
     const BandArgs* views;      // [nViews] device table: each view's FP64 constants, records, mask, list, count
@@ -146,6 +147,10 @@ struct BatchBandArgs
     unsigned* hist;             // [nViews x 512] each view's band sort histogram (BandArgs::listHist), or null:
                                 // lists traced unsorted
     unsigned classifyBlocks;    // classification blocks of all views (BandArgs::classifyBase)
+    unsigned* tileKey;          // [nTiles] cost key of every tile of the batch (each view's BandArgs::tileKey
+                                // points at its first tile), or null: tiles in view order
+    int nTiles;
+    unsigned* tileHist;         // [512] the tile sort's bucket counts and cursors
 };
 constexpr int kBandChunkRays = 64;
 

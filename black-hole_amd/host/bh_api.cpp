@@ -182,7 +182,8 @@ struct BatchSlot
 // (device table, and its pinned host staging for the eager launch), every
 // view's band mask and list (slices of one buffer each, in view order), and
 // the pass's counters: list lengths [n], chunk offsets [n + 1], per-view
-// chunk completions [n], queue head [1].
+// chunk completions [n], queue head [1]; and the batch's tile cost keys and
+// tile order (all views' tiles, in the FP32 trace's tile numbering).
 struct BatchBandBufs
 {
     bhk::BandArgs* table = nullptr;
@@ -193,7 +194,10 @@ struct BatchBandBufs
     unsigned* sorted = nullptr;   // each view's list, longest rays first
     unsigned* hist = nullptr;     // [viewCap x 512] sort histograms
     unsigned* words = nullptr;
-    size_t pixCap = 0;
+    unsigned* tileKey = nullptr;  // [tileCap]
+    unsigned* tileOrder = nullptr;// [tileCap]
+    unsigned* tileHist = nullptr; // [512]
+    size_t pixCap = 0, tileCap = 0;
     int viewCap = 0;
 };
 
@@ -207,18 +211,22 @@ void freeBatchBand(BatchBandBufs& b)
     (void)hipFree(b.sorted);
     (void)hipFree(b.hist);
     (void)hipFree(b.words);
+    (void)hipFree(b.tileKey);
+    (void)hipFree(b.tileOrder);
+    (void)hipFree(b.tileHist);
     b = BatchBandBufs{};
 }
 
-// Grows the buffers to n views and `pix` pixels (after `st` has finished
-// with the old ones); `pinned`: with a host staging table.
-hipError_t growBatchBand(BatchBandBufs& b, int n, size_t pix, bool pinned, hipStream_t st)
+// Grows the buffers to n views, `pix` pixels and `tiles` tiles (after `st`
+// has finished with the old ones); `pinned`: with a host staging table.
+hipError_t growBatchBand(BatchBandBufs& b, int n, size_t pix, size_t tiles, bool pinned, hipStream_t st)
 {
-    if (n <= b.viewCap && pix <= b.pixCap && (!pinned || b.hostTable)) return hipSuccess;
+    if (n <= b.viewCap && pix <= b.pixCap && tiles <= b.tileCap && (!pinned || b.hostTable)) return hipSuccess;
     hipError_t e = st ? hipStreamSynchronize(st) : hipSuccess;
     if (e != hipSuccess) return e;
     const int nv = std::max(n, b.viewCap);
     const size_t np = std::max(std::max(pix, b.pixCap), (size_t)1);
+    const size_t nt = std::max(std::max(tiles, b.tileCap), (size_t)1);
     freeBatchBand(b);
     e = hipMalloc(&b.table, sizeof(bhk::BandArgs) * nv);
     if (e == hipSuccess && pinned) e = hipHostMalloc(&b.hostTable, sizeof(bhk::BandArgs) * nv, hipHostMallocDefault);
@@ -228,9 +236,13 @@ hipError_t growBatchBand(BatchBandBufs& b, int n, size_t pix, bool pinned, hipSt
     if (e == hipSuccess) e = hipMalloc(&b.sorted, np * sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc(&b.hist, sizeof(unsigned) * 512 * (size_t)nv);
     if (e == hipSuccess) e = hipMalloc(&b.words, sizeof(unsigned) * (3 * (size_t)nv + 2));
+    if (e == hipSuccess) e = hipMalloc(&b.tileKey, sizeof(unsigned) * nt);
+    if (e == hipSuccess) e = hipMalloc(&b.tileOrder, sizeof(unsigned) * nt);
+    if (e == hipSuccess) e = hipMalloc(&b.tileHist, sizeof(unsigned) * 512);
     if (e != hipSuccess) { freeBatchBand(b); return e; }
     b.viewCap = nv;
     b.pixCap = np;
+    b.tileCap = nt;
     return hipSuccess;
 }
 
@@ -960,6 +972,7 @@ struct BatchBandPlan
     hipStream_t aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     int blocks = 1;
+    unsigned* order = nullptr;  // the batch's tile order, written after the classification
 };
 
 // Band descriptors of every view of T (records assigned) in the buffers b,
@@ -994,12 +1007,17 @@ void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b,
         a.tilesX = d.tilesX;
         a.jitter = d.jitter;
         a.classifyBase = blocks;
+        a.tileKey = b.tileKey + d.tileBase;
         d.bandMask = a.mask;
         const size_t px = (size_t)d.P.width * d.P.height;
         blocks += (unsigned)((px + 255) / 256);
         off += px;
     }
     plan.A.classifyBlocks = blocks;
+    plan.A.tileKey = b.tileKey;
+    plan.A.nTiles = T.tiles;
+    plan.A.tileHist = b.tileHist;
+    plan.order = b.tileOrder;
     plan.A.views = b.table;
     plan.A.nViews = n;
     plan.A.counts = w;
@@ -1035,6 +1053,7 @@ hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int 
         B.stamps = A.stamps;
         e = bhk::launchBatchBandReset(B, st);
         if (e == hipSuccess) e = bhk::launchBandClassifyBatch(B, st);
+        if (e == hipSuccess) e = bhk::launchTileOrder(B.tileKey, B.nTiles, B.tileHist, band->order, st);
         if (e == hipSuccess) e = bhk::launchBatchBandSort(B, st);
         if (e == hipSuccess) e = bhk::launchBatchBandScan(B, st);
         if (e == hipSuccess) e = hipEventRecord(band->fork, st);
@@ -1046,7 +1065,11 @@ hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int 
             // once the auxiliary stream is forked it is joined back, also when
             // the trace launch fails, so no work of this batch outlives the
             // caller's stream order
-            e = math ? bhk::launchTraceBatchFast(A, L, band->aux) : bhk::launchTraceBatchExact(A, L, band->aux);
+            // the FP32 trace takes the batch's tiles costliest first, by the
+            // classification's tile keys
+            bhk::BatchArgs AO = A;
+            AO.order = band->order;
+            e = math ? bhk::launchTraceBatchFast(AO, L, band->aux) : bhk::launchTraceBatchExact(AO, L, band->aux);
             hipError_t j = hipEventRecord(band->join, band->aux);
             if (j == hipSuccess) j = hipStreamWaitEvent(st, band->join, 0);
             if (e == hipSuccess) e = j;
@@ -1082,7 +1105,7 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
     BatchBandPlan plan;
     if (band)
     {
-        e = growBatchBand(S->bband, n, T.totalPix, true, st);
+        e = growBatchBand(S->bband, n, T.totalPix, (size_t)T.tiles, true, st);
         if (e == hipSuccess && !S->aux)
         {
             e = hipStreamCreateWithFlags(&S->aux, hipStreamNonBlocking);
@@ -1134,6 +1157,7 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
     A.counter = S->counter;
     A.viewDone = S->viewDone;
     A.stamps = S->stamps;
+    A.order = nullptr;
     S->lastViews = n;
     S->lastPersistent = true;
     const BhSchedule sc = currentSchedule();
@@ -1564,7 +1588,7 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
         assignBatchRecords(T, g->rec);
         if (g->band)
         {
-            e = growBatchBand(g->bband, nViews, T.totalPix, false, nullptr);
+            e = growBatchBand(g->bband, nViews, T.totalPix, (size_t)T.tiles, false, nullptr);
             if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->plan.aux, hipStreamNonBlocking);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&g->plan.fork, hipEventDisableTiming);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&g->plan.join, hipEventDisableTiming);
@@ -1590,6 +1614,7 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
     A.counter = g->counter;
     A.viewDone = g->viewDone;
     A.stamps = g->stamps;
+    A.order = nullptr;
     const BhSchedule sc = currentSchedule();
     g->wgSize = sc.wgSize;
     if (autotune)
