@@ -311,7 +311,7 @@ std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 // workgroups: a resident FP64 band wavefront then takes exactly one
 // wavefront slot from the tile grid (a 4-wavefront workgroup cannot start
 // on a CU whose free slots do not take all four).
-BhSchedule g_sched = {64, 4096, 64, 1, 0};
+BhSchedule g_sched = {64, 4096, 64, 1, 2};
 BhPrecision g_prec = {1, 4e-2f, 1e-3f};
 constexpr int kBandCountSlot = 16;            // counter[16]: band list length (own 64-byte line)
 constexpr int kCandCountSlot = 32;            // counter[32]: surrogate candidates
@@ -551,7 +551,7 @@ ViewMemo g_fixedMemo[64];
 // are recorded, and the FP64 band and FP32 tile traces run one after the
 // other on `st` (no auxiliary stream).  `memo` holds the last view traced
 // into `rec` (null: the stream's own).
-hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, const BhSchedule& sc,
+hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, const BhSchedule& scIn,
                         hipStream_t st, const int* dynIndex = nullptr, bool allowTimers = true,
                         ViewMemo* memo = nullptr, const bhk::FrameBufs* fixed = nullptr)
 {
@@ -563,7 +563,21 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     if (!memo) memo = fixed ? &g_fixedMemo[(dev >= 0 && dev < 64) ? dev : 0] : &Sc->memo;
     if (Sc) Sc->routeTimed = false;
     const int kerr = isKerr(P.model) ? 1 : 0;
+    BhSchedule sc = scIn;
     const int tiles = sc.raysPerRefill >= 64 ? 1 : 0;
+    if (sc.diskCoop == 2)
+    {
+        // automatic: cooperative disk sampling when the whole-tile frame has
+        // no more tiles than the tile grid has wavefronts.  Each wavefront
+        // then traces one tile, the frame is as long as its slowest tile,
+        // and lanes whose rays have ended take over the disk sub-samples of
+        // the rays still running (the same operations in the same order).
+        const int occ0 = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, 0, tiles)
+                                     : bhk::traceOccupancyExact(kerr, sc.wgSize, 0, tiles);
+        const long long waves = (long long)occ0 * numCUs(dev) * (sc.wgSize / 64);
+        const long long nTiles = (long long)((P.width + 7) / 8) * ((P.height + 7) / 8);
+        sc.diskCoop = (tiles && nTiles <= waves) ? 1 : 0;
+    }
     const int occ = sc.mathMode ? bhk::traceOccupancyFast(kerr, sc.wgSize, sc.diskCoop, tiles)
                                 : bhk::traceOccupancyExact(kerr, sc.wgSize, sc.diskCoop, tiles);
     const long long want = (long long)occ * numCUs(dev);
@@ -1990,7 +2004,7 @@ bh_error_t bhSetSchedule(const BhSchedule* s)
     if (s->stepsPerChunk < 1 || s->stepsPerChunk > 4096) return hipErrorInvalidValue;
     if (s->mathMode != 0 && s->mathMode != 1) return hipErrorInvalidValue;
     if (s->raysPerRefill < 1 || s->raysPerRefill > 64) return hipErrorInvalidValue;
-    if (s->diskCoop != 0 && s->diskCoop != 1) return hipErrorInvalidValue;
+    if (s->diskCoop < 0 || s->diskCoop > 2) return hipErrorInvalidValue;
     std::lock_guard<std::mutex> lk(g_mu);
     g_sched = *s;
     return hipSuccess;

@@ -376,7 +376,9 @@ typedef struct BhSchedule
     int raysPerRefill;  /* rays a wavefront takes from the queue at once: it refills
                            when this many of its 64 lanes are idle (1..64) */
     int mathMode;       /* 0 exact IEEE arithmetic, 1 hardware fast intrinsics */
-    int diskCoop;       /* 1: disk chords sampled cooperatively by the wavefront */
+    int diskCoop;       /* 1: disk chords sampled cooperatively by the wavefront;
+                           2 (default): cooperatively when a whole-tile frame has
+                           no more 8x8 tiles than the tile grid has wavefronts */
 } BhSchedule;
 void bhGetSchedule(BhSchedule* s);
 bh_error_t bhSetSchedule(const BhSchedule* s);   /* -1 (hipErrorInvalidValue-like) on bad values */

@@ -178,14 +178,20 @@ def test_T6_pipeline_determinism_and_accumulation(gpu, oracle):
 # ----------------------------------------------------------------- schedule invariance
 @pytest.mark.parametrize("math", [0, 1])
 def test_schedule_invariance(gpu, math):
-    """Workgroup size and steps per refill chunk change only the order in
-    which rays are traced, never a result bit."""
+    """Workgroup size, steps per refill chunk, lane refill and cooperative
+    disk sampling change only the order in which rays and sub-samples are
+    traced, never a result bit."""
     P = scaled_workload(gpu, "kerr09_1080p_high", 0, 256, 144)
     outs = []
-    for wg, chunk in [(256, 16), (64, 1), (128, 7), (256, 128)]:
-        gpu.set_schedule(wg=wg, chunk=chunk, math=math)
-        say("schedule wg", wg, "chunk", chunk, "math", math)
-        outs.append(gpu.render_pipeline(P))
+    before = gpu.get_schedule()
+    try:
+        for wg, chunk, refill, coop in [(256, 16, 64, 0), (64, 1, 64, 0), (128, 7, 64, 0), (256, 128, 64, 0),
+                                        (64, 4096, 64, 1), (64, 4096, 16, 1), (64, 32, 16, 0), (64, 4096, 64, 2)]:
+            gpu.set_schedule(wg=wg, chunk=chunk, math=math, refill=refill, coop=coop)
+            say("schedule wg", wg, "chunk", chunk, "refill", refill, "coop", coop, "math", math)
+            outs.append(gpu.render_pipeline(P))
+    finally:
+        gpu.restore_schedule(before)
     for rgba, acc in outs[1:]:
         assert np.array_equal(rgba, outs[0][0])
         assert np.array_equal(acc.view(np.uint32), outs[0][1].view(np.uint32))
