@@ -416,9 +416,9 @@ def other_configs(bt, lib, fp64):
     pf = persistent(0)
     out["uneven_views_64_persistent"] = dict(
         rays=rays, **pb,
-        launch="bhRenderBatchPersistent / bhBatchGraph: every view classified, then one persistent FP32 trace over "
-               "a device queue of (view, tile) items beside one FP64 trace of all views' band rays over a queue of "
-               "(view, 64-ray chunk) items; batched shade/bloom/composite",
+        launch="bhRenderBatchPersistent / bhBatchGraph: every view classified in one launch, then one persistent FP32 "
+               "trace over a device queue of all views' tiles (costliest first) beside one FP64 trace of all views' "
+               "band rays over a queue of (view, 64-ray chunk) items; batched shade/bloom/composite",
         fp32_only={k: pf[k] for k in ("eager_batch_ms", "eager_mrays_s", "fresh_eager_batch_ms", "graph_batch_ms",
                                       "graph_mrays_s")})
     del B2
