@@ -12,6 +12,34 @@
 
 namespace bhk {
 
+// Lowers tileKey[tile] to `key` (every lane of the wavefront comes in; lanes
+// with tile < 0 or an all-ones key have none).  Eight consecutive lanes
+// usually hold eight pixels of one tile row: such groups reduce their keys
+// first and issue one atomic.
+BHK_INL void tileKeyMin(unsigned* tileKey, int tile, unsigned key)
+{
+    const unsigned lane = __lane_id();
+    const int lead = __shfl(tile, (int)(lane & ~7u));
+    const unsigned long long same = __ballot(tile == lead);
+    if (((same >> (lane & ~7u)) & 0xFFull) == 0xFFull)
+    {
+        key = min(key, (unsigned)__shfl_xor((int)key, 1));
+        key = min(key, (unsigned)__shfl_xor((int)key, 2));
+        key = min(key, (unsigned)__shfl_xor((int)key, 4));
+        if ((lane & 7u) == 0u && tile >= 0 && key != 0xFFFFFFFFu) atomicMin(tileKey + tile, key);
+    }
+    else if (tile >= 0 && key != 0xFFFFFFFFu) atomicMin(tileKey + tile, key);
+}
+
+// Cost key of pixel i of a view traced again, from the RK4 step count the
+// previous frame left in its record: 64 / (steps + 1) (smaller: longer).
+BHK_INL unsigned stepKey(const BhRayRecord* rec, int i)
+{
+    int steps = reinterpret_cast<const int*>(rec + i)[3] >> 8;
+    steps = steps < 0 ? 0 : (steps > (1 << 20) ? (1 << 20) : steps);
+    return __float_as_uint(64.0f / (float)(steps + 1));
+}
+
 // Pixel i of A's view (every lane of the wavefront comes in; lanes past the
 // image take part in the wavefront's list appends only).
 BHK_INL void classifyPixel(const BandArgs& A, int i)
@@ -56,12 +84,7 @@ BHK_INL void classifyPixel(const BandArgs& A, int i)
         // Non-negative floats order like their bit patterns.  Keys only
         // order work: any key gives the same frame.
         unsigned cost;
-        if (A.prevRec)
-        {
-            int steps = reinterpret_cast<const int*>(A.prevRec + i)[3] >> 8;
-            steps = steps < 0 ? 0 : (steps > (1 << 20) ? (1 << 20) : steps);
-            cost = __float_as_uint(64.0f / (float)(steps + 1));
-        }
+        if (A.prevRec) cost = stepKey(A.prevRec, i);
         else cost = ok ? __float_as_uint((float)fabs(delta)) : 0u;
         if (!band) key = cost;
         else bandKey = cost;
@@ -82,21 +105,7 @@ BHK_INL void classifyPixel(const BandArgs& A, int i)
         }
     }
     const unsigned lane = __lane_id();
-    if (A.tileKey)
-    {
-        // Eight consecutive lanes usually hold eight pixels of one tile row:
-        // such groups reduce their keys first and issue one atomic.
-        const int lead = __shfl(tile, (int)(lane & ~7u));
-        const unsigned long long same = __ballot(tile == lead);
-        if (((same >> (lane & ~7u)) & 0xFFull) == 0xFFull)
-        {
-            key = min(key, (unsigned)__shfl_xor((int)key, 1));
-            key = min(key, (unsigned)__shfl_xor((int)key, 2));
-            key = min(key, (unsigned)__shfl_xor((int)key, 4));
-            if ((lane & 7u) == 0u && tile >= 0 && key != 0xFFFFFFFFu) atomicMin(A.tileKey + tile, key);
-        }
-        else if (tile >= 0 && key != 0xFFFFFFFFu) atomicMin(A.tileKey + tile, key);
-    }
+    if (A.tileKey) tileKeyMin(A.tileKey, tile, key);
     const unsigned long long m = __ballot(band);
     if (m != 0ull)
     {
@@ -437,6 +446,55 @@ hipError_t launchBatchBandSort(const BatchBandArgs& A, hipStream_t st)
 hipError_t launchBatchBandScan(const BatchBandArgs& A, hipStream_t st)
 {
     hipLaunchKernelGGL(batchBandScanKernel, dim3(1), dim3(64), 0, st, A);
+    return hipGetLastError();
+}
+
+// Tile keys of a view traced again with the band pass off (no
+// classification): the smallest step key over each tile's pixels.
+__global__ __launch_bounds__(256) void tileKeysKernel(const BhRayRecord* rec, int W, int H, int tilesX,
+                                                      unsigned* tileKey)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int tile = -1;
+    unsigned key = 0xFFFFFFFFu;
+    if (i < W * H)
+    {
+        tile = ((i / W) >> 3) * tilesX + ((i % W) >> 3);
+        key = stepKey(rec, i);
+    }
+    tileKeyMin(tileKey, tile, key);
+}
+
+// The same for every view of a batch (blockIdx.z = view; blocks past a
+// view's pixels leave whole), into the batch's tile numbering.
+__global__ __launch_bounds__(256) void tileKeysBatchKernel(const ViewDesc* views, unsigned* tileKeys)
+{
+    const ViewDesc& V = views[blockIdx.z];
+    const int W = V.P.width, n = W * V.P.height;
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int tile = -1;
+    unsigned key = 0xFFFFFFFFu;
+    if (i < n)
+    {
+        tile = ((i / W) >> 3) * V.tilesX + ((i % W) >> 3);
+        key = stepKey(V.rec, i);
+    }
+    tileKeyMin(tileKeys + V.tileBase, tile, key);
+}
+
+hipError_t launchTileKeys(const BhRayRecord* rec, int W, int H, int tilesX, unsigned* tileKey, hipStream_t st)
+{
+    const int n = W * H;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tileKeysKernel, dim3((n + 255) / 256), dim3(256), 0, st, rec, W, H, tilesX, tileKey);
+    return hipGetLastError();
+}
+
+hipError_t launchTileKeysBatch(const ViewDesc* views, int nViews, int maxPix, unsigned* tileKeys, hipStream_t st)
+{
+    if (nViews <= 0 || maxPix <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tileKeysBatchKernel, dim3((maxPix + 255) / 256, 1, nViews), dim3(256), 0, st, views, tileKeys);
     return hipGetLastError();
 }
 

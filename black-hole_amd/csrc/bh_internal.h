@@ -182,6 +182,11 @@ int traceOccupancyFast(int kerr, int wgSize, int coop, int tiles);
 hipError_t launchBandClassify(const BandArgs& A, hipStream_t st);
 // Every view of a batch classified in one launch (A.views: the device table).
 hipError_t launchBandClassifyBatch(const BatchBandArgs& A, hipStream_t st);
+// Tile cost keys from the step counts of a view's previous records (the band
+// pass off, so no classification gives them), for one view and for every
+// view of a batch (the batch's tile numbering; maxPix: the largest view).
+hipError_t launchTileKeys(const BhRayRecord* rec, int W, int H, int tilesX, unsigned* tileKey, hipStream_t st);
+hipError_t launchTileKeysBatch(const ViewDesc* views, int nViews, int maxPix, unsigned* tileKeys, hipStream_t st);
 hipError_t launchSurrogateDataset(const DatasetArgs& A, hipStream_t st);
 hipError_t launchSurrogateRoute(const RouteArgs& A, int maxRows, int numCU, hipStream_t st);
 

@@ -182,8 +182,7 @@ struct BatchSlot
 // (device table, and its pinned host staging for the eager launch), every
 // view's band mask and list (slices of one buffer each, in view order), and
 // the pass's counters: list lengths [n], chunk offsets [n + 1], per-view
-// chunk completions [n], queue head [1]; and the batch's tile cost keys and
-// tile order (all views' tiles, in the FP32 trace's tile numbering).
+// chunk completions [n], queue head [1].
 struct BatchBandBufs
 {
     bhk::BandArgs* table = nullptr;
@@ -194,12 +193,45 @@ struct BatchBandBufs
     unsigned* sorted = nullptr;   // each view's list, longest rays first
     unsigned* hist = nullptr;     // [viewCap x 512] sort histograms
     unsigned* words = nullptr;
-    unsigned* tileKey = nullptr;  // [tileCap]
-    unsigned* tileOrder = nullptr;// [tileCap]
-    unsigned* tileHist = nullptr; // [512]
-    size_t pixCap = 0, tileCap = 0;
+    size_t pixCap = 0;
     int viewCap = 0;
 };
+
+// The tile order of a persistent batch: every tile's cost key (all views'
+// tiles in the FP32 trace's tile numbering), the order, costliest first,
+// and the sort's bucket counts and cursors.
+struct BatchTileBufs
+{
+    unsigned* key = nullptr;      // [cap]
+    unsigned* order = nullptr;    // [cap]
+    unsigned* hist = nullptr;     // [512]
+    size_t cap = 0;
+};
+
+void freeBatchTiles(BatchTileBufs& b)
+{
+    (void)hipFree(b.key);
+    (void)hipFree(b.order);
+    (void)hipFree(b.hist);
+    b = BatchTileBufs{};
+}
+
+// Grows the tile buffers to `tiles` tiles (after `st` has finished with the
+// old ones).
+hipError_t growBatchTiles(BatchTileBufs& b, size_t tiles, hipStream_t st)
+{
+    if (tiles <= b.cap && b.key) return hipSuccess;
+    hipError_t e = st ? hipStreamSynchronize(st) : hipSuccess;
+    if (e != hipSuccess) return e;
+    const size_t nt = std::max(std::max(tiles, b.cap), (size_t)1);
+    freeBatchTiles(b);
+    e = hipMalloc(&b.key, sizeof(unsigned) * nt);
+    if (e == hipSuccess) e = hipMalloc(&b.order, sizeof(unsigned) * nt);
+    if (e == hipSuccess) e = hipMalloc(&b.hist, sizeof(unsigned) * 512);
+    if (e != hipSuccess) { freeBatchTiles(b); return e; }
+    b.cap = nt;
+    return hipSuccess;
+}
 
 void freeBatchBand(BatchBandBufs& b)
 {
@@ -211,22 +243,18 @@ void freeBatchBand(BatchBandBufs& b)
     (void)hipFree(b.sorted);
     (void)hipFree(b.hist);
     (void)hipFree(b.words);
-    (void)hipFree(b.tileKey);
-    (void)hipFree(b.tileOrder);
-    (void)hipFree(b.tileHist);
     b = BatchBandBufs{};
 }
 
-// Grows the buffers to n views, `pix` pixels and `tiles` tiles (after `st`
-// has finished with the old ones); `pinned`: with a host staging table.
-hipError_t growBatchBand(BatchBandBufs& b, int n, size_t pix, size_t tiles, bool pinned, hipStream_t st)
+// Grows the buffers to n views and `pix` pixels (after `st` has finished
+// with the old ones); `pinned`: with a host staging table.
+hipError_t growBatchBand(BatchBandBufs& b, int n, size_t pix, bool pinned, hipStream_t st)
 {
-    if (n <= b.viewCap && pix <= b.pixCap && tiles <= b.tileCap && (!pinned || b.hostTable)) return hipSuccess;
+    if (n <= b.viewCap && pix <= b.pixCap && (!pinned || b.hostTable)) return hipSuccess;
     hipError_t e = st ? hipStreamSynchronize(st) : hipSuccess;
     if (e != hipSuccess) return e;
     const int nv = std::max(n, b.viewCap);
     const size_t np = std::max(std::max(pix, b.pixCap), (size_t)1);
-    const size_t nt = std::max(std::max(tiles, b.tileCap), (size_t)1);
     freeBatchBand(b);
     e = hipMalloc(&b.table, sizeof(bhk::BandArgs) * nv);
     if (e == hipSuccess && pinned) e = hipHostMalloc(&b.hostTable, sizeof(bhk::BandArgs) * nv, hipHostMallocDefault);
@@ -236,13 +264,9 @@ hipError_t growBatchBand(BatchBandBufs& b, int n, size_t pix, size_t tiles, bool
     if (e == hipSuccess) e = hipMalloc(&b.sorted, np * sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc(&b.hist, sizeof(unsigned) * 512 * (size_t)nv);
     if (e == hipSuccess) e = hipMalloc(&b.words, sizeof(unsigned) * (3 * (size_t)nv + 2));
-    if (e == hipSuccess) e = hipMalloc(&b.tileKey, sizeof(unsigned) * nt);
-    if (e == hipSuccess) e = hipMalloc(&b.tileOrder, sizeof(unsigned) * nt);
-    if (e == hipSuccess) e = hipMalloc(&b.tileHist, sizeof(unsigned) * 512);
     if (e != hipSuccess) { freeBatchBand(b); return e; }
     b.viewCap = nv;
     b.pixCap = np;
-    b.tileCap = nt;
     return hipSuccess;
 }
 
@@ -301,6 +325,7 @@ struct Scratch
     hipStream_t aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     BatchBandBufs bband;         // band pass of persistent batches on this stream
+    BatchTileBufs btiles;        // their tile order
     std::vector<RenderParams> lastBatchViews;   // views of the last persistent batch (sample index, disk time 0)
     const BhRayRecord* lastBatchRec = nullptr;  // and the records it traced into
 };
@@ -706,10 +731,29 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
             if (e != hipSuccess) return e;
         }
     }
+    bool orderedNoBand = false;
+    if (!band && !fixed && tiles && repeatView && P.width > 0 && P.height > 0)
+    {
+        // band pass off (the reference's arithmetic): no classification, but
+        // a view traced again still orders its tiles longest first, by the
+        // step counts its previous frame left in the records
+        const long long nT = (long long)((P.width + 7) / 8) * ((P.height + 7) / 8);
+        hipError_t e = getTileScratch(Sc, (size_t)nT, st);
+        if (e == hipSuccess) e = bhk::launchFrameReset(counter, bhk::kFrameCounterWords, Sc->tileKey, (int)nT, Sc->tileHist,
+                                                      nullptr, st);
+        if (e == hipSuccess) e = bhk::launchTileKeys(rec, P.width, P.height, (P.width + 7) / 8, Sc->tileKey, st);
+        if (e == hipSuccess) e = bhk::launchTileOrder(Sc->tileKey, (int)nT, Sc->tileHist, Sc->tileOrder, st);
+        if (e != hipSuccess) return e;
+        tileOrder = Sc->tileOrder;
+        Sc->lastOrderTiles = (int)nT;
+        orderedNoBand = true;
+    }
+    else if (!band && Sc)
+        Sc->lastOrderTiles = 0;
     bhk::TraceArgs A;
     const BhSchedule& scT = sc;
     A.bandMask = bandMask;
-    A.tileOrder = BA.tileKey ? tileOrder : nullptr;
+    A.tileOrder = (BA.tileKey || orderedNoBand) ? tileOrder : nullptr;
     A.P = P;
     A.rec = rec;
     A.counter = counter;
@@ -727,7 +771,8 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     S.wgSize = scT.wgSize;
     S.blocks = (int)(wantT < maxUseful ? wantT : maxUseful);
     if (S.blocks < 1) S.blocks = 1;
-    hipError_t e = band ? hipSuccess : bhk::launchResetCounter(counter, st);   // with the band: reset above
+    // with the band, or the tiles ordered without it: counters reset above
+    hipError_t e = (band || orderedNoBand) ? hipSuccess : bhk::launchResetCounter(counter, st);
     if (e != hipSuccess) return e;
     const int bandBlocks = 4 * numCUs(dev);
     if (fixed)
@@ -995,7 +1040,8 @@ struct BatchBandPlan
 // first, by keys from the step counts its records hold when the batch
 // repeats (`repeat`), else from the band statistics.  The device table
 // b.table is not uploaded here.
-void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b, bool repeat, BatchBandPlan& plan)
+void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b, const BatchTileBufs& tb, bool repeat,
+                   BatchBandPlan& plan)
 {
     const int n = (int)T.D.size();
     plan.views.assign((size_t)n, bhk::BandArgs{});
@@ -1021,17 +1067,17 @@ void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b,
         a.tilesX = d.tilesX;
         a.jitter = d.jitter;
         a.classifyBase = blocks;
-        a.tileKey = b.tileKey + d.tileBase;
+        a.tileKey = tb.key + d.tileBase;
         d.bandMask = a.mask;
         const size_t px = (size_t)d.P.width * d.P.height;
         blocks += (unsigned)((px + 255) / 256);
         off += px;
     }
     plan.A.classifyBlocks = blocks;
-    plan.A.tileKey = b.tileKey;
+    plan.A.tileKey = tb.key;
     plan.A.nTiles = T.tiles;
-    plan.A.tileHist = b.tileHist;
-    plan.order = b.tileOrder;
+    plan.A.tileHist = tb.hist;
+    plan.order = tb.order;
     plan.A.views = b.table;
     plan.A.nViews = n;
     plan.A.counts = w;
@@ -1046,9 +1092,13 @@ void planBatchBand(BatchTable& T, const BhPrecision& pr, const BatchBandBufs& b,
 // With `band`: every view classified, then the FP64 band rays of all views
 // traced on `st` beside the FP32 batch trace on the auxiliary stream (forked
 // after the classifications, joined before shading), as the single-view
-// pipeline does.  No memset and no copy: capturable as a graph.
+// pipeline does.  Without it, `stepOrder` (a batch that repeats the
+// previous one into the same records): the tiles ordered costliest first by
+// the step counts those records hold.  No memset and no copy: capturable as
+// a graph.
 hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int wgSize, int math, hipStream_t st,
-                               Scratch* timers, const BatchBandPlan* band = nullptr)
+                               Scratch* timers, const BatchBandPlan* band = nullptr,
+                               const BatchTileBufs* stepOrder = nullptr)
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
@@ -1089,7 +1139,19 @@ hipError_t enqueueBatchKernels(const bhk::BatchArgs& A, int maxW, int maxH, int 
             if (e == hipSuccess) e = j;
         }
     }
-    else e = math ? bhk::launchTraceBatchFast(A, L, st) : bhk::launchTraceBatchExact(A, L, st);
+    else
+    {
+        bhk::BatchArgs AO = A;
+        if (stepOrder)
+        {
+            e = bhk::launchFrameReset(nullptr, 0, stepOrder->key, A.totalTiles, stepOrder->hist, nullptr, st);
+            if (e == hipSuccess) e = bhk::launchTileKeysBatch(A.views, A.nViews, maxW * maxH, stepOrder->key, st);
+            if (e == hipSuccess) e = bhk::launchTileOrder(stepOrder->key, A.totalTiles, stepOrder->hist, stepOrder->order, st);
+            if (e != hipSuccess) return e;
+            AO.order = stepOrder->order;
+        }
+        e = math ? bhk::launchTraceBatchFast(AO, L, st) : bhk::launchTraceBatchExact(AO, L, st);
+    }
     if (e != hipSuccess) return e;
     if (timers) recordEvent(timers, 1, st);
     e = bhk::launchShadeBlendBatch(A.views, A.nViews, maxW, maxH, st);
@@ -1116,20 +1178,11 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
     assignBatchRecords(T, S->rec);
     const BhPrecision pr = currentPrecision();
     const bool band = pr.fp64Band != 0;
-    BatchBandPlan plan;
-    if (band)
+    // the same views as the last persistent batch on this stream (apart from
+    // sample index and disk time), into the same records: their step counts
+    // order the tiles and band rays
+    bool repeat;
     {
-        e = growBatchBand(S->bband, n, T.totalPix, (size_t)T.tiles, true, st);
-        if (e == hipSuccess && !S->aux)
-        {
-            e = hipStreamCreateWithFlags(&S->aux, hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&S->fork, hipEventDisableTiming);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&S->join, hipEventDisableTiming);
-        }
-        if (e != hipSuccess) return e;
-        // the same views as the last persistent batch on this stream (apart
-        // from sample index and disk time), into the same records: their
-        // step counts order the band rays
         std::vector<RenderParams> key(T.D.size());
         for (size_t v = 0; v < T.D.size(); ++v)
         {
@@ -1137,11 +1190,28 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
             key[v].sampleIndex = 0;
             key[v].diskTime = 0.f;
         }
-        const bool repeat = S->lastBatchRec == S->rec && S->lastBatchViews.size() == key.size() &&
-                            std::memcmp(S->lastBatchViews.data(), key.data(), sizeof(RenderParams) * key.size()) == 0;
+        repeat = S->lastBatchRec == S->rec && S->lastBatchViews.size() == key.size() &&
+                 std::memcmp(S->lastBatchViews.data(), key.data(), sizeof(RenderParams) * key.size()) == 0;
         S->lastBatchViews = key;
         S->lastBatchRec = S->rec;
-        planBatchBand(T, pr, S->bband, repeat, plan);
+    }
+    if (band || repeat)
+    {
+        e = growBatchTiles(S->btiles, (size_t)T.tiles, st);
+        if (e != hipSuccess) return e;
+    }
+    BatchBandPlan plan;
+    if (band)
+    {
+        e = growBatchBand(S->bband, n, T.totalPix, true, st);
+        if (e == hipSuccess && !S->aux)
+        {
+            e = hipStreamCreateWithFlags(&S->aux, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&S->fork, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&S->join, hipEventDisableTiming);
+        }
+        if (e != hipSuccess) return e;
+        planBatchBand(T, pr, S->bband, S->btiles, repeat, plan);
         plan.aux = S->aux;
         plan.fork = S->fork;
         plan.join = S->join;
@@ -1176,7 +1246,8 @@ hipError_t renderBatchPersistent(int n, const RenderParams* views, uchar4* const
     S->lastPersistent = true;
     const BhSchedule sc = currentSchedule();
     const bool timed = g_timers.load() != 0;
-    e = enqueueBatchKernels(A, T.maxW, T.maxH, sc.wgSize, sc.mathMode, st, timed ? S : nullptr, band ? &plan : nullptr);
+    e = enqueueBatchKernels(A, T.maxW, T.maxH, sc.wgSize, sc.mathMode, st, timed ? S : nullptr, band ? &plan : nullptr,
+                            (!band && repeat) ? &S->btiles : nullptr);
     if (e != hipSuccess) return e;
     if (timed)
     {
@@ -1366,6 +1437,7 @@ void freeScratch(Scratch& S)
     if (S.fork) (void)hipEventDestroy(S.fork);
     if (S.join) (void)hipEventDestroy(S.join);
     freeBatchBand(S.bband);
+    freeBatchTiles(S.btiles);
 }
 
 // Releases what the library holds for `st` on the current device: its
@@ -1435,6 +1507,7 @@ struct BhBatchGraph
     float tunedMs[3] = {0.f, 0.f, 0.f}; // trace+shade+post of one launch at workgroup 64, 128, 256 (0: not timed)
     bool band = false;                  // FP64 band pass (the precision at creation)
     BatchBandBufs bband;
+    BatchTileBufs btiles;               // tile order (keys from the previous replay's step counts)
     BatchBandPlan plan;                 // its aux stream and fork/join events are the graph's own
 };
 
@@ -1565,6 +1638,7 @@ void bhBatchGraphDestroy(BhBatchGraph* g)
     if (g->plan.fork) (void)hipEventDestroy(g->plan.fork);
     if (g->plan.join) (void)hipEventDestroy(g->plan.join);
     freeBatchBand(g->bband);
+    freeBatchTiles(g->btiles);
     (void)hipFree(g->table);
     (void)hipFree(g->rec);
     (void)hipFree(g->counter);
@@ -1600,9 +1674,10 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
     if (e == hipSuccess)
     {
         assignBatchRecords(T, g->rec);
-        if (g->band)
+        e = growBatchTiles(g->btiles, (size_t)T.tiles, nullptr);
+        if (e == hipSuccess && g->band)
         {
-            e = growBatchBand(g->bband, nViews, T.totalPix, (size_t)T.tiles, false, nullptr);
+            e = growBatchBand(g->bband, nViews, T.totalPix, false, nullptr);
             if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->plan.aux, hipStreamNonBlocking);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&g->plan.fork, hipEventDisableTiming);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&g->plan.join, hipEventDisableTiming);
@@ -1610,7 +1685,7 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
             {
                 // replays repeat the batch: band rays ordered by the step
                 // counts of the previous launch's records
-                planBatchBand(T, pr, g->bband, true, g->plan);
+                planBatchBand(T, pr, g->bband, g->btiles, true, g->plan);
                 int dev = 0;
                 if (hipGetDevice(&dev) != hipSuccess) dev = 0;
                 g->plan.blocks = kBatchBandWavesPerCU * numCUs(dev);
@@ -1647,7 +1722,7 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
                 e = hipEventRecord(t0, g->cs);
                 if (e == hipSuccess)
                     e = enqueueBatchKernels(A, g->maxW, g->maxH, wgs[w], sc.mathMode, g->cs, nullptr,
-                                            g->band ? &g->plan : nullptr);
+                                            g->band ? &g->plan : nullptr, g->band ? nullptr : &g->btiles);
                 if (e == hipSuccess) e = hipEventRecord(t1, g->cs);
                 if (e == hipSuccess) e = hipEventSynchronize(t1);
             }
@@ -1665,8 +1740,10 @@ bh_error_t bhBatchGraphCreate(int nViews, const RenderParams* views, bh_uchar4* 
     }
     e = hipStreamBeginCapture(g->cs, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) { bhBatchGraphDestroy(g); return e; }
+    // replays repeat the batch: tiles (and band rays) ordered by the step
+    // counts of the previous replay's records
     hipError_t le = enqueueBatchKernels(A, g->maxW, g->maxH, g->wgSize, sc.mathMode, g->cs, nullptr,
-                                        g->band ? &g->plan : nullptr);
+                                        g->band ? &g->plan : nullptr, g->band ? nullptr : &g->btiles);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(g->cs, &graph);
     if (le != hipSuccess || e != hipSuccess)
