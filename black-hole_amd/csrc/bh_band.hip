@@ -157,7 +157,6 @@ __global__ __launch_bounds__(256) void bandClassifyBatchKernel(const BandArgs* v
     classifyPixel(A, (int)(b - A.classifyBase) * 256 + (int)threadIdx.x);
 }
 
-
 // ---------------------------------------------------------------- tile order
 // The FP32 tile tracer takes tiles from a queue; a tile costs as many RK4
 // steps as its longest ray.  Taking the costliest tiles first keeps the last
@@ -232,11 +231,10 @@ __global__ __launch_bounds__(kTileBuckets) void tileScatterKernel(const unsigned
 // ---------------------------------------------------------------- band ray order
 // The FP64 band kernel hands its rays out from a queue, 64 to a wavefront.
 // Sorted longest first (smallest key: closest to the critical curve, or
-// fewest steps in the previous frame), each wavefront holds rays of similar
-// length: only
-// the first few run as long as the longest ray, the rest end early and give
-// their SIMD slots (each resident band wavefront displaces one wavefront of
-// the FP32 tile grid) back to the tile kernel.  The list is counting-sorted
+// most steps in the previous frame), each wavefront holds rays of similar
+// length: only the first few run as long as the longest ray, the rest end
+// early and give their SIMD slots (each resident band wavefront displaces
+// two wavefronts of the FP32 tile grid) back to the tile kernel.  The list is counting-sorted
 // into 256 buckets of 1/8 octave of the key (2^-24 .. 2^8); order within a
 // bucket is arbitrary, which changes no result (rays are independent).
 constexpr int kBandBuckets = 256;

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) void traceBatchKernel(const ViewDesc* __restri
         t = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl(t, 0));
         if (t >= (unsigned)A.totalTiles) break;
         // the batch tile at this queue position (costliest first when the
-        // classification ordered them)
+        // batch's tiles were ordered by their cost keys)
         if (A.order) t = (unsigned)__builtin_amdgcn_readfirstlane((int)A.order[t]);
         int lo = 0, hi = A.nViews - 1;
         while (lo < hi)
