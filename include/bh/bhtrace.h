@@ -429,15 +429,15 @@ bh_error_t bhLastBandStats(unsigned* count, float* bandMs, bh_stream_t stream);
  * traced on `stream` with kernel timers on.  hipErrorNotReady when that
  * frame was not timed.  Synchronizes the stream. */
 bh_error_t bhLastTraceKernelMs(float* tileMs, float* bandMs, bh_stream_t stream);
-/* Tile order of the last frame on `stream` (whole-tile schedule with the
- * band pass on, where the queue hands out the costliest tiles first):
- * per-tile cost keys, the smallest over the tile's FP32 rays of the float
- * bits of 64 / (steps + 1) from the previous frame's records when the frame
- * repeats the previous view into the same records, else of |delta|
- * (all ones when the tile has no FP32 ray), and the
- * queue order, `capacity` entries each (either may be NULL).  *nTiles
- * receives the tile count; hipErrorNotReady when the frame was not ordered.
- * Synchronizes the stream. */
+/* Tile order of the last frame on `stream` (whole-tile schedule, where the
+ * queue hands out the costliest tiles first): per-tile cost keys, the
+ * smallest over the tile's FP32 rays of the float bits of 64 / (steps + 1)
+ * from the previous frame's records when the frame repeats the previous
+ * view into the same records, else (band pass on) of |delta| (all ones when
+ * the tile has no FP32 ray), and the queue order, `capacity` entries each
+ * (either may be NULL).  *nTiles receives the tile count; hipErrorNotReady
+ * when the frame was not ordered (band pass off and a view not traced just
+ * before).  Synchronizes the stream. */
 bh_error_t bhLastTileOrder(unsigned* keys, unsigned* order, int capacity, int* nTiles, bh_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
