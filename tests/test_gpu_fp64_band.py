@@ -202,6 +202,37 @@ def test_tile_order_from_previous_steps(gpu, oracle, band_on, name, frame, w, h)
     say(name, "tiles", tx * ty, "ordered by previous steps; first buckets", bo[:4], "last", bo[-4:])
 
 
+@pytest.mark.parametrize("name,frame,w,h", [("kerr09_1080p_high", 0, 480, 270), ("schw_256_balanced", 0, 256, 256)])
+def test_tile_order_from_previous_steps_band_off(gpu, band_off, name, frame, w, h):
+    """With the band pass off (every ray FP32, no classification), a view
+    traced again is still ordered by the step counts of its previous frame:
+    each tile's key is the smallest 64 / (steps + 1) over all its rays, the
+    order is a permutation, and the records equal the first frame's."""
+    P = scaled_workload(gpu, name, frame, w, h)
+    Q, _ = gpu.sanitized(P)
+    buf = gpu.DeviceBuffer(Q.width * Q.height * gpu.RECORD_DTYPE.itemsize)
+    try:
+        r1 = gpu.trace_records(P, buf).copy()
+        r2 = gpu.trace_records(P, buf)
+    finally:
+        buf.close()
+    assert np.array_equal(r1.view(np.uint8), r2.view(np.uint8))
+    tx, ty = (w + 7) // 8, (h + 7) // 8
+    got = gpu.last_tile_order(tx * ty)
+    assert got is not None, "frame was not ordered"
+    keys, order = got
+    assert np.array_equal(np.sort(order), np.arange(tx * ty))
+    steps = np.clip(r1["flags"] >> 8, 0, 1 << 20).astype(np.float32)
+    k = (np.float32(64.0) / (steps + np.float32(1.0))).astype(np.float32)
+    pad = np.full((ty * 8, tx * 8), np.inf, np.float32)
+    pad[:h, :w] = k
+    want = pad.reshape(ty, 8, tx, 8).min(axis=(1, 3)).ravel()
+    gk = keys.view(np.float32).copy()
+    gk[keys == 0xFFFFFFFF] = np.inf
+    assert np.array_equal(gk, want)
+    say(name, "band off: tiles", tx * ty, "ordered by previous steps")
+
+
 # Views of the 64-view uneven batch (seed 1, 'balanced' step dPhi = 0.012) on
 # which the band widths of the 'high' step left rays over 1e-3 rad; the bands
 # now widen with the step (critical by dPhi / 0.007, polar by its square).
