@@ -610,13 +610,13 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
     // the sample index and disk time)?  Then its records predict this
     // frame's work.  Under graph capture always: replays repeat this view.
     // The prediction only orders the work; any prediction gives the same frame.
-    bool repeatView;
+    bool repeatView, capturing;
     {
         RenderParams v = P;
         v.sampleIndex = 0;
         v.diskTime = 0.f;
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        const bool capturing = hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+        capturing = hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
         repeatView = capturing || (memo->lastRec == rec && std::memcmp(&v, &memo->lastView, sizeof(v)) == 0);
         memo->lastView = v;
         memo->lastRec = rec;
@@ -732,12 +732,16 @@ hipError_t enqueueTrace(const RenderParams& P, BhRayRecord* rec, Scratch* Sc, co
         }
     }
     bool orderedNoBand = false;
-    if (!band && !fixed && tiles && repeatView && P.width > 0 && P.height > 0)
+    const long long nTilesFrame = (long long)((P.width + 7) / 8) * ((P.height + 7) / 8);
+    // (under graph capture nothing may be allocated: without the buffers the
+    // captured frame keeps the row-major order)
+    if (!band && !fixed && tiles && repeatView && P.width > 0 && P.height > 0 &&
+        (!capturing || Sc->tileCap >= (size_t)nTilesFrame))
     {
         // band pass off (the reference's arithmetic): no classification, but
         // a view traced again still orders its tiles longest first, by the
         // step counts its previous frame left in the records
-        const long long nT = (long long)((P.width + 7) / 8) * ((P.height + 7) / 8);
+        const long long nT = nTilesFrame;
         hipError_t e = getTileScratch(Sc, (size_t)nT, st);
         if (e == hipSuccess) e = bhk::launchFrameReset(counter, bhk::kFrameCounterWords, Sc->tileKey, (int)nT, Sc->tileHist,
                                                       nullptr, st);
@@ -1944,10 +1948,12 @@ bh_error_t bhAccumCapture(BhAccumulator* acc, const RenderParams& p, bh_stream_t
     const size_t n = (size_t)P.width * (size_t)P.height;
     Scratch* S = nullptr;
     e = getScratch(cs, n, &S);
+    // the tile order's buffers with the band pass on or off (a captured step
+    // repeats its view, so its tiles are ordered either way)
+    if (e == hipSuccess) e = getTileScratch(S, (size_t)((P.width + 7) / 8) * (size_t)((P.height + 7) / 8), cs);
     if (e == hipSuccess && currentPrecision().fp64Band)
     {
         e = getBandScratch(S, n, cs);
-        if (e == hipSuccess) e = getTileScratch(S, (size_t)((P.width + 7) / 8) * (size_t)((P.height + 7) / 8), cs);
         if (e == hipSuccess && currentSurrogate().s) e = getCandScratch(S, n, cs);
     }
     if (e != hipSuccess) return e;
