@@ -47,7 +47,7 @@ BAND_FLOPS = {0: 4791139978, 1: 4793639269, 2: 4794055461}
 # HBM bytes per launch of the FP32 tile kernel on WORKLOAD (FP64 band on):
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by a
 # plain-copy calibration (FETCH x2.0, WRITE x1.0); profiles/pmc_trace_kerr09_1080p_high.md
-TRAFFIC_BYTES = (7621 * 2 + 65446) * 1024
+TRAFFIC_BYTES = (7611 * 2 + 65680) * 1024
 FP32_NONPACKED_TFLOPS = 78.6
 # The surrogate network: 224 -> 128 -> 128 -> 128 -> 128 -> 16 (padded)
 # multiply-adds per row, 2 flops each; a training step is the forward pass
