@@ -558,7 +558,7 @@ def main():
     os.environ["OMP_PLACES"] = "cores"
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(pixels shard across GPUs by rows, see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU run would shard the pixels of a frame across GPUs by rows)", file=sys.stderr)
         sys.exit(2)
     import bhtrace as bt
 
