@@ -374,12 +374,16 @@ typedef struct BhSchedule
     int wgSize;         /* threads per workgroup of the trace kernel: 64, 128, 256 */
     int stepsPerChunk;  /* max RK4 steps a wavefront runs between refill checks: 1..4096 */
     int raysPerRefill;  /* rays a wavefront takes from the queue at once: it refills
-                           when this many of its 64 lanes are idle (1..64) */
+                           when this many of its 64 lanes are idle (1..63); 64
+                           takes whole 8x8 tiles, costliest first */
     int mathMode;       /* 0 exact IEEE arithmetic, 1 hardware fast intrinsics */
-    int diskCoop;       /* 1: disk chords sampled cooperatively by the wavefront;
-                           2 (default): cooperatively when a whole-tile frame has
-                           no more 8x8 tiles than the tile grid has wavefronts */
+    int diskCoop;       /* 0: each lane samples its own ray's disk chords;
+                           1: chords sampled cooperatively by the wavefront;
+                           2: cooperatively when a whole-tile frame has no
+                           more 8x8 tiles than the tile grid has wavefronts */
 } BhSchedule;
+/* Process default: {64, 4096, 64, 1, 2} (one-wavefront workgroups, whole
+ * tiles, fast math, automatic cooperative sampling). */
 void bhGetSchedule(BhSchedule* s);
 bh_error_t bhSetSchedule(const BhSchedule* s);   /* -1 (hipErrorInvalidValue-like) on bad values */
 /* Times the trace kernel of `p` under 18 candidate schedules (workgroup
