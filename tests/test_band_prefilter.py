@@ -2,7 +2,9 @@
 the FP64 evaluation far from the band (csrc/bh_band.h bandStatsF32).
 
 A float32 restatement of its arithmetic (camera terms, invariants, 32-point
-scan, golden-section search, polar turning point) is compared with the
+scan, 10-step golden-section search, polar turning point; the potential's
+fused multiply-adds rounded once, from exact float64 products of the float32
+operands) is compared with the
 oracle's FP64 delta and sin^2 theta_min on Kerr-family views with a horizon.
 The classification trusts the estimate only farther than 0.01 (delta) and
 1e-3 (polar) from the band's edges; its error must stay an order of
@@ -69,11 +71,19 @@ def estimate32(P, lo, hi):
     xa = xi - a
     etx, axi, a2Q2 = eta + xa * xa, a * xi, a * a + Q * Q
 
+    a2s, twoM = a * a, F(2) * M
+
+    def fma(x, y, z):
+        # one rounding: the float64 product of two float32 values is exact
+        x, y, z = np.asarray(x, F), np.asarray(y, F), np.asarray(z, F)
+        return (x.astype(np.float64) * y.astype(np.float64) + z.astype(np.float64)).astype(F)
+
     def pot(r):
-        ra = r * r + a * a
+        r = np.asarray(r, F)
+        ra = fma(r, r, a2s)
         P_ = ra - axi
-        Del = r * r - F(2) * M * r + a2Q2
-        return (P_ * P_ - Del * etx) * (F(1) / (ra * ra))
+        Del = fma(r, r - twoM, a2Q2)
+        return fma(P_, P_, -(Del * etx)) * (F(1) / (ra * ra))
 
     lo, hi = F(lo), F(hi)
     dr = (hi - lo) * (F(1) / F(31))
@@ -88,7 +98,7 @@ def estimate32(P, lo, hi):
     gr = F(0.618034)
     x1, x2 = x3 - gr * (x3 - x0), x0 + gr * (x3 - x0)
     f1, f2 = pot(x1), pot(x2)
-    for _ in range(24):
+    for _ in range(10):
         m = f1 < f2
         a3, a2, af2 = x2, x1, f1
         a1 = a3 - gr * (a3 - x0)
